@@ -1,0 +1,178 @@
+"""Benchmark: Heston QE-M path-steps/s on one MI355X, FP32 paths against the FP64 mode of the same kernel.
+
+Workload (one "step" = one pass of the hot path over one batch): a strip of 21 European calls (K = 70..130 step 3)
+at 8 maturities (0.25..2 y) on 2^20 Heston paths (Andersen QE-M, psi_c 1.5; S0 100, v0 .04, kappa 1.5, theta .04,
+xi .6, rho -.7, r .03, q 0) of 504 uniform steps to T = 2 y, FP32 path state with compensated FP64 payoff sums.
+Inputs are parameters (nothing to stage in HBM); the outputs are 9 x 22 estimates.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--fp64-steps 3] [--no-cpu]
+
+Prints one JSON line: value = FP32 path-steps/s of the whole call (wall clock around the timed steps), the FP64
+figure and the FP32/FP64 ratio beside it, the roofline of the strip kernel (its average duration from HIP events
+recorded around it on its stream, by the library) and the CPU oracle timed on a bounded sample on this host."""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "option-pricing-lab_amd"))
+
+METRIC = "path-steps/s Heston QE FP32 vs FP64; AAD-vs-bump Greeks speedup at ≤4 SE"
+N_PATHS = 1 << 20
+N_STEPS = 504
+T = 2.0
+STRIKES = [70.0 + 3.0 * k for k in range(21)]
+MATS = [0.25 * (j + 1) for j in range(8)]
+# Algorithmic VALU lane-slots per path-step of FP32 QE-M (one leg, one step): 10 special functions (div, sqrt, log,
+# erfc) at 4 slots, 28 other FP32 ops, and the amortised normals -- a quarter of a Philox4x32-10 block (40 32-bit
+# multiplies at 4 slots + 60 integer ops) and half a Box-Muller (3 specials + 10 ops). 40 + 28 + 55 + 11 = 134.
+SLOTS_PER_PATH_STEP = 134
+# MI355X: 256 CUs x 64 lanes x 2.4 GHz = 3.93e13 unpacked VALU lane-slots/s, i.e. 78.6 TFLOP/s of FMA (the FP64
+# vector peak and the unpacked FP32 peak; packed FP32 is not used).
+LANE_SLOTS_PER_S = 256 * 64 * 2.4e9
+PEAK_TFLOPS = 78.6
+
+
+def _request(backend_model, n_paths):
+    from optlab.core.market import Market
+    from optlab.core.models import HestonModel, HestonParams
+    from optlab.native import gpu
+
+    market = Market.flat(100.0, 0.03, 0.0)
+    model = HestonModel(HestonParams(0.04, 1.5, 0.04, 0.6, -0.7))
+    steps = [round(m / T * N_STEPS) for m in MATS]
+    return gpu.strip_request(model, market, STRIKES, steps, N_STEPS, T, n_paths, 42)
+
+
+def _time_gpu(lib, backend: int, steps: int, warmup: int):
+    import torch
+
+    rq, out = _request(backend, N_PATHS)
+    for _ in range(warmup):
+        assert lib.optlab_strip(C.byref(rq), backend) == 0, lib.optlab_last_error()
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        assert lib.optlab_strip(C.byref(rq), backend) == 0, lib.optlab_last_error()
+        kernel_ms.append(rq.kernel_ms)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall / steps * 1e3, sum(kernel_ms) / len(kernel_ms), out["price"].copy(), out["se"].copy()
+
+
+def _time_cpu(threads: int):
+    """The oracle (FP64, the reference CPU engine's arithmetic) on a bounded sample: the same strip on 2^18 paths."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from _support import oracle
+
+    n = 1 << 18
+    rq, _ = _request(0, n)
+    lib = oracle()
+    assert lib.oracle_strip(C.byref(rq), threads) == 0  # warm-up (page faults, thread start)
+    reps = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        assert lib.oracle_strip(C.byref(rq), threads) == 0
+        reps.append(time.perf_counter() - t0)
+    reps.sort()
+    return n * N_STEPS / reps[1], n
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--fp64-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this engine has no multi-GPU path (the reference has none either); "
+              "the per-GPU throughput is the figure, at --gpus 1", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from optlab.native import _abi
+
+    lib = _abi.load()
+    if lib.optlab_gpu_count() < 1 or not torch.cuda.is_available():
+        print("no HIP device", file=sys.stderr)
+        return 1
+    torch.cuda.init()
+    ms32, kern32, p32, se32 = _time_gpu(lib, 1, args.steps, args.warmup)
+    ms64, kern64, p64, se64 = _time_gpu(lib, 0, max(1, args.fp64_steps), 1)
+    work = N_PATHS * N_STEPS
+    v32, v64 = work / (ms32 * 1e-3), work / (ms64 * 1e-3)
+    k = len(STRIKES)
+    gap = (p32[:, :k] - p64[:, :k]) / se64[:, :k]
+    achieved_slots = SLOTS_PER_PATH_STEP * work / (kern32 * 1e-3)
+    achieved = 2 * achieved_slots / 1e12  # one lane-slot = one FMA-equivalent = 2 flops
+    line = {
+        "metric": METRIC,
+        "value": v32,
+        "unit": "path-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "Heston QE-M strip: 21 calls x 8 maturities on 2^20 paths x 504 steps (T 2y), FP32 paths, "
+                        "compensated FP64 sums; FP64 mode of the same kernel timed beside it",
+            "paths": N_PATHS,
+            "time_steps": N_STEPS,
+            "strikes": k,
+            "maturities": len(MATS),
+            "psi_c": 1.5,
+        },
+        "fp64": {"value": v64, "ms_per_step": ms64, "kernel_ms": kern64, "steps": max(1, args.fp64_steps)},
+        "fp32_over_fp64": v32 / v64,
+        "parity": {
+            "max_abs_gap_fp32_vs_fp64": float(abs(p32[:, :k] - p64[:, :k]).max()),
+            "max_gap_in_se": float(abs(gap).max()),
+            "forward_rel_err_fp32": float(max(abs(p32[j, k] / (100 * math.exp(0.03 * MATS[j])) - 1) for j in range(8))),
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_TFLOPS,
+            "traffic": None,
+            "kernel_ms": kern32,
+            "slots_per_path_step": SLOTS_PER_PATH_STEP,
+            "roofline_path_steps_per_s": LANE_SLOTS_PER_S / SLOTS_PER_PATH_STEP,
+        },
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cpu_rate, n_cpu = _time_cpu(threads)
+        line["cpu_baseline"] = {
+            "value": cpu_rate,
+            "unit": "path-steps/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"the same QE-M strip in FP64 on {n_cpu} paths x {N_STEPS} steps (median of 3 after a warm-up), "
+                      f"oracle/oracle.cpp on {threads} threads",
+        }
+        line["gpu_over_cpu"] = v32 / cpu_rate
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

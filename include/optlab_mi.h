@@ -1,0 +1,196 @@
+/* optlab_mi.h -- C ABI of the MI355X Monte Carlo path engine (liboptlab_mi.so).
+ *
+ * The entry points replace the checked requests of the reference engine, `optlab::api` in
+ * packages/native/cuda/include/optlab_cuda/requests.hpp:95-110, which its nanobind module binds
+ * (packages/native/bindings/ext.cpp:57-178). The request structs keep the reference layouts field for
+ * field (requests.hpp:12-93) so a binding written for them fills these unchanged; the additions are new
+ * structs (strip, SLV particle run, adjoint, book), never new fields in the old ones.
+ *
+ * Conventions, all entry points:
+ *   - every function returns OPTLAB_OK (0) or an error code; optlab_last_error() gives the message of the
+ *     last failure on the calling thread (the reference throws std::invalid_argument / std::runtime_error,
+ *     requests.cpp:140-219, which ext.cpp:43-53 turns into EngineError);
+ *   - backend 0 is double precision: the numbers of the reference CPU engine (the same Philox draws and the
+ *     same operations in FP64), computed on the GPU; backend 1 is the float engine of the reference GPU
+ *     (FP32 path state, FP64 sums). Any other backend is OPTLAB_EINVAL;
+ *   - n_threads is accepted for ABI compatibility and ignored: all work runs on the current HIP device;
+ *   - buffers named "host" are caller-owned host memory, written in place; "_dev" entry points take
+ *     caller-owned device pointers and an optional hipStream_t passed as void* (NULL: the null stream);
+ *   - calls on one device are not thread-safe (requests.hpp:103).
+ */
+#ifndef OPTLAB_MI_H
+#define OPTLAB_MI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OPTLAB_MI_ABI_VERSION 1
+
+#if defined(__GNUC__) || defined(__clang__)
+#define OPTLAB_API __attribute__((visibility("default")))
+#else
+#define OPTLAB_API
+#endif
+
+enum {
+    OPTLAB_OK = 0,
+    OPTLAB_EINVAL = 1,   /* bad request: std::invalid_argument in the reference */
+    OPTLAB_EDEVICE = 2,  /* no device, or a HIP error: std::runtime_error in the reference */
+    OPTLAB_ENOMEM = 3,   /* device allocation failed */
+    OPTLAB_EINTERNAL = 4
+};
+
+/* ---- fixed products: requests.hpp:12-29, unchanged layout -------------------------------------------------- */
+/* model: 0 Black-Scholes (exact log step), 1 Heston (Andersen QE-M). product: 0 European, 1 Asian, 2 down-and-out
+   barrier (0 < H < S0), 3 up-and-out barrier (H > S0; an addition: the reference refuses product 3).
+   legs: 0 none, 1 call, 2 put, 3 forward. Outputs (params.hpp:27-32): European out[k] = leg k; Asian out[0]
+   arithmetic, out[1] geometric, out[2] the arithmetic with the geometric control variate (an addition: the
+   reference leaves out[2] at 0; Black-Scholes only, else 0); barrier out[0] knocked-out leg 0, out[1] same leg
+   without barrier; out[3] = E[S_T] undiscounted. Pair p reads the normals of Philox stream 0 of pair p. */
+typedef struct {
+    int model, product;
+    int legs[3];
+    double strikes[3];
+    double H;
+    int n_steps;
+    double S0, r, q, T;
+    double sigma;
+    double v0, kappa, theta, xi, rho;
+    long long n_paths;
+    unsigned long long seed;
+} optlab_request;
+
+typedef struct {
+    double price[4], se[4];
+    long long n_paths;
+    double kernel_ms, total_ms;
+} optlab_result;
+
+/* ---- path engine: requests.hpp:32-63, unchanged layout ----------------------------------------------------- */
+/* Every model of optlab (0 constant variance, 1 tabulated local variance, 2 Heston, 3 SLV = table of squared
+   leverage times v+), 1 to 8 correlated names, optional local correlation. Arrays in double, row major; out has
+   n_names * n_cols columns of 2 n_pairs rows, column (a, c) at out + (a * n_cols + c) * 2 n_pairs. */
+typedef struct {
+    int n_names, n_steps, n_obs, n_x, n_rows;
+    long long n_pairs, first_pair;
+    unsigned long long seed;
+    const double* dt;
+    const double* t0;
+    const int* obs_step;
+    const double* root;
+    const int* kind;
+    const double* cvar;
+    const double* heston;
+    int same;
+    const int* step_row;
+    const double* row_lo;
+    const double* row_inv_dx;
+    const double* table;
+    const double* fwd;
+    int mixer;
+    double mix[8];
+    const double* low_root;
+    int n_cols, n_keep;
+    const int* col_of;
+    int n_windows;
+    const int* win_lo;
+    const int* win_hi;
+    double* out;
+    double kernel_ms, total_ms;
+} optlab_paths_request;
+
+/* requests.hpp:66-71, unchanged layout (sm_count = compute units, major/minor = gfx major/minor). */
+typedef struct {
+    char name[256];
+    int major, minor, sm_count, clock_mhz;
+    long long total_mem, free_mem;
+    int integrated, kernel_timeout;
+} optlab_device;
+
+/* ---- SLV leverage by particles: requests.hpp:74-93, unchanged layout -------------------------------------- */
+typedef struct {
+    int n_steps;
+    const double* dt;
+    const double* t0;
+    long long n_pairs;
+    unsigned long long seed;
+    double v0, kappa, theta, xi, rho;
+    int n_x, n_bins, n_fine;
+    double atm_vol, bandwidth_scale, l_min, l_max, q_lo, q_hi;
+    int local_linear, positive_part;
+    int n_lv;
+    const double* lv_lo;
+    const double* lv_inv_dx;
+    const double* lv_rows;
+    double* x_grids;
+    double* values;
+    double total_ms;
+} optlab_leverage_request;
+
+/* ---- addition: a strip of vanillas on one simulation (Heston QE-M or Black-Scholes) ------------------------ */
+/* Calls (or puts) at n_strikes strikes and n_mats maturities t_j = T * mat_step[j] / n_steps, all on the same
+   paths; price[j * (n_strikes + 1) + k] is strike k at maturity j, discounted by exp(-r t_j), and column
+   n_strikes holds E[S_{t_j}] undiscounted (the martingale check). Sums of pair averages in compensated FP64. */
+typedef struct {
+    int model;              /* 0 Black-Scholes, 1 Heston QE-M */
+    int put;                /* 0 calls, 1 puts */
+    int n_steps;            /* uniform steps of T / n_steps */
+    double S0, r, q, T;
+    double sigma;
+    double v0, kappa, theta, xi, rho;
+    double psi_c;           /* QE switching level (Andersen: 1.5) */
+    int n_strikes;          /* 1 to 31 */
+    const double* strikes;  /* host, n_strikes */
+    int n_mats;             /* 1 to 16 */
+    const int* mat_step;    /* host, n_mats, increasing in [1, n_steps] */
+    long long n_paths;      /* even, >= 4 */
+    unsigned long long seed;
+    double* price;          /* host, n_mats x (n_strikes + 1) */
+    double* se;             /* host, n_mats x (n_strikes + 1) */
+    double kernel_ms, total_ms;
+} optlab_strip_request;
+
+/* ---- entry points ------------------------------------------------------------------------------------------- */
+
+/* requests.hpp:97-101 */
+OPTLAB_API int optlab_gpu_count(void);
+OPTLAB_API int optlab_has_hip(void);                 /* has_cuda(): built with the HIP backend (always 1 for this library) */
+OPTLAB_API int optlab_cpu_threads(void);
+OPTLAB_API int optlab_gpu_name(char* buf, int len);  /* device 0 as "name (gfxNNN, N CU)", "" without a device */
+OPTLAB_API int optlab_device_info(int device, optlab_device* out);
+OPTLAB_API const char* optlab_last_error(void);
+OPTLAB_API int optlab_abi_version(void);
+
+/* requests.hpp:104 (api::price). */
+OPTLAB_API int optlab_price(const optlab_request* rq, int backend, int n_threads, optlab_result* out);
+
+/* requests.hpp:106 (api::paths): host arrays in, host out. */
+OPTLAB_API int optlab_paths(optlab_paths_request* rq, int backend, int n_threads);
+
+/* Addition: api::paths with the output left on the device (d_out: n_names * n_cols * 2 n_pairs doubles); rq->out
+   is not read. The specification arrays are still host memory. */
+OPTLAB_API int optlab_paths_dev(optlab_paths_request* rq, int backend, double* d_out, void* stream);
+
+/* requests.hpp:108 (api::leverage). */
+OPTLAB_API int optlab_leverage(optlab_leverage_request* rq, int backend, int n_threads);
+
+/* Addition: the strip of vanillas (no counterpart in the reference's fixed products, which price three legs at
+   one maturity: params.hpp:27-40). */
+OPTLAB_API int optlab_strip(optlab_strip_request* rq, int backend);
+
+/* Addition, for tests of the integer stream: the Philox4x32-10 words of counters (c0 + i, c1, c2, c3), i < n,
+   key (k0, k1), computed by the device function the kernels use; out_dev holds 4 n uint32 (device memory). */
+OPTLAB_API int optlab_philox_dev(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1, long long n,
+                      unsigned* out_dev, void* stream);
+
+/* Addition, for tests: the normals NormalStream (paths.hpp:213-233) gives pair p, stream tag, in the given
+   precision (backend 0 double, 1 float widened to double): out_dev holds n_pairs * n_normals doubles, pair-major. */
+OPTLAB_API int optlab_normals_dev(long long first_pair, long long n_pairs, int n_normals, unsigned tag, unsigned long long seed,
+                       int backend, double* out_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPTLAB_MI_H */

@@ -1,0 +1,139 @@
+// Device building blocks shared by the kernels of liboptlab_mi: Philox4x32-10, Box-Muller, the per-pair normal
+// streams, math in the path precision R (float or double), and wave64 reductions.
+//
+// Draws follow packages/native/cuda/include/optlab_cuda/philox.hpp:22-83 and paths.hpp:207-233: the integer stream
+// is bit-exact to the reference; the double-precision Box-Muller takes the reference CPU form cos/sin(2 pi v), the
+// float one computes sincospi(2 v) as the reference GPU did.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace optlab_mi {
+
+struct U4 {
+    std::uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox10(U4 c, std::uint32_t k0, std::uint32_t k1) {
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        const std::uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        const std::uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// ---- math in the path precision ---------------------------------------------------------------------------------
+__device__ __forceinline__ float r_exp(float x) { return expf(x); }
+__device__ __forceinline__ double r_exp(double x) { return exp(x); }
+__device__ __forceinline__ float r_log(float x) { return logf(x); }
+__device__ __forceinline__ double r_log(double x) { return log(x); }
+__device__ __forceinline__ float r_sqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double r_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float r_erfc(float x) { return erfcf(x); }
+__device__ __forceinline__ double r_erfc(double x) { return erfc(x); }
+template <typename R>
+__device__ __forceinline__ R r_max(R a, R b) { return a > b ? a : b; }
+template <typename R>
+__device__ __forceinline__ R r_min(R a, R b) { return a < b ? a : b; }
+
+// (i + 1/2) 2^-32 in (0, 1), in the precision R (philox.hpp:45-49).
+template <typename R>
+__device__ __forceinline__ R unit01(std::uint32_t i) {
+    return static_cast<R>(i) * static_cast<R>(2.3283064365386963e-10) + static_cast<R>(1.1641532182693481e-10);
+}
+
+__device__ __forceinline__ void gauss2(std::uint32_t a, std::uint32_t b, float& z0, float& z1) {
+    const float u = unit01<float>(a), v = unit01<float>(b);
+    const float radius = sqrtf(-2.0f * logf(u));
+    float s, c;
+    sincospif(2.0f * v, &s, &c);
+    z0 = radius * c;
+    z1 = radius * s;
+}
+
+__device__ __forceinline__ void gauss2(std::uint32_t a, std::uint32_t b, double& z0, double& z1) {
+    const double u = unit01<double>(a), v = unit01<double>(b);
+    const double radius = sqrt(-2.0 * log(u));
+    const double angle = 6.283185307179586 * v;
+    z0 = radius * cos(angle);
+    z1 = radius * sin(angle);
+}
+
+constexpr std::uint32_t kAuxTag = 0x80000000u;
+
+// Normals of one pair on one stream, four per Philox block (paths.hpp:213-233). The consumption pattern is the
+// same on every lane of a launch, so `used` and `block` stay wave-uniform.
+template <typename R>
+struct NormalStream {
+    std::uint32_t p0, p1, k0, k1, tag;
+    std::uint32_t block;
+    int used;
+    R buf[4];
+
+    __device__ __forceinline__ void init(std::uint64_t pair, std::uint32_t key0, std::uint32_t key1, std::uint32_t t) {
+        p0 = static_cast<std::uint32_t>(pair);
+        p1 = static_cast<std::uint32_t>(pair >> 32);
+        k0 = key0;
+        k1 = key1;
+        tag = t;
+        block = 0;
+        used = 4;
+    }
+    __device__ __forceinline__ R next() {
+        if (used == 4) {
+            const U4 r = philox10(U4{p0, p1, block, tag}, k0, k1);
+            gauss2(r.x, r.y, buf[0], buf[1]);
+            gauss2(r.z, r.w, buf[2], buf[3]);
+            ++block;
+            used = 0;
+        }
+        const int u = used++;
+        return u == 0 ? buf[0] : u == 1 ? buf[1] : u == 2 ? buf[2] : buf[3];
+    }
+};
+
+// ---- wave64 reductions -----------------------------------------------------------------------------------------
+
+__device__ __forceinline__ double shfl_xor_d(double v, int mask) { return __shfl_xor(v, mask, 64); }
+
+// Reduce-scatter of 16 doubles over the 64 lanes of a wave: on return, v[0] of lane l holds the wave total of
+// value (l & 15) (every group of 16 lanes holds the same totals). Two all-reduce levels (xor 32, 16) then four
+// halving levels (xor 8, 4, 2, 1): 47 shuffles and adds for 16 values instead of 96 for one all-reduce each.
+__device__ __forceinline__ void reduce_scatter16(double v[16], int lane) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] += shfl_xor_d(v[i], 32);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] += shfl_xor_d(v[i], 16);
+#pragma unroll
+    for (int half = 8; half >= 1; half >>= 1) {
+        const bool upper = (lane & half) != 0;
+#pragma unroll
+        for (int i = 0; i < half; ++i) {
+            const double keep = upper ? v[i + half] : v[i];
+            const double give = upper ? v[i] : v[i + half];
+            v[i] = keep + shfl_xor_d(give, half);
+        }
+    }
+}
+
+// Neumaier-compensated FP64 accumulator: s + c is the sum.
+struct Comp {
+    double s, c;
+    __device__ __forceinline__ void add(double x) {
+        const double t = s + x;
+        if (fabs(s) >= fabs(x))
+            c += (s - t) + x;
+        else
+            c += (x - t) + s;
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return s + c; }
+};
+
+}  // namespace optlab_mi

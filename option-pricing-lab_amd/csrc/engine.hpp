@@ -1,0 +1,51 @@
+// Host-side declarations shared by the translation units of liboptlab_mi (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/optlab_mi.h"
+
+namespace optlab_mi {
+
+// Errors cross the C boundary as status codes; inside the library they are exceptions.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& what) : std::runtime_error(what), code(c) {}
+};
+[[noreturn]] inline void invalid(const std::string& what) { throw Error(OPTLAB_EINVAL, what); }
+void hip_check(hipError_t e, const char* what);
+
+// Grow-only device buffers reused across calls (calls on one device are serialised by contract).
+void* scratch(int slot, size_t bytes);
+enum Slot { kSlotSpec = 0, kSlotOut, kSlotPartials, kSlotTotals, kSlotParticles, kSlotBins, kSlotAux, kSlotCount };
+
+int device_cu_count();
+
+// ---- path engine (paths.hip) ----------------------------------------------------------------------------------
+// Launch the path kernel for every pair of `rq` in precision prec (0 double, 1 float), output to d_out (device,
+// n_names x n_cols x 2 n_pairs doubles) on `stream`. Returns the kernel time in ms (HIP events).
+double launch_paths(const optlab_paths_request& rq, int prec, double* d_out, hipStream_t stream);
+void check_paths_request(const optlab_paths_request& rq);
+
+// ---- fixed products and strips (fixed.hip) ---------------------------------------------------------------------
+void check_price_request(const optlab_request& rq);
+void run_price(const optlab_request& rq, int prec, optlab_result& out);
+void check_strip_request(const optlab_strip_request& rq);
+void run_strip(optlab_strip_request& rq, int prec);
+
+// ---- SLV particles (leverage.hip) -----------------------------------------------------------------------------
+void check_leverage_request(const optlab_leverage_request& rq);
+void run_leverage(optlab_leverage_request& rq, int prec);
+
+// ---- test entry points (paths.hip) -----------------------------------------------------------------------------
+void launch_philox(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1, long long n,
+                   unsigned* out, hipStream_t stream);
+void launch_normals(long long first_pair, long long n_pairs, int n_normals, unsigned tag, unsigned long long seed,
+                    int prec, double* out, hipStream_t stream);
+
+}  // namespace optlab_mi

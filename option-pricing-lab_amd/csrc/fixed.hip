@@ -1,0 +1,516 @@
+// Fixed products on one underlying -- Black-Scholes exact log steps or Heston by Andersen's QE-M -- and strips of
+// vanillas, with every payoff summed on the device in compensated FP64.
+// Semantics: packages/native/cuda/include/optlab_cuda/schemes.hpp:159-311 (make_consts, qe_step, path units),
+// src/cpu_mc.cpp:14-18 (estimates on pair averages). Pair p reads the normals of Philox stream 0 of pair p
+// (paths.hpp:213-233), two per Heston step (Zv then Zx) and one per Black-Scholes step.
+//
+// MI355X layout: one antithetic pair per lane per iteration of a grid-stride loop, path state in VGPRs (FP32 or
+// FP64). At each payoff event the wave's pair averages (widened to FP64) and their squares are reduced across the
+// 64 lanes by reduce-scatters of 16 values, so that lane l ends up owning output slot l and keeps it in a
+// Neumaier-compensated FP64 register pair. Each wave writes its 64 slots once at the end; a second kernel adds the
+// waves in a fixed order. No atomics: results are deterministic for a given launch geometry.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device.hpp"
+#include "engine.hpp"
+
+namespace optlab_mi {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxStrikes = 31;
+constexpr int kMaxEvents = 16;
+
+template <typename R>
+struct Consts {  // schemes.hpp:159-207, computed in double on the host and cast once
+    R S0, D, bs_drift, bs_vol, ln_h, K[3];
+    int leg[3], n_steps;
+    R inv_n;
+    R v0, theta, mu_dt, E, c1, c2, K1, K2, K3, K4, A, K0p, psi_c;
+};
+
+struct HostConsts {
+    double S0, D, bs_drift, bs_vol, ln_h, K[3];
+    int leg[3], n_steps;
+    double inv_n, v0, theta, mu_dt, E, c1, c2, K1, K2, K3, K4, A, K0p, psi_c;
+    template <typename R>
+    Consts<R> as() const {
+        Consts<R> c{};
+        c.S0 = R(S0), c.D = R(D), c.bs_drift = R(bs_drift), c.bs_vol = R(bs_vol), c.ln_h = R(ln_h);
+        for (int k = 0; k < 3; ++k) c.K[k] = R(K[k]), c.leg[k] = leg[k];
+        c.n_steps = n_steps, c.inv_n = R(inv_n), c.v0 = R(v0), c.theta = R(theta), c.mu_dt = R(mu_dt), c.E = R(E);
+        c.c1 = R(c1), c.c2 = R(c2), c.K1 = R(K1), c.K2 = R(K2), c.K3 = R(K3), c.K4 = R(K4), c.A = R(A);
+        c.K0p = R(K0p), c.psi_c = R(psi_c);
+        return c;
+    }
+};
+
+HostConsts make_consts(int n_steps, double S0, double r, double q, double T, double sigma, double v0, double kappa,
+                       double theta, double xi, double rho, double H, const int* legs, const double* strikes,
+                       double psi_c) {
+    const double dt = T / n_steps;
+    HostConsts c{};
+    c.S0 = S0;
+    c.D = std::exp(-r * T);
+    c.bs_drift = (r - q - 0.5 * sigma * sigma) * dt;
+    c.bs_vol = sigma * std::sqrt(dt);
+    c.ln_h = H > 0 ? std::log(H / S0) : 0.0;
+    for (int k = 0; k < 3; ++k) c.K[k] = strikes ? strikes[k] : 0.0, c.leg[k] = legs ? legs[k] : 0;
+    c.n_steps = n_steps;
+    c.inv_n = 1.0 / n_steps;
+    const double E = std::exp(-kappa * dt), xi2 = xi * xi;
+    const double k12 = 0.5 * dt * (kappa * rho / xi - 0.5);
+    c.v0 = v0, c.theta = theta, c.mu_dt = (r - q) * dt, c.E = E;
+    c.c1 = xi2 * E * (1 - E) / kappa;
+    c.c2 = theta * xi2 * (1 - E) * (1 - E) / (2 * kappa);
+    c.K1 = k12 - rho / xi;
+    c.K2 = k12 + rho / xi;
+    c.K3 = 0.5 * dt * (1 - rho * rho);
+    c.K4 = c.K3;
+    c.A = (k12 + rho / xi) + 0.25 * dt * (1 - rho * rho);
+    c.K0p = -rho * kappa * theta * dt / xi;
+    c.psi_c = psi_c;
+    return c;
+}
+
+template <typename R>
+__device__ __forceinline__ R leg_payoff(int leg, R S, R K) {  // schemes.hpp:209-217
+    return leg == 1 ? r_max(S - K, R(0)) : leg == 2 ? r_max(K - S, R(0)) : leg == 3 ? S - K : R(0);
+}
+
+// schemes.hpp:219-252: one QE-M step of (y, v); the martingale correction's log is taken once after the branch.
+template <typename R>
+__device__ __forceinline__ void qe_step(const Consts<R>& c, R& y, R& v, R zv, R zx) {
+    const R m = c.theta + (v - c.theta) * c.E;
+    const R psi = (v * c.c1 + c.c2) / (m * m);
+    R vn, lin, w, arg;
+    bool finite;
+    if (psi <= c.psi_c) {
+        const R inv = R(2) / psi;
+        const R b2 = inv - R(1) + r_sqrt(inv) * r_sqrt(r_max(inv - R(1), R(0)));
+        const R a = m / (R(1) + b2), b = r_sqrt(b2);
+        vn = a * (b + zv) * (b + zv);
+        arg = R(1) - R(2) * c.A * a;
+        finite = c.A < R(1) / (R(2) * a);
+        lin = -c.A * b2 * a / arg;
+        w = R(0.5);
+    } else {
+        const R p = (psi - R(1)) / (psi + R(1)), beta = (R(1) - p) / m;
+        // 1 - U = Phi(-Zv) = erfc(Zv / sqrt 2) / 2, exact in both tails
+        const R one_minus_u = R(0.5) * r_erfc(zv * R(0.70710678118654752));
+        vn = one_minus_u >= R(1) - p ? R(0) : r_log((R(1) - p) / one_minus_u) / beta;
+        arg = p + beta * (R(1) - p) / (beta - c.A);
+        finite = c.A < beta;
+        lin = R(0);
+        w = R(-1);
+    }
+    const R K0 = finite ? lin + w * r_log(arg) - (c.K1 + R(0.5) * c.K3) * v : c.K0p;
+    y += c.mu_dt + K0 + c.K1 * v + c.K2 * vn + r_sqrt(r_max(c.K3 * v + c.K4 * vn, R(0))) * zx;
+    v = vn;
+}
+
+// schemes.hpp:254-311: European / Asian / down-and-out (+ up-and-out) pair averages; out[4] = out[0] out[1].
+template <typename R, int MODEL, int PRODUCT>
+__device__ __forceinline__ void fixed_unit(const Consts<R>& c, NormalStream<R>& src, R out[5]) {
+    R o[4] = {R(0), R(0), R(0), R(0)};
+    if (MODEL == 0 && PRODUCT == 0) {
+        const R z = src.next();
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const R S = c.S0 * r_exp(c.bs_drift + c.bs_vol * (a == 0 ? z : -z));
+#pragma unroll
+            for (int k = 0; k < 3; ++k) o[k] += c.D * leg_payoff(c.leg[k], S, c.K[k]);
+            o[3] += S;
+        }
+    } else {
+        R y[2] = {R(0), R(0)}, v[2] = {c.v0, c.v0}, sum_s[2] = {R(0), R(0)}, sum_y[2] = {R(0), R(0)};
+        bool alive[2] = {true, true};
+        for (int i = 0; i < c.n_steps; ++i) {
+            R zv = R(0), zx = R(0), z = R(0);
+            if (MODEL == 1) {
+                zv = src.next();
+                zx = src.next();
+            } else {
+                z = src.next();
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const R sg = a == 0 ? R(1) : R(-1);
+                if (MODEL == 1)
+                    qe_step(c, y[a], v[a], sg * zv, sg * zx);
+                else
+                    y[a] += c.bs_drift + c.bs_vol * sg * z;
+                if (PRODUCT == 1) {
+                    sum_s[a] += r_exp(y[a]);
+                    sum_y[a] += y[a];
+                }
+                if (PRODUCT == 2) alive[a] = alive[a] && !(y[a] <= c.ln_h);
+                if (PRODUCT == 3) alive[a] = alive[a] && !(y[a] >= c.ln_h);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const R S = c.S0 * r_exp(y[a]);
+            if (PRODUCT == 0) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) o[k] += c.D * leg_payoff(c.leg[k], S, c.K[k]);
+            } else if (PRODUCT == 1) {
+                o[0] += c.D * leg_payoff(c.leg[0], c.S0 * sum_s[a] * c.inv_n, c.K[0]);
+                o[1] += c.D * leg_payoff(c.leg[0], c.S0 * r_exp(sum_y[a] * c.inv_n), c.K[0]);
+            } else {
+                const R pay = c.D * leg_payoff(c.leg[0], S, c.K[0]);
+                o[0] += alive[a] ? pay : R(0);
+                o[1] += pay;
+            }
+            o[3] += S;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[k] = R(0.5) * o[k];
+    out[4] = out[0] * out[1];
+}
+
+// Slots of a fixed product: 0..4 the sums of out[0..4], 5..9 the sums of their squares (in FP64).
+template <typename R, int MODEL, int PRODUCT>
+__global__ void __launch_bounds__(kBlock) fixed_kernel(Consts<R> c, long long n_pairs, std::uint32_t k0,
+                                                        std::uint32_t k1, double* partials) {
+    const int lane = threadIdx.x & 63;
+    const long long wave = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    Comp acc{0.0, 0.0};
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long base = static_cast<long long>(blockIdx.x) * blockDim.x; base < n_pairs; base += stride) {
+        const long long p = base + threadIdx.x;
+        const bool active = p < n_pairs;
+        NormalStream<R> src;
+        src.init(static_cast<std::uint64_t>(active ? p : 0), k0, k1, 0u);
+        R out[5];
+        fixed_unit<R, MODEL, PRODUCT>(c, src, out);
+        double vals[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) vals[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const double yk = active ? static_cast<double>(out[k]) : 0.0;
+            vals[k] = yk;
+            vals[5 + k] = yk * yk;
+        }
+        reduce_scatter16(vals, lane);
+        const int v = lane & 15;
+        if (lane < 16 && v < 10) acc.add(vals[0]);
+    }
+    if (lane < 16) partials[wave * 16 + lane] = acc.value();
+}
+
+// The strip: events at mat_step[e], values k < n_strikes the discounted pair-average payoffs of strike k, value
+// n_strikes the pair average of S. Slots of event e: 0..31 sums of values, 32..63 sums of their squares.
+template <typename R>
+struct StripArgs {
+    Consts<R> c;
+    int model, put, n_strikes, n_mats;
+    R strikes[kMaxStrikes];
+    R disc[kMaxEvents];
+    int mat_step[kMaxEvents];
+};
+
+template <typename R>
+__device__ __forceinline__ R strip_value(const StripArgs<R>& s, int k, int e, R S0, R S1) {
+    if (k < s.n_strikes) {
+        const R K = s.strikes[k], D = s.disc[e];
+        const R o = s.put ? D * r_max(K - S0, R(0)) + D * r_max(K - S1, R(0))
+                          : D * r_max(S0 - K, R(0)) + D * r_max(S1 - K, R(0));
+        return R(0.5) * o;
+    }
+    return k == s.n_strikes ? R(0.5) * (S0 + S1) : R(0);
+}
+
+template <typename R, int MODEL>
+__global__ void __launch_bounds__(kBlock) strip_kernel(StripArgs<R> s, long long n_pairs, std::uint32_t k0,
+                                                        std::uint32_t k1, double* partials) {
+    const int lane = threadIdx.x & 63;
+    const long long wave = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const int n_values = s.n_strikes + 1;
+    Comp acc[kMaxEvents];
+#pragma unroll
+    for (int e = 0; e < kMaxEvents; ++e) acc[e] = Comp{0.0, 0.0};
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long base = static_cast<long long>(blockIdx.x) * blockDim.x; base < n_pairs; base += stride) {
+        const long long p = base + threadIdx.x;
+        const bool active = p < n_pairs;
+        NormalStream<R> src;
+        src.init(static_cast<std::uint64_t>(active ? p : 0), k0, k1, 0u);
+        R y0 = R(0), y1 = R(0), v0 = s.c.v0, v1 = s.c.v0;
+        int e = 0;
+        for (int i = 1; i <= s.c.n_steps; ++i) {
+            if (MODEL == 1) {
+                const R zv = src.next(), zx = src.next();
+                qe_step(s.c, y0, v0, zv, zx);
+                qe_step(s.c, y1, v1, -zv, -zx);
+            } else {
+                const R z = src.next();
+                y0 += s.c.bs_drift + s.c.bs_vol * z;
+                y1 += s.c.bs_drift + s.c.bs_vol * -z;
+            }
+            if (e < s.n_mats && s.mat_step[e] == i) {
+                const R S0 = s.c.S0 * r_exp(y0), S1 = s.c.S0 * r_exp(y1);
+#pragma unroll
+                for (int chunk = 0; chunk < 4; ++chunk) {
+                    const int first = (chunk & 1) * 16;
+                    if (first >= n_values) continue;
+                    double vals[16];
+#pragma unroll
+                    for (int i2 = 0; i2 < 16; ++i2) {
+                        const double yk = active ? static_cast<double>(strip_value(s, first + i2, e, S0, S1)) : 0.0;
+                        vals[i2] = chunk < 2 ? yk : yk * yk;
+                    }
+                    reduce_scatter16(vals, lane);
+                    if ((lane >> 4) == chunk) {
+#pragma unroll
+                        for (int ee = 0; ee < kMaxEvents; ++ee)
+                            if (ee == e) acc[ee].add(vals[0]);
+                    }
+                }
+                ++e;
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < kMaxEvents; ++e)
+        if (e < s.n_mats) partials[(wave * s.n_mats + e) * 64 + lane] = acc[e].value();
+}
+
+// totals[slot] = sum over waves of partials[w * n_slots + slot], in a fixed order: thread t of block `slot` adds
+// waves t, t + 256, ..., then a tree in LDS.
+__global__ void __launch_bounds__(256) reduce_kernel(const double* partials, long long n_waves, int n_slots,
+                                                      double* totals) {
+    const int slot = blockIdx.x;
+    Comp acc{0.0, 0.0};
+    for (long long w = threadIdx.x; w < n_waves; w += blockDim.x) acc.add(partials[w * n_slots + slot]);
+    __shared__ double hi[256], lo[256];
+    hi[threadIdx.x] = acc.s;
+    lo[threadIdx.x] = acc.c;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (static_cast<int>(threadIdx.x) < h) {
+            Comp a{hi[threadIdx.x], lo[threadIdx.x]};
+            a.add(hi[threadIdx.x + h]);
+            a.c += lo[threadIdx.x + h];
+            hi[threadIdx.x] = a.s;
+            lo[threadIdx.x] = a.c;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) totals[slot] = hi[0] + lo[0];
+}
+
+void estimate(double sum, double sumsq, long long n_pairs, double& price, double& se) {  // cpu_mc.cpp:14-18
+    const double n = static_cast<double>(n_pairs), mean = sum / n;
+    const double var = sumsq / n - mean * mean;
+    price = mean;
+    se = n_pairs > 1 ? std::sqrt(std::fmax(var, 0.0) / (n - 1.0)) : 0.0;
+}
+
+double norm_cdf(double x) { return 0.5 * std::erfc(-x / std::sqrt(2.0)); }
+
+// Geometric Asian on n equally spaced fixings (the arithmetic Asian's control variate): ln G is Gaussian with mean
+// ln S + (r - q - sigma^2 / 2) mean(t_i) and variance sigma^2 mean_ij min(t_i, t_j).
+double geometric_asian(double S, double K, double T, double r, double q, double sigma, int n, bool call) {
+    double mean_t = 0.0, mean_min = 0.0;
+    for (int i = 1; i <= n; ++i) {
+        mean_t += i * T / n;
+        for (int j = 1; j <= n; ++j) mean_min += (i < j ? i : j) * T / n;
+    }
+    mean_t /= n;
+    mean_min /= double(n) * n;
+    const double mu = std::log(S) + (r - q - 0.5 * sigma * sigma) * mean_t, sd = sigma * std::sqrt(mean_min);
+    const double d2 = (mu - std::log(K)) / sd, d1 = d2 + sd, eg = std::exp(mu + 0.5 * sd * sd), D = std::exp(-r * T);
+    return call ? D * (eg * norm_cdf(d1) - K * norm_cdf(d2)) : D * (K * norm_cdf(-d2) - eg * norm_cdf(-d1));
+}
+
+unsigned grid_for(long long n_pairs) {
+    const long long want = (n_pairs + kBlock - 1) / kBlock;
+    const long long cap = static_cast<long long>(device_cu_count()) * 16;
+    return static_cast<unsigned>(std::max<long long>(1, std::min(want, cap)));
+}
+
+// HIP events around the main kernel of a call (mark() after it); stop() waits for the stream and returns the
+// main kernel's time in ms.
+struct Timer {
+    hipEvent_t e0{}, e1{};
+    hipStream_t st;
+    bool marked = false;
+    explicit Timer(hipStream_t s) : st(s) {
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+        hip_check(hipEventRecord(e0, st), "event record");
+    }
+    void mark() {
+        hip_check(hipEventRecord(e1, st), "event record");
+        marked = true;
+    }
+    double stop() {
+        if (!marked) hip_check(hipEventRecord(e1, st), "event record");
+        hip_check(hipStreamSynchronize(st), "kernel");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
+        return ms;
+    }
+    ~Timer() {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+};
+
+template <typename R>
+void launch_fixed(const HostConsts& hc, int model, int product, long long n_pairs, std::uint64_t seed, unsigned grid,
+                  double* partials) {
+    const Consts<R> c = hc.as<R>();
+    const std::uint32_t k0 = static_cast<std::uint32_t>(seed), k1 = static_cast<std::uint32_t>(seed >> 32);
+#define OPTLAB_FIXED(M, P) \
+    hipLaunchKernelGGL((fixed_kernel<R, M, P>), dim3(grid), dim3(kBlock), 0, nullptr, c, n_pairs, k0, k1, partials)
+    if (model == 0) {
+        if (product == 0) OPTLAB_FIXED(0, 0);
+        else if (product == 1) OPTLAB_FIXED(0, 1);
+        else if (product == 2) OPTLAB_FIXED(0, 2);
+        else OPTLAB_FIXED(0, 3);
+    } else {
+        if (product == 0) OPTLAB_FIXED(1, 0);
+        else if (product == 1) OPTLAB_FIXED(1, 1);
+        else if (product == 2) OPTLAB_FIXED(1, 2);
+        else OPTLAB_FIXED(1, 3);
+    }
+#undef OPTLAB_FIXED
+    hip_check(hipGetLastError(), "fixed kernel launch");
+}
+
+template <typename R>
+void launch_strip(const StripArgs<R>& s, long long n_pairs, std::uint64_t seed, unsigned grid, double* partials) {
+    const std::uint32_t k0 = static_cast<std::uint32_t>(seed), k1 = static_cast<std::uint32_t>(seed >> 32);
+    if (s.model == 1)
+        hipLaunchKernelGGL((strip_kernel<R, 1>), dim3(grid), dim3(kBlock), 0, nullptr, s, n_pairs, k0, k1, partials);
+    else
+        hipLaunchKernelGGL((strip_kernel<R, 0>), dim3(grid), dim3(kBlock), 0, nullptr, s, n_pairs, k0, k1, partials);
+    hip_check(hipGetLastError(), "strip kernel launch");
+}
+
+template <typename R>
+StripArgs<R> strip_args(const optlab_strip_request& rq) {
+    StripArgs<R> s{};
+    s.c = make_consts(rq.n_steps, rq.S0, rq.r, rq.q, rq.T, rq.sigma, rq.v0, rq.kappa, rq.theta, rq.xi, rq.rho, 0.0,
+                      nullptr, nullptr, rq.psi_c)
+              .template as<R>();
+    s.model = rq.model, s.put = rq.put, s.n_strikes = rq.n_strikes, s.n_mats = rq.n_mats;
+    for (int k = 0; k < rq.n_strikes; ++k) s.strikes[k] = static_cast<R>(rq.strikes[k]);
+    for (int e = 0; e < rq.n_mats; ++e) {
+        s.mat_step[e] = rq.mat_step[e];
+        s.disc[e] = static_cast<R>(std::exp(-rq.r * (rq.T * rq.mat_step[e] / rq.n_steps)));
+    }
+    return s;
+}
+
+}  // namespace
+
+void check_price_request(const optlab_request& c) {  // requests.cpp:140-166, with product 3 added
+    if (c.model < 0 || c.model > 1) invalid("model must be 0 (Black-Scholes) or 1 (Heston)");
+    if (c.product < 0 || c.product > 3) invalid("product must be 0, 1, 2 or 3");
+    if (c.n_steps < 1) invalid("n_steps must be >= 1");
+    if (c.n_paths < 4 || c.n_paths % 2) invalid("n_paths must be even and >= 4");
+    if (!(c.S0 > 0 && c.T > 0)) invalid("S0 and T must be positive");
+    if (c.model == 0 && !(c.sigma > 0)) invalid("sigma must be positive");
+    if (c.model == 1 && !(c.v0 >= 0 && c.kappa > 0 && c.theta > 0 && c.xi > 0 && std::fabs(c.rho) <= 1))
+        invalid("Heston parameters out of range");
+    if (c.product == 2 && !(c.H > 0 && c.H < c.S0)) invalid("barrier: need 0 < H < S0");
+    if (c.product == 3 && !(c.H > c.S0)) invalid("up-and-out barrier: need H > S0");
+    for (int k = 0; k < 3; ++k)
+        if (c.legs[k] < 0 || c.legs[k] > 3) invalid("legs must be 0 to 3");
+}
+
+void run_price(const optlab_request& c, int prec, optlab_result& out) {
+    const auto t_start = std::chrono::steady_clock::now();
+    const HostConsts hc = make_consts(c.n_steps, c.S0, c.r, c.q, c.T, c.sigma, c.v0, c.kappa, c.theta, c.xi, c.rho,
+                                      c.H, c.legs, c.strikes, 1.5);
+    const long long n_pairs = c.n_paths / 2;
+    const unsigned grid = grid_for(n_pairs);
+    const long long n_waves = static_cast<long long>(grid) * (kBlock / 64);
+    double* partials = static_cast<double*>(scratch(kSlotPartials, static_cast<size_t>(n_waves) * 16 * sizeof(double)));
+    double* totals = static_cast<double*>(scratch(kSlotTotals, 16 * sizeof(double)));
+    Timer timer(nullptr);
+    if (prec == 0)
+        launch_fixed<double>(hc, c.model, c.product, n_pairs, c.seed, grid, partials);
+    else
+        launch_fixed<float>(hc, c.model, c.product, n_pairs, c.seed, grid, partials);
+    timer.mark();
+    hipLaunchKernelGGL(reduce_kernel, dim3(16), dim3(256), 0, nullptr, partials, n_waves, 16, totals);
+    hip_check(hipGetLastError(), "reduce kernel launch");
+    const double kernel_ms = timer.stop();
+    double t[16];
+    hip_check(hipMemcpy(t, totals, sizeof t, hipMemcpyDeviceToHost), "copy totals");
+    std::memset(&out, 0, sizeof out);
+    for (int k = 0; k < 4; ++k) estimate(t[k], t[5 + k], n_pairs, out.price[k], out.se[k]);
+    if (c.product == 1) {
+        out.price[2] = out.se[2] = 0.0;
+        if (c.model == 0 && (c.legs[0] == 1 || c.legs[0] == 2)) {
+            const double n = static_cast<double>(n_pairs);
+            const double ma = t[0] / n, mg = t[1] / n;
+            const double va = t[5] / n - ma * ma, vg = t[6] / n - mg * mg;
+            const double cov = t[4] / n - ma * mg;
+            const double beta = vg > 0 ? cov / vg : 0.0;
+            const double exact = geometric_asian(c.S0, c.strikes[0], c.T, c.r, c.q, c.sigma, c.n_steps, c.legs[0] == 1);
+            out.price[2] = ma - beta * (mg - exact);
+            const double vr = va - 2.0 * beta * cov + beta * beta * vg;
+            out.se[2] = std::sqrt(std::fmax(vr, 0.0) / (n - 1.0));
+        }
+    }
+    out.n_paths = c.n_paths;
+    out.kernel_ms = kernel_ms;
+    out.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+}
+
+void check_strip_request(const optlab_strip_request& c) {
+    if (c.model < 0 || c.model > 1) invalid("model must be 0 (Black-Scholes) or 1 (Heston)");
+    if (c.n_steps < 1) invalid("n_steps must be >= 1");
+    if (c.n_paths < 4 || c.n_paths % 2) invalid("n_paths must be even and >= 4");
+    if (!(c.S0 > 0 && c.T > 0)) invalid("S0 and T must be positive");
+    if (c.model == 0 && !(c.sigma > 0)) invalid("sigma must be positive");
+    if (c.model == 1 && !(c.v0 >= 0 && c.kappa > 0 && c.theta > 0 && c.xi > 0 && std::fabs(c.rho) <= 1))
+        invalid("Heston parameters out of range");
+    if (c.n_strikes < 1 || c.n_strikes > kMaxStrikes || !c.strikes) invalid("n_strikes must be 1 to 31");
+    if (c.n_mats < 1 || c.n_mats > kMaxEvents || !c.mat_step) invalid("n_mats must be 1 to 16");
+    for (int j = 0; j < c.n_mats; ++j)
+        if (c.mat_step[j] < 1 || c.mat_step[j] > c.n_steps || (j && c.mat_step[j] <= c.mat_step[j - 1]))
+            invalid("mat_step must increase within [1, n_steps]");
+    if (!(c.psi_c >= 1.0 && c.psi_c <= 2.0)) invalid("psi_c must be in [1, 2]");
+    if (!c.price || !c.se) invalid("null output");
+}
+
+void run_strip(optlab_strip_request& rq, int prec) {
+    const auto t_start = std::chrono::steady_clock::now();
+    const long long n_pairs = rq.n_paths / 2;
+    const unsigned grid = grid_for(n_pairs);
+    const long long n_waves = static_cast<long long>(grid) * (kBlock / 64);
+    const int n_slots = rq.n_mats * 64;
+    double* partials = static_cast<double*>(scratch(kSlotPartials, static_cast<size_t>(n_waves) * n_slots * sizeof(double)));
+    double* totals = static_cast<double*>(scratch(kSlotTotals, static_cast<size_t>(n_slots) * sizeof(double)));
+    Timer timer(nullptr);
+    if (prec == 0)
+        launch_strip(strip_args<double>(rq), n_pairs, rq.seed, grid, partials);
+    else
+        launch_strip(strip_args<float>(rq), n_pairs, rq.seed, grid, partials);
+    timer.mark();
+    hipLaunchKernelGGL(reduce_kernel, dim3(n_slots), dim3(256), 0, nullptr, partials, n_waves, n_slots, totals);
+    hip_check(hipGetLastError(), "reduce kernel launch");
+    rq.kernel_ms = timer.stop();
+    std::vector<double> t(n_slots);
+    hip_check(hipMemcpy(t.data(), totals, n_slots * sizeof(double), hipMemcpyDeviceToHost), "copy totals");
+    const int width = rq.n_strikes + 1;
+    for (int e = 0; e < rq.n_mats; ++e)
+        for (int k = 0; k < width; ++k)
+            estimate(t[e * 64 + k], t[e * 64 + 32 + k], n_pairs, rq.price[e * width + k], rq.se[e * width + k]);
+    rq.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+}
+
+}  // namespace optlab_mi

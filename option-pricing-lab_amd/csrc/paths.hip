@@ -1,0 +1,404 @@
+// Path engine kernel: every model of optlab (constant variance, Dupire local variance table, Heston, SLV), 1 to 8
+// correlated names, optional local correlation, observations and running window extremes written as doubles.
+// Semantics: packages/native/cuda/include/optlab_cuda/paths.hpp:196-361 (simulate_pair, record, table_value).
+//
+// MI355X layout: one antithetic pair per lane, state (x, v, window extremes, the normal buffer) in VGPRs with every
+// per-name loop unrolled to the compile-time bound N so nothing spills; the table rows of the current step are
+// staged in LDS by the workgroup (double-buffered: the rows of step i+1 are copied while step i runs, one barrier
+// per step); output columns are structure-of-arrays, so a wave writes 64 consecutive doubles per column.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "device.hpp"
+#include "engine.hpp"
+
+namespace optlab_mi {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxNames = 8;
+constexpr size_t kStageBytes = 64 * 1024;  // LDS for two buffers of the step's table rows
+
+template <typename R>
+struct PathArgs {
+    int n_names, n_steps, n_obs, n_x;
+    long long n_pairs, first_pair, rows;
+    std::uint32_t key0, key1;
+    const R* dt;
+    const R* t0;
+    const int* obs_step;
+    const R* root;
+    const int* kind;
+    const R* cvar;
+    const R* hes;
+    int same;
+    const int* step_row;
+    const R* lo;
+    const R* inv_dx;
+    const R* table;
+    const R* fwd;
+    int mixer;
+    R mix[8];
+    const R* low_root;
+    int n_cols, n_keep;
+    const int* col_of;
+    int n_win;
+    int win_lo[2], win_hi[2];
+    double* out;
+    int n_tab;           // names reading a table
+    int tab_of[8];       // those names, in order
+    int tab_slot[8];     // slot of name a among them (-1: none)
+    int staged;          // 1: the step's rows are copied to LDS
+    int has_var, has_slv;
+};
+
+template <typename R>
+__device__ __forceinline__ void stage_rows(const PathArgs<R>& s, int step_index, R* buf) {
+    for (int t = 0; t < s.n_tab; ++t) {
+        const int a = s.tab_of[t];
+        const int row = s.step_row[a * s.n_steps + step_index];
+        const R* src = s.table + static_cast<long long>(row) * s.n_x;
+        R* dst = buf + t * s.n_x;
+        for (int c = threadIdx.x; c < s.n_x; c += blockDim.x) dst[c] = src[c];
+    }
+}
+
+// paths.hpp:196-205 on a row held at y.
+template <typename R>
+__device__ __forceinline__ R lerp_row(const R* y, R lo, R inv_dx, int n_x, R x) {
+    R u = (x - lo) * inv_dx;
+    const R top = static_cast<R>(n_x - 1) - static_cast<R>(1e-4);
+    u = r_min(r_max(u, R(0)), top);
+    const int i = static_cast<int>(u);
+    const R w = u - static_cast<R>(i);
+    return y[i] * (R(1) - w) + y[i + 1] * w;
+}
+
+template <typename R, int N>
+struct Extremes {
+    R lo[2][N][2], hi[2][N][2];
+};
+
+// paths.hpp:241-272.
+template <typename R, int N>
+__device__ __forceinline__ void record(const PathArgs<R>& s, long long p, bool active, int j, const R (&x)[2][N],
+                                       Extremes<R, N>& e) {
+    const long long row0 = p, row1 = s.n_pairs + p;
+    const int c = s.col_of[j];
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+        if (a >= s.n_names) break;
+        const R f = s.fwd[a * s.n_obs + j];
+        const R S0 = f * r_exp(x[0][a]), S1 = f * r_exp(x[1][a]);
+        double* base = s.out + static_cast<long long>(a) * s.n_cols * s.rows;
+        if (c >= 0 && active) {
+            base[c * s.rows + row0] = static_cast<double>(S0);
+            base[c * s.rows + row1] = static_cast<double>(S1);
+        }
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+            if (w >= s.n_win || j < s.win_lo[w] || j > s.win_hi[w]) continue;
+            if (j == s.win_lo[w]) {
+                e.lo[0][a][w] = e.hi[0][a][w] = S0;
+                e.lo[1][a][w] = e.hi[1][a][w] = S1;
+            } else {
+                e.lo[0][a][w] = r_min(e.lo[0][a][w], S0);
+                e.hi[0][a][w] = r_max(e.hi[0][a][w], S0);
+                e.lo[1][a][w] = r_min(e.lo[1][a][w], S1);
+                e.hi[1][a][w] = r_max(e.hi[1][a][w], S1);
+            }
+            if (j == s.win_hi[w] && active) {
+                const long long cl = s.n_keep + 2 * w, ch = cl + 1;
+                base[cl * s.rows + row0] = static_cast<double>(e.lo[0][a][w]);
+                base[ch * s.rows + row0] = static_cast<double>(e.hi[0][a][w]);
+                base[cl * s.rows + row1] = static_cast<double>(e.lo[1][a][w]);
+                base[ch * s.rows + row1] = static_cast<double>(e.hi[1][a][w]);
+            }
+        }
+    }
+}
+
+// paths.hpp:274-361: both legs of pair p through every step.
+template <typename R, int N>
+__global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
+    extern __shared__ unsigned char lds_raw[];
+    R* lds = reinterpret_cast<R*>(lds_raw);
+    const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool active = p < s.n_pairs;
+    const int n = s.n_names;
+    const std::uint64_t pair = static_cast<std::uint64_t>(s.first_pair + (active ? p : 0));
+
+    R x[2][N], v[2][N];
+    int kind[N];
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+        kind[a] = a < n ? s.kind[a] : 0;
+        const bool hv = kind[a] == 2 || kind[a] == 3;
+        x[0][a] = x[1][a] = R(0);
+        v[0][a] = v[1][a] = (a < n && hv) ? s.hes[a * 5] : R(0);
+    }
+    Extremes<R, N> ext;
+    int j = 0;
+    while (j < s.n_obs && s.obs_step[j] == 0) record<R, N>(s, p, active, j++, x, ext);
+
+    NormalStream<R> rng, aux;
+    rng.init(pair, s.key0, s.key1, 0u);
+    aux.init(pair, s.key0, s.key1, kAuxTag);
+    const int stage_len = s.n_tab * s.n_x;
+    if (s.staged && s.n_steps > 0) stage_rows(s, 0, lds);
+
+    for (int i = 1; i <= s.n_steps; ++i) {
+        const R* cur = lds + ((i - 1) & 1) * stage_len;
+        if (s.staged) {
+            __syncthreads();
+            if (i < s.n_steps) stage_rows(s, i, lds + (i & 1) * stage_len);
+        }
+        const R dt = s.dt[i - 1], t0 = s.t0[i - 1];
+        R zs[N], zr[N], fresh = R(0);
+#pragma unroll
+        for (int a = 0; a < N; ++a)
+            if (a < n) zs[a] = rng.next();
+        if (s.has_var) {
+#pragma unroll
+            for (int a = 0; a < N; ++a)
+                if (a < n) zr[a] = s.has_slv ? aux.next() : rng.next();
+        }
+        if (s.mixer) fresh = rng.next();
+#pragma unroll
+        for (int leg = 0; leg < 2; ++leg) {
+            const R sign = leg ? R(-1) : R(1);
+            R z[N], w[N];
+            if (s.mixer) {
+                R m = R(0);
+#pragma unroll
+                for (int a = 0; a < N; ++a)
+                    if (a < n) m += r_exp(x[leg][a]);
+                const R y = r_log(m / static_cast<R>(n));
+                const R u = y / (s.mix[4] * r_sqrt(r_max(t0, R(1.0 / 12))));
+                R wt = s.mix[0] + s.mix[1] + s.mix[2] * r_max(-u, R(0)) - s.mix[3] * r_max(u, R(0));
+                wt = r_min(r_max(wt, R(0)), R(1));
+                R zlast = zs[0];
+#pragma unroll
+                for (int a = 0; a < N; ++a)
+                    if (a == n - 1) zlast = zs[a];
+                const R market = s.mix[5] * sign * zlast, fr = sign * fresh;
+                const R common = s.mix[6] * market + s.mix[7] * fr;
+                R other[N];
+#pragma unroll
+                for (int a = 0; a < N; ++a) other[a] = (a == n - 1) ? s.mix[7] * market - s.mix[6] * fr : sign * zs[a];
+                const R sw = r_sqrt(wt), sl = r_sqrt(R(1) - wt);
+#pragma unroll
+                for (int a = 0; a < N; ++a) {
+                    if (a >= n) break;
+                    R acc = R(0);
+#pragma unroll
+                    for (int b = 0; b < N; ++b)
+                        if (b < n) acc += s.low_root[a * n + b] * other[b];
+                    z[a] = sw * common + sl * acc;
+                }
+            } else {
+#pragma unroll
+                for (int a = 0; a < N; ++a) {
+                    if (a >= n) break;
+                    R acc = R(0);
+#pragma unroll
+                    for (int b = 0; b < N; ++b)
+                        if (b < n) acc += s.root[a * n + b] * zs[b];
+                    z[a] = sign * acc;
+                }
+            }
+            if (s.has_var) {
+#pragma unroll
+                for (int a = 0; a < N; ++a) {
+                    if (a >= n) break;
+                    if (s.same) {
+                        R acc = R(0);
+#pragma unroll
+                        for (int b = 0; b < N; ++b)
+                            if (b < n) acc += s.root[a * n + b] * zr[b];
+                        w[a] = sign * acc;
+                    } else {
+                        w[a] = sign * zr[a];
+                    }
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < N; ++a) {
+                if (a >= n) break;
+                const int k = kind[a];
+                R var;
+                const R vp = r_max(v[leg][a], R(0));
+                if (k == 0) {
+                    var = s.cvar[a];
+                } else if (k == 2) {
+                    var = vp;
+                } else {
+                    const int row = s.step_row[a * s.n_steps + (i - 1)];
+                    const R* y = s.staged ? cur + s.tab_slot[a] * s.n_x : s.table + static_cast<long long>(row) * s.n_x;
+                    const R t = lerp_row(y, s.lo[row], s.inv_dx[row], s.n_x, x[leg][a]);
+                    var = k == 1 ? t : t * vp;
+                }
+                if (k == 2 || k == 3) {
+                    const R* h = s.hes + a * 5;
+                    const R rho = h[4];
+                    const R zv = rho * z[a] + r_sqrt(R(1) - rho * rho) * w[a];
+                    v[leg][a] += h[1] * (h[2] - vp) * dt + h[3] * r_sqrt(vp * dt) * zv;
+                }
+                x[leg][a] += R(-0.5) * var * dt + r_sqrt(var * dt) * z[a];
+            }
+        }
+        while (j < s.n_obs && s.obs_step[j] == i) record<R, N>(s, p, active, j++, x, ext);
+    }
+}
+
+// ---- device copy of a request --------------------------------------------------------------------------------
+
+struct Arena {
+    std::vector<unsigned char> host;
+    size_t put(const void* src, size_t bytes) {
+        const size_t at = (host.size() + 255) & ~size_t(255);
+        host.resize(at + bytes);
+        if (bytes) std::memcpy(host.data() + at, src, bytes);
+        return at;
+    }
+    template <typename R>
+    size_t reals(const double* src, long long n) {
+        std::vector<R> tmp(src ? static_cast<size_t>(n) : 0);
+        for (size_t i = 0; i < tmp.size(); ++i) tmp[i] = static_cast<R>(src[i]);
+        return put(tmp.data(), tmp.size() * sizeof(R));
+    }
+    size_t ints(const int* src, long long n) { return put(src, src ? static_cast<size_t>(n) * sizeof(int) : 0); }
+};
+
+template <typename R, int N>
+void launch_n(const PathArgs<R>& s, size_t lds, hipStream_t stream) {
+    const unsigned grid = static_cast<unsigned>((s.n_pairs + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL((paths_kernel<R, N>), dim3(grid), dim3(kBlock), lds, stream, s);
+}
+
+template <typename R>
+double launch_typed(const optlab_paths_request& rq, double* d_out, hipStream_t stream) {
+    const int n = rq.n_names;
+    const long long nsn = static_cast<long long>(n) * rq.n_steps;
+    Arena ar;
+    const size_t o_dt = ar.reals<R>(rq.dt, rq.n_steps), o_t0 = ar.reals<R>(rq.t0, rq.n_steps);
+    const size_t o_obs = ar.ints(rq.obs_step, rq.n_obs);
+    const size_t o_root = ar.reals<R>(rq.root, n * n), o_kind = ar.ints(rq.kind, n);
+    const size_t o_cvar = ar.reals<R>(rq.cvar, n), o_hes = ar.reals<R>(rq.heston, 5 * n);
+    bool any_table = false;
+    for (int a = 0; a < n; ++a) any_table |= rq.kind[a] == 1 || rq.kind[a] == 3;
+    const size_t o_row = ar.ints(any_table ? rq.step_row : nullptr, nsn);
+    const size_t o_lo = ar.reals<R>(any_table ? rq.row_lo : nullptr, rq.n_rows);
+    const size_t o_inv = ar.reals<R>(any_table ? rq.row_inv_dx : nullptr, rq.n_rows);
+    const size_t o_tab = ar.reals<R>(any_table ? rq.table : nullptr, static_cast<long long>(rq.n_rows) * rq.n_x);
+    const size_t o_fwd = ar.reals<R>(rq.fwd, static_cast<long long>(n) * rq.n_obs);
+    const size_t o_low = ar.reals<R>(rq.mixer ? rq.low_root : nullptr, n * n);
+    const size_t o_col = ar.ints(rq.col_of, rq.n_obs);
+    ar.put(nullptr, 0);
+    unsigned char* dev = static_cast<unsigned char*>(scratch(kSlotSpec, std::max<size_t>(ar.host.size(), 256)));
+    hip_check(hipMemcpyAsync(dev, ar.host.data(), ar.host.size(), hipMemcpyHostToDevice, stream), "copy spec");
+
+    PathArgs<R> s{};
+    s.n_names = n, s.n_steps = rq.n_steps, s.n_obs = rq.n_obs, s.n_x = rq.n_x;
+    s.n_pairs = rq.n_pairs, s.first_pair = rq.first_pair, s.rows = 2 * rq.n_pairs;
+    s.key0 = static_cast<std::uint32_t>(rq.seed), s.key1 = static_cast<std::uint32_t>(rq.seed >> 32);
+    auto at = [&](size_t o) { return dev + o; };
+    s.dt = reinterpret_cast<const R*>(at(o_dt)), s.t0 = reinterpret_cast<const R*>(at(o_t0));
+    s.obs_step = reinterpret_cast<const int*>(at(o_obs));
+    s.root = reinterpret_cast<const R*>(at(o_root)), s.kind = reinterpret_cast<const int*>(at(o_kind));
+    s.cvar = reinterpret_cast<const R*>(at(o_cvar)), s.hes = reinterpret_cast<const R*>(at(o_hes));
+    s.same = rq.same;
+    s.step_row = reinterpret_cast<const int*>(at(o_row));
+    s.lo = reinterpret_cast<const R*>(at(o_lo)), s.inv_dx = reinterpret_cast<const R*>(at(o_inv));
+    s.table = reinterpret_cast<const R*>(at(o_tab)), s.fwd = reinterpret_cast<const R*>(at(o_fwd));
+    s.mixer = rq.mixer;
+    for (int k = 0; k < 8; ++k) s.mix[k] = static_cast<R>(rq.mix[k]);
+    s.low_root = reinterpret_cast<const R*>(at(o_low));
+    s.n_cols = rq.n_cols, s.n_keep = rq.n_keep, s.col_of = reinterpret_cast<const int*>(at(o_col));
+    s.n_win = rq.n_windows;
+    for (int w = 0; w < 2; ++w) {
+        s.win_lo[w] = w < rq.n_windows ? rq.win_lo[w] : 0;
+        s.win_hi[w] = w < rq.n_windows ? rq.win_hi[w] : -1;
+    }
+    s.out = d_out;
+    s.n_tab = 0;
+    s.has_var = s.has_slv = 0;
+    for (int a = 0; a < 8; ++a) s.tab_slot[a] = -1, s.tab_of[a] = 0;
+    for (int a = 0; a < n; ++a) {
+        const int k = rq.kind[a];
+        if (k == 1 || k == 3) s.tab_slot[a] = s.n_tab, s.tab_of[s.n_tab++] = a;
+        s.has_var |= (k == 2 || k == 3);
+        s.has_slv |= k == 3;
+    }
+    const size_t stage_bytes = 2 * static_cast<size_t>(s.n_tab) * rq.n_x * sizeof(R);
+    s.staged = s.n_tab > 0 && rq.n_steps > 0 && stage_bytes <= kStageBytes;
+    const size_t lds = s.staged ? stage_bytes : 0;
+
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, stream), "event record");
+    if (n <= 1) launch_n<R, 1>(s, lds, stream);
+    else if (n <= 2) launch_n<R, 2>(s, lds, stream);
+    else if (n <= 4) launch_n<R, 4>(s, lds, stream);
+    else launch_n<R, kMaxNames>(s, lds, stream);
+    hip_check(hipGetLastError(), "paths kernel launch");
+    hip_check(hipEventRecord(e1, stream), "event record");
+    hip_check(hipEventSynchronize(e1), "paths kernel");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms;
+}
+
+__global__ void philox_kernel(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1, long long n,
+                              unsigned* out) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const U4 w = philox10(U4{c0 + static_cast<unsigned>(i), c1, c2, c3}, k0, k1);
+    out[4 * i] = w.x, out[4 * i + 1] = w.y, out[4 * i + 2] = w.z, out[4 * i + 3] = w.w;
+}
+
+template <typename R>
+__global__ void normals_kernel(long long first_pair, long long n_pairs, int n_normals, unsigned tag, unsigned k0,
+                               unsigned k1, double* out) {
+    const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n_pairs) return;
+    NormalStream<R> d;
+    d.init(static_cast<std::uint64_t>(first_pair + p), k0, k1, tag);
+    for (int k = 0; k < n_normals; ++k) out[p * n_normals + k] = static_cast<double>(d.next());
+}
+
+}  // namespace
+
+double launch_paths(const optlab_paths_request& rq, int prec, double* d_out, hipStream_t stream) {
+    return prec == 0 ? launch_typed<double>(rq, d_out, stream) : launch_typed<float>(rq, d_out, stream);
+}
+
+void launch_philox(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1, long long n,
+                   unsigned* out, hipStream_t stream) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(philox_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, c0, c1, c2,
+                       c3, k0, k1, n, out);
+    hip_check(hipGetLastError(), "philox kernel launch");
+}
+
+void launch_normals(long long first_pair, long long n_pairs, int n_normals, unsigned tag, unsigned long long seed,
+                    int prec, double* out, hipStream_t stream) {
+    if (n_pairs <= 0 || n_normals <= 0) return;
+    const dim3 grid(static_cast<unsigned>((n_pairs + 255) / 256));
+    const unsigned k0 = static_cast<unsigned>(seed), k1 = static_cast<unsigned>(seed >> 32);
+    if (prec == 0)
+        hipLaunchKernelGGL(normals_kernel<double>, grid, dim3(256), 0, stream, first_pair, n_pairs, n_normals, tag, k0,
+                           k1, out);
+    else
+        hipLaunchKernelGGL(normals_kernel<float>, grid, dim3(256), 0, stream, first_pair, n_pairs, n_normals, tag, k0,
+                           k1, out);
+    hip_check(hipGetLastError(), "normals kernel launch");
+}
+
+}  // namespace optlab_mi

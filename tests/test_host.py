@@ -1,0 +1,216 @@
+"""The host side without a GPU: the C ABI's exports, struct layouts and request checks; the plan layer (time grids,
+forwards, SVI / SSVI Dupire tables, baskets); the engine registry. The product path must fail loudly without a
+device: there is no CPU fallback."""
+
+from __future__ import annotations
+
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+from _support import ROOT, simulated_context
+
+from optlab import native
+from optlab.core import engines
+from optlab.core.market import Market
+from optlab.core.models import BlackScholesModel, HestonModel, LocalVolModel, time_grid
+from optlab.core.ssvi import SSVI
+from optlab.native import _abi
+
+HEADER = ROOT / "include" / "optlab_mi.h"
+
+
+def _declared():
+    text = HEADER.read_text()
+    return re.findall(r"OPTLAB_API\s+[\w\s\*]+?\b(optlab_\w+)\s*\(", text)
+
+
+def test_the_library_loads_and_exports_every_declared_symbol():
+    lib = _abi.load()
+    names = _declared()
+    assert len(names) >= 14
+    for name in names:
+        assert hasattr(lib, name), name
+    assert lib.optlab_abi_version() == 1 and lib.optlab_has_hip() == 1
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_abi.library_path())], capture_output=True, text=True)
+    exported = {ln.split()[-1] for ln in out.stdout.splitlines() if " T " in ln}
+    assert set(names) <= exported
+    assert all(n.startswith("optlab_") for n in exported)  # the C++ internals stay hidden
+
+
+def test_ctypes_layouts_match_the_c_header(tmp_path):
+    """sizeof and offsetof of every struct, from the C compiler, against the ctypes mirror."""
+    structs = {
+        "optlab_request": _abi.Request,
+        "optlab_result": _abi.Result,
+        "optlab_paths_request": _abi.PathsRequest,
+        "optlab_device": _abi.Device,
+        "optlab_leverage_request": _abi.LeverageRequest,
+        "optlab_strip_request": _abi.StripRequest,
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", str(src), "-o", str(exe)], check=True)
+    got = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    for line in filter(None, got):
+        cname, what, value = line.split()
+        py = structs[cname]
+        want = C.sizeof(py) if what == "size" else getattr(py, what).offset
+        assert int(value) == want, line
+
+
+def _price_request(**over):
+    rq = _abi.Request()
+    rq.model, rq.product, rq.n_steps, rq.S0, rq.T, rq.sigma = 0, 0, 1, 100.0, 1.0, 0.2
+    rq.legs[0], rq.strikes[0], rq.n_paths, rq.seed = 1, 100.0, 1000, 1
+    for k, v in over.items():
+        setattr(rq, k, v)
+    return rq
+
+
+@pytest.mark.parametrize(
+    "over,message",
+    [
+        ({"model": 3}, b"model must be 0"),
+        ({"product": 4}, b"product must be 0, 1, 2 or 3"),
+        ({"n_steps": 0}, b"n_steps must be >= 1"),
+        ({"n_paths": 1001}, b"n_paths must be even"),
+        ({"sigma": -1.0}, b"sigma must be positive"),
+        ({"product": 2, "H": 120.0}, b"need 0 < H < S0"),
+        ({"product": 3, "H": 90.0}, b"need H > S0"),
+    ],
+)
+def test_price_requests_are_checked_like_the_reference(over, message):
+    lib = _abi.load()
+    out = _abi.Result()
+    assert lib.optlab_price(C.byref(_price_request(**over)), 1, 0, C.byref(out)) == _abi.EINVAL
+    assert message in lib.optlab_last_error()
+
+
+def test_paths_requests_are_checked_like_the_reference():
+    market, surface, heston = simulated_context()
+    p = native.plan(LocalVolModel(surface), market, [0.5])
+    lib = _abi.load()
+    rq, keep, _ = native.paths_request(p, 100, 1)
+    rq.n_names = 9
+    assert lib.optlab_paths(C.byref(rq), 1, 0) == _abi.EINVAL
+    assert b"n_names must be 1 to 8" in lib.optlab_last_error()
+    rq, keep, _ = native.paths_request(p, 100, 1, windows=[(0, 0)])
+    rq.n_cols = 5
+    assert lib.optlab_paths(C.byref(rq), 1, 0) == _abi.EINVAL
+    assert b"n_cols must be n_keep + 2 n_windows" in lib.optlab_last_error()
+    rq, keep, _ = native.paths_request(p, 100, 1)
+    assert lib.optlab_paths(C.byref(rq), 5, 0) == _abi.EINVAL  # backend
+
+
+@pytest.mark.skipif(_abi.load().optlab_gpu_count() > 0, reason="this machine has a device")
+def test_without_a_device_the_engine_refuses_loudly():
+    market = Market.flat(100.0, 0.03, 0.0)
+    with pytest.raises(engines.EngineError, match="no HIP device"):
+        native.simulate(BlackScholesModel(0.2), market, [1.0], 1000, seed=1, backend="gpu")
+    assert not native.gpu_available()
+    with pytest.raises(engines.EngineError):
+        engines.choose("gpu", BlackScholesModel(0.2))
+    assert engines.choose("auto", BlackScholesModel(0.2)) == ("numpy", "no HIP device")
+
+
+def test_the_engine_is_found_by_name():
+    assert engines.compiled() is native
+    assert native.MAX_WINDOWS == 2 and callable(native.plan) and callable(native.run)
+    with pytest.raises(ValueError, match="engine"):
+        engines.choose("cuda", BlackScholesModel(0.2))
+    assert engines.seed64(-1) == 2**64 - 1
+
+
+def test_time_grids():
+    g = time_grid([0.25, 0.5, 1.0], np.inf)
+    np.testing.assert_array_equal(g, [0.0, 0.25, 0.5, 1.0])
+    g = time_grid([1.0], 1 / 252)
+    assert len(g) == 253 and g[-1] == 1.0
+    g = time_grid([1.0], 1 / 100, 1 / 2000)
+    h = np.diff(g)
+    assert h[0] == pytest.approx(1 / 2000) and g[-1] == 1.0
+    assert h[:-1].max() <= 1 / 100 + 1e-15 and h[-1] < 1.5 / 100  # the last step lands on the date
+    assert np.all(np.diff(h[:-1]) >= -1e-15)  # graded: steps grow with t until the cap
+
+
+def test_forward_and_discount_curves():
+    m = Market.flat(100.0, 0.04, 0.012)
+    T = np.array([0.0, 0.5, 1.0, 3.0])
+    np.testing.assert_allclose(m.F(T), 100 * np.exp(0.028 * T), rtol=1e-14)
+    np.testing.assert_allclose(m.D(T), np.exp(-0.04 * T), rtol=1e-14)
+    assert m.flat_rates(2.0) == pytest.approx((0.04, 0.012))
+
+
+def _fd_local_var(surface, k, T, h=1e-4):
+    """Dupire in total variance from finite differences of w alone: an independent route to the analytic one."""
+    w = lambda kk, TT: np.asarray(surface.w(kk, TT), dtype=float)  # noqa: E731
+    w0 = w(k, T)
+    wk = (w(k + h, T) - w(k - h, T)) / (2 * h)
+    wkk = (w(k + h, T) - 2 * w0 + w(k - h, T)) / (h * h)
+    wT = (w(k, T + h) - w(k, T - h)) / (2 * h)
+    g = (1 - k * wk / (2 * w0)) ** 2 - wk**2 / 4 * (1 / w0 + 0.25) + wkk / 2
+    return wT / g
+
+
+def test_svi_local_variance_matches_finite_differences():
+    _, surface, _ = simulated_context()
+    k = np.linspace(-0.4, 0.3, 15)
+    for T in (0.3, 0.9, 1.7):
+        np.testing.assert_allclose(surface.local_var(k, T), _fd_local_var(surface, k, T), rtol=2e-5)
+
+
+def test_ssvi_local_variance_matches_finite_differences():
+    s = SSVI.power_law(-0.6, 1.2, 0.5, [0.05, 0.25, 0.5, 1.0, 2.0, 3.0], 0.04 * np.array([0.05, 0.25, 0.5, 1.0, 2.0, 3.0]))
+    k = np.linspace(-0.5, 0.4, 13)
+    for T in (0.4, 1.3, 2.6):
+        np.testing.assert_allclose(s.local_var(k, T), _fd_local_var(s, k, T), rtol=2e-5)
+        # the power-law slice at a pillar: theta = 0.04 T, psi = theta phi(theta)
+    th, psi, rho = s.params(1.0)
+    assert float(th) == pytest.approx(0.04) and float(rho) == pytest.approx(-0.6)
+    assert float(psi) == pytest.approx(0.04 * 1.2 / (0.04**0.5 * 1.04**0.5))
+
+
+def test_plan_tables_and_basket_layout():
+    from test_oracle import _basket
+
+    b, market = _basket(mixer=True, slv=True)
+    p = native.plan(b, None, [0.25, 0.5])
+    assert p.names == ("SPX", "AAA", "BBB") and list(p.kind) == [1, 3, 0]
+    n = len(p.dt)
+    # name 0 reads its local-variance rows (one per step), name 1 the SLV rows after them
+    np.testing.assert_array_equal(p.step_row[0], np.arange(n))
+    assert p.step_row[1].min() >= n and p.step_row[2].max() == 0
+    assert p.table.shape == (len(p.lo), native.N_X)
+    np.testing.assert_allclose(p.root @ p.root.T, b.corr, atol=1e-12)
+    assert p.mixer == 1 and p.low_root.shape == (3, 3)
+    np.testing.assert_array_equal(p.obs_step, np.searchsorted(np.concatenate([[0.0], np.cumsum(p.dt)]), [0.25, 0.5]))
+    # local variance rows at each step start on [-8, 6] ATM standard deviations
+    _, surface, _ = simulated_context()
+    T = max(float(p.t0[40]), 1 / 365)
+    sd = float(np.sqrt(surface.w(0.0, T)))
+    assert p.lo[40] == pytest.approx(-8 * sd) and p.inv_dx[40] == pytest.approx((native.N_X - 1) / (14 * sd))
+    np.testing.assert_allclose(p.table[40, 400], surface.local_var(np.array([-8 * sd + 400 * 14 * sd / 800]), T)[0])
+
+
+def test_unsupported_models_are_named():
+    from optlab.structured.localcorr import StochasticMix
+    from optlab.structured.multiasset import Basket
+
+    market, surface, heston = simulated_context()
+    b = Basket(("A", "B"), {"A": HestonModel(heston), "B": BlackScholesModel(0.2)}, {"A": market, "B": market},
+               np.eye(2), mixer=StochasticMix())
+    assert native.unsupported(b) == "StochasticMix is not in the path engine (local correlation only)"
+    assert native.unsupported(object()) == "object is not in the path engine"
+    with pytest.raises(engines.EngineError, match="StochasticMix"):
+        engines.choose("cpu", b)
