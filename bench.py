@@ -75,7 +75,7 @@ def _time_cpu(threads: int):
     from _support import oracle
 
     n = 1 << 18
-    rq, _ = _request(0, n)
+    rq, out = _request(0, n)  # `out` owns the arrays rq points to
     lib = oracle()
     assert lib.oracle_strip(C.byref(rq), threads) == 0  # warm-up (page faults, thread start)
     reps = []

@@ -208,19 +208,21 @@ __global__ void __launch_bounds__(kBlock) fixed_kernel(Consts<R> c, long long n_
 
 // The strip: events at mat_step[e], values k < n_strikes the discounted pair-average payoffs of strike k, value
 // n_strikes the pair average of S. Slots of event e: 0..31 sums of values, 32..63 sums of their squares.
+// The strikes, discounts and event steps sit in device memory (read with wave-uniform scalar loads); each wave keeps
+// its compensated accumulators in its own slice of LDS, lane l at slot l, so registers hold only the path state.
 template <typename R>
 struct StripArgs {
     Consts<R> c;
     int model, put, n_strikes, n_mats;
-    R strikes[kMaxStrikes];
-    R disc[kMaxEvents];
-    int mat_step[kMaxEvents];
+    const R* strikes;    // n_strikes
+    const R* disc;       // n_mats
+    const int* mat_step; // n_mats
 };
 
 template <typename R>
-__device__ __forceinline__ R strip_value(const StripArgs<R>& s, int k, int e, R S0, R S1) {
+__device__ __forceinline__ R strip_value(const StripArgs<R>& s, int k, R D, R S0, R S1) {
     if (k < s.n_strikes) {
-        const R K = s.strikes[k], D = s.disc[e];
+        const R K = s.strikes[k];
         const R o = s.put ? D * r_max(K - S0, R(0)) + D * r_max(K - S1, R(0))
                           : D * r_max(S0 - K, R(0)) + D * r_max(S1 - K, R(0));
         return R(0.5) * o;
@@ -231,12 +233,13 @@ __device__ __forceinline__ R strip_value(const StripArgs<R>& s, int k, int e, R 
 template <typename R, int MODEL>
 __global__ void __launch_bounds__(kBlock) strip_kernel(StripArgs<R> s, long long n_pairs, std::uint32_t k0,
                                                         std::uint32_t k1, double* partials) {
+    extern __shared__ double strip_lds[];  // [wave in block][n_mats][64] sums, then the same for compensations
     const int lane = threadIdx.x & 63;
     const long long wave = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
-    const int n_values = s.n_strikes + 1;
-    Comp acc[kMaxEvents];
-#pragma unroll
-    for (int e = 0; e < kMaxEvents; ++e) acc[e] = Comp{0.0, 0.0};
+    const int n_values = s.n_strikes + 1, n_mats = s.n_mats;
+    double* acc_s = strip_lds + (threadIdx.x >> 6) * (2 * 64 * n_mats);
+    double* acc_c = acc_s + 64 * n_mats;
+    for (int e = 0; e < n_mats; ++e) acc_s[e * 64 + lane] = 0.0, acc_c[e * 64 + lane] = 0.0;
     const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
     for (long long base = static_cast<long long>(blockIdx.x) * blockDim.x; base < n_pairs; base += stride) {
         const long long p = base + threadIdx.x;
@@ -244,7 +247,7 @@ __global__ void __launch_bounds__(kBlock) strip_kernel(StripArgs<R> s, long long
         NormalStream<R> src;
         src.init(static_cast<std::uint64_t>(active ? p : 0), k0, k1, 0u);
         R y0 = R(0), y1 = R(0), v0 = s.c.v0, v1 = s.c.v0;
-        int e = 0;
+        int e = 0, next_event = s.mat_step[0];
         for (int i = 1; i <= s.c.n_steps; ++i) {
             if (MODEL == 1) {
                 const R zv = src.next(), zx = src.next();
@@ -255,8 +258,9 @@ __global__ void __launch_bounds__(kBlock) strip_kernel(StripArgs<R> s, long long
                 y0 += s.c.bs_drift + s.c.bs_vol * z;
                 y1 += s.c.bs_drift + s.c.bs_vol * -z;
             }
-            if (e < s.n_mats && s.mat_step[e] == i) {
+            if (i == next_event) {
                 const R S0 = s.c.S0 * r_exp(y0), S1 = s.c.S0 * r_exp(y1);
+                const R D = s.disc[e];
 #pragma unroll
                 for (int chunk = 0; chunk < 4; ++chunk) {
                     const int first = (chunk & 1) * 16;
@@ -264,23 +268,23 @@ __global__ void __launch_bounds__(kBlock) strip_kernel(StripArgs<R> s, long long
                     double vals[16];
 #pragma unroll
                     for (int i2 = 0; i2 < 16; ++i2) {
-                        const double yk = active ? static_cast<double>(strip_value(s, first + i2, e, S0, S1)) : 0.0;
+                        const double yk = active ? static_cast<double>(strip_value(s, first + i2, D, S0, S1)) : 0.0;
                         vals[i2] = chunk < 2 ? yk : yk * yk;
                     }
                     reduce_scatter16(vals, lane);
                     if ((lane >> 4) == chunk) {
-#pragma unroll
-                        for (int ee = 0; ee < kMaxEvents; ++ee)
-                            if (ee == e) acc[ee].add(vals[0]);
+                        Comp a{acc_s[e * 64 + lane], acc_c[e * 64 + lane]};
+                        a.add(vals[0]);
+                        acc_s[e * 64 + lane] = a.s;
+                        acc_c[e * 64 + lane] = a.c;
                     }
                 }
                 ++e;
+                next_event = e < n_mats ? s.mat_step[e] : -1;
             }
         }
     }
-#pragma unroll
-    for (int e = 0; e < kMaxEvents; ++e)
-        if (e < s.n_mats) partials[(wave * s.n_mats + e) * 64 + lane] = acc[e].value();
+    for (int e = 0; e < n_mats; ++e) partials[(wave * n_mats + e) * 64 + lane] = acc_s[e * 64 + lane] + acc_c[e * 64 + lane];
 }
 
 // totals[slot] = sum over waves of partials[w * n_slots + slot], in a fixed order: thread t of block `slot` adds
@@ -390,13 +394,15 @@ void launch_fixed(const HostConsts& hc, int model, int product, long long n_pair
 template <typename R>
 void launch_strip(const StripArgs<R>& s, long long n_pairs, std::uint64_t seed, unsigned grid, double* partials) {
     const std::uint32_t k0 = static_cast<std::uint32_t>(seed), k1 = static_cast<std::uint32_t>(seed >> 32);
+    const size_t lds = static_cast<size_t>(kBlock / 64) * 2 * 64 * s.n_mats * sizeof(double);
     if (s.model == 1)
-        hipLaunchKernelGGL((strip_kernel<R, 1>), dim3(grid), dim3(kBlock), 0, nullptr, s, n_pairs, k0, k1, partials);
+        hipLaunchKernelGGL((strip_kernel<R, 1>), dim3(grid), dim3(kBlock), lds, nullptr, s, n_pairs, k0, k1, partials);
     else
-        hipLaunchKernelGGL((strip_kernel<R, 0>), dim3(grid), dim3(kBlock), 0, nullptr, s, n_pairs, k0, k1, partials);
+        hipLaunchKernelGGL((strip_kernel<R, 0>), dim3(grid), dim3(kBlock), lds, nullptr, s, n_pairs, k0, k1, partials);
     hip_check(hipGetLastError(), "strip kernel launch");
 }
 
+// The strip's constants on the host, and its strikes, discounts and event steps copied to device memory.
 template <typename R>
 StripArgs<R> strip_args(const optlab_strip_request& rq) {
     StripArgs<R> s{};
@@ -404,11 +410,21 @@ StripArgs<R> strip_args(const optlab_strip_request& rq) {
                       nullptr, nullptr, rq.psi_c)
               .template as<R>();
     s.model = rq.model, s.put = rq.put, s.n_strikes = rq.n_strikes, s.n_mats = rq.n_mats;
-    for (int k = 0; k < rq.n_strikes; ++k) s.strikes[k] = static_cast<R>(rq.strikes[k]);
+    struct Block {
+        R strikes[kMaxStrikes];
+        R disc[kMaxEvents];
+        int mat_step[kMaxEvents];
+    } host{};
+    for (int k = 0; k < rq.n_strikes; ++k) host.strikes[k] = static_cast<R>(rq.strikes[k]);
     for (int e = 0; e < rq.n_mats; ++e) {
-        s.mat_step[e] = rq.mat_step[e];
-        s.disc[e] = static_cast<R>(std::exp(-rq.r * (rq.T * rq.mat_step[e] / rq.n_steps)));
+        host.mat_step[e] = rq.mat_step[e];
+        host.disc[e] = static_cast<R>(std::exp(-rq.r * (rq.T * rq.mat_step[e] / rq.n_steps)));
     }
+    Block* dev = static_cast<Block*>(scratch(kSlotSpec, sizeof(Block)));
+    hip_check(hipMemcpy(dev, &host, sizeof(Block), hipMemcpyHostToDevice), "copy strip");
+    s.strikes = dev->strikes;
+    s.disc = dev->disc;
+    s.mat_step = dev->mat_step;
     return s;
 }
 

@@ -359,6 +359,7 @@ def paths_request(p: Plan, n_pairs: int, seed: int, first_pair: int = 0, keep=No
     for i, v in enumerate(p.mix):
         rq.mix[i] = float(v)
     rq.n_cols, rq.n_keep, rq.n_windows = n_cols, len(keep), len(windows)
+    rq._keepalive = arrays  # the request points into these arrays: they live as long as it does
     return rq, arrays, len(keep)
 
 
@@ -444,6 +445,7 @@ def leverage_request(params: HestonParams, surface, T_max: float, mix: float, n_
     rq.n_lv = N_X
     for name, a in arrays.items():
         setattr(rq, name, _abi.ptr(a))
+    rq._keepalive = arrays
     return rq, arrays, t0
 
 

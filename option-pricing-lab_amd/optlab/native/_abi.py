@@ -216,9 +216,21 @@ def declare(lib: C.CDLL, signatures: dict) -> C.CDLL:
 _LIB: list = []
 
 
+def _one_hip_runtime() -> None:
+    """PyTorch-ROCm ships its own copy of the HIP runtime (libamdhip64.so.7, same soname as /opt/rocm's). Two
+    copies in one process cannot both open the device, so when PyTorch is installed it is imported first and the
+    engine binds to the runtime it loaded: the engine's device buffers and PyTorch's tensors then share one HIP
+    context (the adjoint's autograd.Function and the device-resident outputs rely on it)."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load() -> C.CDLL:
     """The HIP engine. Raises OSError when the library is not built (run build.py): no engine runs without it."""
     if not _LIB:
+        _one_hip_runtime()
         path = library_path()
         if not path.exists():
             raise OSError(f"{path} is not built: run `python __graft_entry__.py` (or build.py) to compile the HIP engine")

@@ -201,6 +201,7 @@ def strip_request(model, market: Market, strikes, mat_steps, n_steps: int, T: fl
     out = {"price": np.empty((len(mat_steps), width)), "se": np.empty((len(mat_steps), width)),
            "strikes": strikes, "mat_step": mat_steps}
     rq.price, rq.se = _abi.ptr(out["price"]), _abi.ptr(out["se"])
+    rq._keepalive = out  # the request points into these arrays: they live as long as it does
     return rq, out
 
 

@@ -56,11 +56,13 @@ def test_normals_match_the_oracle(lib, tag):
     n_pairs, k = 20_000, 13
     want = np.zeros(n_pairs * k)
     oracle().oracle_normals(3, n_pairs, k, tag, 2**35 + 11, want.ctypes.data)
-    for backend, tol in ((0, 4e-15), (1, 3e-6)):
+    # FP32: u and v rounded to 24 bits before the log and the angle, so |z| ~ 6 carries a few 1e-6 (absolute)
+    for backend, tol, typical in ((0, 4e-15, 1e-16), (1, 2e-5, 2e-7)):
         dev = torch.empty(n_pairs * k, dtype=torch.float64, device="cuda")
         assert lib.optlab_normals_dev(3, n_pairs, k, tag, 2**35 + 11, backend, C.c_void_p(dev.data_ptr()), None) == 0
         got = dev.cpu().numpy()
-        assert np.max(np.abs(got - want) / np.maximum(1.0, np.abs(want))) < tol
+        err = np.abs(got - want) / np.maximum(1.0, np.abs(want))
+        assert err.max() < tol and np.median(err) < typical
 
 
 def _gpu(p, n_pairs, seed, backend, **kw):
