@@ -3,7 +3,8 @@
     python option-pricing-lab_amd/build.py [--jobs N]
 
 Each translation unit is compiled separately (in parallel) and linked once. Packed FP32 instructions are disabled
-(-packed-fp32-ops off): plain float code only, the compiler may not fuse lanes into v_pk_* forms."""
+(-packed-fp32-ops off): plain float code only, the compiler may not fuse lanes into v_pk_* forms. FP32 division and
+square root are the hardware's (1 ulp) and FP32 denormals are flushed, as the reference's nvcc flags did."""
 
 from __future__ import annotations
 
@@ -28,6 +29,11 @@ FLAGS = [
     "-fPIC",
     "-Wall",
     "-fvisibility=hidden",
+    # FP32 arithmetic as the reference GPU engine compiled it (-prec-div=false -prec-sqrt=false -ftz=true,
+    # packages/native/cuda/CMakeLists.txt:50): division by v_rcp_f32, sqrt by v_sqrt_f32, FP32 denormals flushed.
+    # FP64 keeps IEEE division, square root and denormals.
+    "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+    "-fgpu-flush-denormals-to-zero",
     "-Xclang",
     "-target-feature",
     "-Xclang",

@@ -25,6 +25,10 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMaxStrikes = 31;
 constexpr int kMaxEvents = 16;
+#ifndef OPTLAB_STRIP_WAVES
+#define OPTLAB_STRIP_WAVES 1
+#endif
+constexpr int kStripWavesF32 = OPTLAB_STRIP_WAVES;
 
 template <typename R>
 struct Consts {  // schemes.hpp:159-207, computed in double on the host and cast once
@@ -206,85 +210,136 @@ __global__ void __launch_bounds__(kBlock) fixed_kernel(Consts<R> c, long long n_
     if (lane < 16) partials[wave * 16 + lane] = acc.value();
 }
 
-// The strip: events at mat_step[e], values k < n_strikes the discounted pair-average payoffs of strike k, value
-// n_strikes the pair average of S. Slots of event e: 0..31 sums of values, 32..63 sums of their squares.
-// The strikes, discounts and event steps sit in device memory (read with wave-uniform scalar loads); each wave keeps
-// its compensated accumulators in its own slice of LDS, lane l at slot l, so registers hold only the path state.
+// The strip: events at mat_step[e] and 32 value slots per event. Slot k < n_strikes is the discounted
+// pair-average payoff of strike k, slot n_strikes the pair average of S (a call struck at 0, undiscounted), the
+// rest zero. Each slot is D max(sign (S - K), 0) averaged over the two legs, with (K, D, sign) from a per-event
+// table, so the unrolled payoff code carries no per-slot conditions. Output slots of event e: 0..31 the sums of the
+// values, 32..63 the sums of their squares.
+template <typename R>
+struct StripSlot {
+    R K, D, sign;
+};
+
 template <typename R>
 struct StripArgs {
     Consts<R> c;
     int model, put, n_strikes, n_mats;
-    const R* strikes;    // n_strikes
-    const R* disc;       // n_mats
-    const int* mat_step; // n_mats
+    const StripSlot<R>* slots;  // n_mats x 32
+    const int* mat_step;        // n_mats
 };
 
 template <typename R>
-__device__ __forceinline__ R strip_value(const StripArgs<R>& s, int k, R D, R S0, R S1) {
-    if (k < s.n_strikes) {
-        const R K = s.strikes[k];
-        const R o = s.put ? D * r_max(K - S0, R(0)) + D * r_max(K - S1, R(0))
-                          : D * r_max(S0 - K, R(0)) + D * r_max(S1 - K, R(0));
-        return R(0.5) * o;
+__device__ __forceinline__ R strip_value(const StripSlot<R>& t, R S0, R S1) {
+    const R o = t.D * r_max(t.sign * (S0 - t.K), R(0)) + t.D * r_max(t.sign * (S1 - t.K), R(0));
+    return R(0.5) * o;
+}
+
+// One path step of both legs from normals z (z[0], z[1] for Heston: Zv then Zx; z[0] for Black-Scholes).
+template <typename R, int MODEL>
+__device__ __forceinline__ void strip_step(const Consts<R>& c, R& y0, R& y1, R& v0, R& v1, R za, R zb) {
+    if (MODEL == 1) {
+        qe_step(c, y0, v0, za, zb);
+        qe_step(c, y1, v1, -za, -zb);
+    } else {
+        y0 += c.bs_drift + c.bs_vol * za;
+        y1 += c.bs_drift + c.bs_vol * -za;
     }
-    return k == s.n_strikes ? R(0.5) * (S0 + S1) : R(0);
+}
+
+// Both legs of pair p through every step. The normals are taken four per Philox block exactly as the pair's
+// stream 0 hands them out (Heston: two steps per block, Black-Scholes: four), straight from registers. At an event
+// the two spots are parked in the wave's LDS slice; the payoffs and their reductions run after the path, so the
+// path loop carries only the path state.
+template <typename R, int MODEL>
+__device__ __forceinline__ void strip_path(const StripArgs<R>& s, std::uint64_t pair, std::uint32_t k0,
+                                           std::uint32_t k1, R* spot0, R* spot1, int lane) {
+    const Consts<R>& c = s.c;
+    R y0 = R(0), y1 = R(0), v0 = c.v0, v1 = c.v0;
+    int e = 0, next_event = s.mat_step[0];
+    const int n = c.n_steps;
+    constexpr int per_block = MODEL == 1 ? 2 : 4;
+    const std::uint32_t p0 = static_cast<std::uint32_t>(pair), p1 = static_cast<std::uint32_t>(pair >> 32);
+    for (int i0 = 0; i0 < n; i0 += per_block) {
+        const U4 r = philox10(U4{p0, p1, static_cast<std::uint32_t>(i0 / per_block), 0u}, k0, k1);
+        R z[4];
+        gauss2(r.x, r.y, z[0], z[1]);
+        gauss2(r.z, r.w, z[2], z[3]);
+#pragma unroll
+        for (int j = 0; j < per_block; ++j) {
+            const int i = i0 + j + 1;
+            if (i > n) break;
+            if (MODEL == 1)
+                strip_step<R, 1>(c, y0, y1, v0, v1, z[2 * j], z[2 * j + 1]);
+            else
+                strip_step<R, 0>(c, y0, y1, v0, v1, z[j], R(0));
+            if (i == next_event) {
+                spot0[e * 64 + lane] = c.S0 * r_exp(y0);
+                spot1[e * 64 + lane] = c.S0 * r_exp(y1);
+                ++e;
+                next_event = e < s.n_mats ? s.mat_step[e] : -1;
+            }
+        }
+    }
 }
 
 template <typename R, int MODEL>
-__global__ void __launch_bounds__(kBlock) strip_kernel(StripArgs<R> s, long long n_pairs, std::uint32_t k0,
-                                                        std::uint32_t k1, double* partials) {
-    extern __shared__ double strip_lds[];  // [wave in block][n_mats][64] sums, then the same for compensations
+__device__ __forceinline__ void strip_body(const StripArgs<R>& s, long long n_pairs, std::uint32_t k0, std::uint32_t k1,
+                                           double* partials) {
+    // LDS per wave: [n_mats][64] compensated sums (s, c), then [n_mats][64] spots of each leg
+    extern __shared__ double strip_lds[];
+    __shared__ StripSlot<R> table[kMaxEvents * 32];  // read with broadcast loads at the payoffs
+    for (int t = threadIdx.x; t < s.n_mats * 32; t += blockDim.x) table[t] = s.slots[t];
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const long long wave = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const int n_values = s.n_strikes + 1, n_mats = s.n_mats;
-    double* acc_s = strip_lds + (threadIdx.x >> 6) * (2 * 64 * n_mats);
+    double* acc_s = strip_lds + (threadIdx.x >> 6) * (2 * 64 * n_mats + (2 * 64 * n_mats * sizeof(R) + 7) / 8);
     double* acc_c = acc_s + 64 * n_mats;
+    R* spot0 = reinterpret_cast<R*>(acc_c + 64 * n_mats);
+    R* spot1 = spot0 + 64 * n_mats;
     for (int e = 0; e < n_mats; ++e) acc_s[e * 64 + lane] = 0.0, acc_c[e * 64 + lane] = 0.0;
     const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
     for (long long base = static_cast<long long>(blockIdx.x) * blockDim.x; base < n_pairs; base += stride) {
         const long long p = base + threadIdx.x;
         const bool active = p < n_pairs;
-        NormalStream<R> src;
-        src.init(static_cast<std::uint64_t>(active ? p : 0), k0, k1, 0u);
-        R y0 = R(0), y1 = R(0), v0 = s.c.v0, v1 = s.c.v0;
-        int e = 0, next_event = s.mat_step[0];
-        for (int i = 1; i <= s.c.n_steps; ++i) {
-            if (MODEL == 1) {
-                const R zv = src.next(), zx = src.next();
-                qe_step(s.c, y0, v0, zv, zx);
-                qe_step(s.c, y1, v1, -zv, -zx);
-            } else {
-                const R z = src.next();
-                y0 += s.c.bs_drift + s.c.bs_vol * z;
-                y1 += s.c.bs_drift + s.c.bs_vol * -z;
-            }
-            if (i == next_event) {
-                const R S0 = s.c.S0 * r_exp(y0), S1 = s.c.S0 * r_exp(y1);
-                const R D = s.disc[e];
+        strip_path<R, MODEL>(s, static_cast<std::uint64_t>(active ? p : 0), k0, k1, spot0, spot1, lane);
+        for (int e = 0; e < n_mats; ++e) {
+            const R S0 = spot0[e * 64 + lane], S1 = spot1[e * 64 + lane];
 #pragma unroll
-                for (int chunk = 0; chunk < 4; ++chunk) {
-                    const int first = (chunk & 1) * 16;
-                    if (first >= n_values) continue;
-                    double vals[16];
+            for (int chunk = 0; chunk < 4; ++chunk) {
+                const int first = (chunk & 1) * 16;
+                if (first >= n_values) continue;
+                double vals[16];
 #pragma unroll
-                    for (int i2 = 0; i2 < 16; ++i2) {
-                        const double yk = active ? static_cast<double>(strip_value(s, first + i2, D, S0, S1)) : 0.0;
-                        vals[i2] = chunk < 2 ? yk : yk * yk;
-                    }
-                    reduce_scatter16(vals, lane);
-                    if ((lane >> 4) == chunk) {
-                        Comp a{acc_s[e * 64 + lane], acc_c[e * 64 + lane]};
-                        a.add(vals[0]);
-                        acc_s[e * 64 + lane] = a.s;
-                        acc_c[e * 64 + lane] = a.c;
-                    }
+                for (int i2 = 0; i2 < 16; ++i2) {
+                    const double yk = active ? static_cast<double>(strip_value(table[e * 32 + first + i2], S0, S1)) : 0.0;
+                    vals[i2] = chunk < 2 ? yk : yk * yk;
                 }
-                ++e;
-                next_event = e < n_mats ? s.mat_step[e] : -1;
+                reduce_scatter16(vals, lane);
+                if ((lane >> 4) == chunk) {
+                    Comp a{acc_s[e * 64 + lane], acc_c[e * 64 + lane]};
+                    a.add(vals[0]);
+                    acc_s[e * 64 + lane] = a.s;
+                    acc_c[e * 64 + lane] = a.c;
+                }
             }
         }
     }
     for (int e = 0; e < n_mats; ++e) partials[(wave * n_mats + e) * 64 + lane] = acc_s[e * 64 + lane] + acc_c[e * 64 + lane];
+}
+
+// Separate FP32 and FP64 kernels, so an occupancy hint can be given to one without the other (OPTLAB_STRIP_WAVES:
+// minimum waves per SIMD asked of the FP32 kernel; 1 = the compiler's choice, measured fastest).
+template <int MODEL>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kStripWavesF32, 8)))
+strip_kernel_f32(StripArgs<float> s, long long n_pairs, std::uint32_t k0, std::uint32_t k1, double* partials) {
+    strip_body<float, MODEL>(s, n_pairs, k0, k1, partials);
+}
+
+template <int MODEL>
+__global__ void __launch_bounds__(kBlock) strip_kernel_f64(StripArgs<double> s, long long n_pairs, std::uint32_t k0,
+                                                            std::uint32_t k1, double* partials) {
+    strip_body<double, MODEL>(s, n_pairs, k0, k1, partials);
 }
 
 // totals[slot] = sum over waves of partials[w * n_slots + slot], in a fixed order: thread t of block `slot` adds
@@ -394,15 +449,23 @@ void launch_fixed(const HostConsts& hc, int model, int product, long long n_pair
 template <typename R>
 void launch_strip(const StripArgs<R>& s, long long n_pairs, std::uint64_t seed, unsigned grid, double* partials) {
     const std::uint32_t k0 = static_cast<std::uint32_t>(seed), k1 = static_cast<std::uint32_t>(seed >> 32);
-    const size_t lds = static_cast<size_t>(kBlock / 64) * 2 * 64 * s.n_mats * sizeof(double);
-    if (s.model == 1)
-        hipLaunchKernelGGL((strip_kernel<R, 1>), dim3(grid), dim3(kBlock), lds, nullptr, s, n_pairs, k0, k1, partials);
-    else
-        hipLaunchKernelGGL((strip_kernel<R, 0>), dim3(grid), dim3(kBlock), lds, nullptr, s, n_pairs, k0, k1, partials);
+    const size_t per_wave = 2 * 64 * s.n_mats * sizeof(double) + ((2 * 64 * s.n_mats * sizeof(R) + 7) / 8) * 8;
+    const size_t lds = static_cast<size_t>(kBlock / 64) * per_wave;
+    if constexpr (sizeof(R) == 4) {
+        if (s.model == 1)
+            hipLaunchKernelGGL(strip_kernel_f32<1>, dim3(grid), dim3(kBlock), lds, nullptr, s, n_pairs, k0, k1, partials);
+        else
+            hipLaunchKernelGGL(strip_kernel_f32<0>, dim3(grid), dim3(kBlock), lds, nullptr, s, n_pairs, k0, k1, partials);
+    } else {
+        if (s.model == 1)
+            hipLaunchKernelGGL(strip_kernel_f64<1>, dim3(grid), dim3(kBlock), lds, nullptr, s, n_pairs, k0, k1, partials);
+        else
+            hipLaunchKernelGGL(strip_kernel_f64<0>, dim3(grid), dim3(kBlock), lds, nullptr, s, n_pairs, k0, k1, partials);
+    }
     hip_check(hipGetLastError(), "strip kernel launch");
 }
 
-// The strip's constants on the host, and its strikes, discounts and event steps copied to device memory.
+// The strip's constants on the host, and its (event, slot) table and event steps copied to device memory.
 template <typename R>
 StripArgs<R> strip_args(const optlab_strip_request& rq) {
     StripArgs<R> s{};
@@ -411,19 +474,25 @@ StripArgs<R> strip_args(const optlab_strip_request& rq) {
               .template as<R>();
     s.model = rq.model, s.put = rq.put, s.n_strikes = rq.n_strikes, s.n_mats = rq.n_mats;
     struct Block {
-        R strikes[kMaxStrikes];
-        R disc[kMaxEvents];
+        StripSlot<R> slots[kMaxEvents * 32];
         int mat_step[kMaxEvents];
     } host{};
-    for (int k = 0; k < rq.n_strikes; ++k) host.strikes[k] = static_cast<R>(rq.strikes[k]);
     for (int e = 0; e < rq.n_mats; ++e) {
         host.mat_step[e] = rq.mat_step[e];
-        host.disc[e] = static_cast<R>(std::exp(-rq.r * (rq.T * rq.mat_step[e] / rq.n_steps)));
+        const R disc = static_cast<R>(std::exp(-rq.r * (rq.T * rq.mat_step[e] / rq.n_steps)));
+        for (int k = 0; k < 32; ++k) {
+            StripSlot<R>& t = host.slots[e * 32 + k];
+            if (k < rq.n_strikes)
+                t = StripSlot<R>{static_cast<R>(rq.strikes[k]), disc, rq.put ? R(-1) : R(1)};
+            else if (k == rq.n_strikes)
+                t = StripSlot<R>{R(0), R(1), R(1)};  // max(S - 0, 0) = S: the forward, undiscounted
+            else
+                t = StripSlot<R>{R(0), R(0), R(1)};
+        }
     }
     Block* dev = static_cast<Block*>(scratch(kSlotSpec, sizeof(Block)));
     hip_check(hipMemcpy(dev, &host, sizeof(Block), hipMemcpyHostToDevice), "copy strip");
-    s.strikes = dev->strikes;
-    s.disc = dev->disc;
+    s.slots = dev->slots;
     s.mat_step = dev->mat_step;
     return s;
 }
