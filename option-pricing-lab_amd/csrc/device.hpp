@@ -16,11 +16,15 @@ struct U4 {
     std::uint32_t x, y, z, w;
 };
 
+// Each round's two 32 x 32 -> 64 products are formed as 64-bit products, which the compiler maps to one
+// v_mad_u64_u32 each instead of a v_mul_hi_u32 / v_mul_lo_u32 pair.
 __device__ __forceinline__ U4 philox10(U4 c, std::uint32_t k0, std::uint32_t k1) {
 #pragma unroll
     for (int round = 0; round < 10; ++round) {
-        const std::uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
-        const std::uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        const std::uint64_t p0 = static_cast<std::uint64_t>(c.x) * 0xD2511F53u;
+        const std::uint64_t p1 = static_cast<std::uint64_t>(c.z) * 0xCD9E8D57u;
+        const std::uint32_t hi0 = static_cast<std::uint32_t>(p0 >> 32), lo0 = static_cast<std::uint32_t>(p0);
+        const std::uint32_t hi1 = static_cast<std::uint32_t>(p1 >> 32), lo1 = static_cast<std::uint32_t>(p1);
         c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
