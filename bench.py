@@ -94,6 +94,9 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the timed region, one device-to-device copy of this many MiB: the known-size kernel "
+                         "that scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this engine has no multi-GPU path (the reference has none either); "
@@ -171,6 +174,12 @@ def main(argv=None) -> int:
         }
         line["gpu_over_cpu"] = v32 / cpu_rate
     print(json.dumps(line), flush=True)
+    if args.copy_mib:
+        src = torch.ones(args.copy_mib << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
     return 0
 
 
