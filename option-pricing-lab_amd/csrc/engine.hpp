@@ -22,7 +22,7 @@ void hip_check(hipError_t e, const char* what);
 
 // Grow-only device buffers reused across calls (calls on one device are serialised by contract).
 void* scratch(int slot, size_t bytes);
-enum Slot { kSlotSpec = 0, kSlotOut, kSlotPartials, kSlotTotals, kSlotParticles, kSlotBins, kSlotAux, kSlotCount };
+enum Slot { kSlotSpec = 0, kSlotOut, kSlotPartials, kSlotTotals, kSlotParticles, kSlotStrip, kSlotAux, kSlotCount };
 
 int device_cu_count();
 

@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "device.hpp"
@@ -259,7 +260,20 @@ __global__ void __launch_bounds__(kBlock) fixed_kernel(Consts<R> c, long long n_
         const int v = lane & 15;
         if (lane < 16 && v < 10) acc.add(vals[0]);
     }
-    if (lane < 16) partials[wave * 16 + lane] = acc.value();
+    // the block's waves in a fixed order: one row of 16 partial sums per block
+    __shared__ double part_s[kBlock / 64][16], part_c[kBlock / 64][16];
+    const int wib = threadIdx.x >> 6;
+    if (lane < 16) part_s[wib][lane] = acc.s, part_c[wib][lane] = acc.c;
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        Comp a{part_s[0][lane], part_c[0][lane]};
+        for (int w = 1; w < kBlock / 64; ++w) {
+            a.add(part_s[w][lane]);
+            a.c += part_c[w][lane];
+        }
+        partials[static_cast<long long>(blockIdx.x) * 16 + lane] = a.value();
+    }
+    (void)wave;
 }
 
 // The strip: events at mat_step[e] and 32 value slots per event. Slot k < n_strikes is the discounted
@@ -271,6 +285,9 @@ template <typename R>
 struct StripSlot {
     R K, D, sign;
 };
+
+// Bytes of the largest (FP64) device table of a strip: (K, D, sign) per event and slot, and the event steps.
+constexpr size_t kStripTableBytes = kMaxEvents * 32 * 3 * sizeof(double) + kMaxEvents * sizeof(int);
 
 template <typename R>
 struct StripArgs {
@@ -377,7 +394,20 @@ __device__ __forceinline__ void strip_body(const StripArgs<R>& s, long long n_pa
             }
         }
     }
-    for (int e = 0; e < n_mats; ++e) partials[(wave * n_mats + e) * 64 + lane] = acc_s[e * 64 + lane] + acc_c[e * 64 + lane];
+    // the block's waves in a fixed order: one row of n_mats x 64 partial sums per block
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int per_wave = 2 * 64 * n_mats + (2 * 64 * n_mats * static_cast<int>(sizeof(R)) + 7) / 8;
+        for (int e = 0; e < n_mats; ++e) {
+            Comp a{acc_s[e * 64 + lane], acc_c[e * 64 + lane]};
+            for (int w = 1; w < kBlock / 64; ++w) {
+                a.add(strip_lds[w * per_wave + e * 64 + lane]);
+                a.c += strip_lds[w * per_wave + 64 * n_mats + e * 64 + lane];
+            }
+            partials[(static_cast<long long>(blockIdx.x) * n_mats + e) * 64 + lane] = a.value();
+        }
+    }
+    (void)wave;
 }
 
 // Separate FP32 and FP64 kernels, so an occupancy hint can be given to one without the other (OPTLAB_STRIP_WAVES:
@@ -394,28 +424,28 @@ __global__ void __launch_bounds__(kBlock) strip_kernel_f64(StripArgs<double> s, 
     strip_body<double, MODEL>(s, n_pairs, k0, k1, partials);
 }
 
-// totals[slot] = sum over waves of partials[w * n_slots + slot], in a fixed order: thread t of block `slot` adds
-// waves t, t + 256, ..., then a tree in LDS.
-__global__ void __launch_bounds__(256) reduce_kernel(const double* partials, long long n_waves, int n_slots,
+// totals[slot] = sum over rows r of partials[r n_slots + slot], in a fixed order. Block g takes slots
+// g 64 .. g 64 + 63; wave w of the block adds rows w, w + 4, ... reading 64 consecutive doubles per row (one
+// coalesced 512 B load), then the four waves are added in order.
+__global__ void __launch_bounds__(256) reduce_kernel(const double* partials, long long n_rows, int n_slots,
                                                       double* totals) {
-    const int slot = blockIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int slot = blockIdx.x * 64 + lane;
     Comp acc{0.0, 0.0};
-    for (long long w = threadIdx.x; w < n_waves; w += blockDim.x) acc.add(partials[w * n_slots + slot]);
-    __shared__ double hi[256], lo[256];
-    hi[threadIdx.x] = acc.s;
-    lo[threadIdx.x] = acc.c;
+    if (slot < n_slots)
+        for (long long r = w; r < n_rows; r += 4) acc.add(partials[r * n_slots + slot]);
+    __shared__ double hi[4][64], lo[4][64];
+    hi[w][lane] = acc.s;
+    lo[w][lane] = acc.c;
     __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if (static_cast<int>(threadIdx.x) < h) {
-            Comp a{hi[threadIdx.x], lo[threadIdx.x]};
-            a.add(hi[threadIdx.x + h]);
-            a.c += lo[threadIdx.x + h];
-            hi[threadIdx.x] = a.s;
-            lo[threadIdx.x] = a.c;
+    if (w == 0 && slot < n_slots) {
+        Comp a{hi[0][lane], lo[0][lane]};
+        for (int k = 1; k < 4; ++k) {
+            a.add(hi[k][lane]);
+            a.c += lo[k][lane];
         }
-        __syncthreads();
+        totals[slot] = a.value();
     }
-    if (threadIdx.x == 0) totals[slot] = hi[0] + lo[0];
 }
 
 void estimate(double sum, double sumsq, long long n_pairs, double& price, double& se) {  // cpu_mc.cpp:14-18
@@ -442,21 +472,32 @@ double geometric_asian(double S, double K, double T, double r, double q, double 
     return call ? D * (eg * norm_cdf(d1) - K * norm_cdf(d2)) : D * (K * norm_cdf(-d2) - eg * norm_cdf(-d1));
 }
 
-unsigned grid_for(long long n_pairs) {
+// One pair per lane up to eight rounds of resident workgroups (as many blocks per CU as the kernel's registers and
+// LDS allow); beyond that the grid-stride loop gives lanes more pairs. Several rounds let the dispatcher balance
+// the uneven wave times of QE-M's branches (measured: one resident round with two pairs per lane is 4 % slower).
+template <typename K>
+unsigned resident_grid(K kernel, size_t lds, long long n_pairs) {
+    int per_cu = 0;
+    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, lds), "occupancy");
     const long long want = (n_pairs + kBlock - 1) / kBlock;
-    const long long cap = static_cast<long long>(device_cu_count()) * 16;
+    const long long cap = static_cast<long long>(device_cu_count()) * std::max(per_cu, 1) * 8;
     return static_cast<unsigned>(std::max<long long>(1, std::min(want, cap)));
 }
 
 // HIP events around the main kernel of a call (mark() after it); stop() waits for the stream and returns the
-// main kernel's time in ms.
+// main kernel's time in ms. The two events are created once and reused (calls on a device are serialised).
 struct Timer {
-    hipEvent_t e0{}, e1{};
     hipStream_t st;
+    hipEvent_t e0, e1;
     bool marked = false;
     explicit Timer(hipStream_t s) : st(s) {
-        hip_check(hipEventCreate(&e0), "event");
-        hip_check(hipEventCreate(&e1), "event");
+        static hipEvent_t events[2] = {nullptr, nullptr};
+        if (!events[0]) {
+            hip_check(hipEventCreate(&events[0]), "event");
+            hip_check(hipEventCreate(&events[1]), "event");
+        }
+        e0 = events[0];
+        e1 = events[1];
         hip_check(hipEventRecord(e0, st), "event record");
     }
     void mark() {
@@ -470,51 +511,88 @@ struct Timer {
         hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
         return ms;
     }
-    ~Timer() {
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-    }
 };
 
-template <typename R>
-void launch_fixed(const HostConsts& hc, int model, int product, long long n_pairs, std::uint64_t seed, unsigned grid,
-                  double* partials) {
-    const Consts<R> c = hc.as<R>();
-    const std::uint32_t k0 = static_cast<std::uint32_t>(seed), k1 = static_cast<std::uint32_t>(seed >> 32);
-#define OPTLAB_FIXED(M, P) \
-    hipLaunchKernelGGL((fixed_kernel<R, M, P>), dim3(grid), dim3(kBlock), 0, nullptr, c, n_pairs, k0, k1, partials)
+// Calls f with the fixed-product kernel of (R, model, product).
+template <typename R, typename F>
+void with_fixed_kernel(int model, int product, F f) {
     if (model == 0) {
-        if (product == 0) OPTLAB_FIXED(0, 0);
-        else if (product == 1) OPTLAB_FIXED(0, 1);
-        else if (product == 2) OPTLAB_FIXED(0, 2);
-        else OPTLAB_FIXED(0, 3);
+        if (product == 0) f(fixed_kernel<R, 0, 0>);
+        else if (product == 1) f(fixed_kernel<R, 0, 1>);
+        else if (product == 2) f(fixed_kernel<R, 0, 2>);
+        else f(fixed_kernel<R, 0, 3>);
     } else {
-        if (product == 0) OPTLAB_FIXED(1, 0);
-        else if (product == 1) OPTLAB_FIXED(1, 1);
-        else if (product == 2) OPTLAB_FIXED(1, 2);
-        else OPTLAB_FIXED(1, 3);
+        if (product == 0) f(fixed_kernel<R, 1, 0>);
+        else if (product == 1) f(fixed_kernel<R, 1, 1>);
+        else if (product == 2) f(fixed_kernel<R, 1, 2>);
+        else f(fixed_kernel<R, 1, 3>);
     }
-#undef OPTLAB_FIXED
-    hip_check(hipGetLastError(), "fixed kernel launch");
+}
+
+// Calls f with the strip kernel of (R, model).
+template <typename R, typename F>
+void with_strip_kernel(int model, F f) {
+    if constexpr (sizeof(R) == 4) {
+        if (model == 1) f(strip_kernel_f32<1>);
+        else f(strip_kernel_f32<0>);
+    } else {
+        if (model == 1) f(strip_kernel_f64<1>);
+        else f(strip_kernel_f64<0>);
+    }
 }
 
 template <typename R>
-void launch_strip(const StripArgs<R>& s, long long n_pairs, std::uint64_t seed, unsigned grid, double* partials) {
+size_t strip_lds_bytes(int n_mats) {
+    const size_t per_wave = 2 * 64 * n_mats * sizeof(double) + ((2 * 64 * n_mats * sizeof(R) + 7) / 8) * 8;
+    return static_cast<size_t>(kBlock / 64) * per_wave;
+}
+
+// One fixed-product pricing: the resident grid, the partial sums (one row per block), the main kernel between
+// the timer's events, then the fixed-order reduction. Returns the main kernel's time in ms.
+template <typename R>
+double price_on_device(const HostConsts& hc, int model, int product, long long n_pairs, std::uint64_t seed,
+                       double* totals) {
+    const Consts<R> c = hc.as<R>();
     const std::uint32_t k0 = static_cast<std::uint32_t>(seed), k1 = static_cast<std::uint32_t>(seed >> 32);
-    const size_t per_wave = 2 * 64 * s.n_mats * sizeof(double) + ((2 * 64 * s.n_mats * sizeof(R) + 7) / 8) * 8;
-    const size_t lds = static_cast<size_t>(kBlock / 64) * per_wave;
-    if constexpr (sizeof(R) == 4) {
-        if (s.model == 1)
-            hipLaunchKernelGGL(strip_kernel_f32<1>, dim3(grid), dim3(kBlock), lds, nullptr, s, n_pairs, k0, k1, partials);
-        else
-            hipLaunchKernelGGL(strip_kernel_f32<0>, dim3(grid), dim3(kBlock), lds, nullptr, s, n_pairs, k0, k1, partials);
-    } else {
-        if (s.model == 1)
-            hipLaunchKernelGGL(strip_kernel_f64<1>, dim3(grid), dim3(kBlock), lds, nullptr, s, n_pairs, k0, k1, partials);
-        else
-            hipLaunchKernelGGL(strip_kernel_f64<0>, dim3(grid), dim3(kBlock), lds, nullptr, s, n_pairs, k0, k1, partials);
-    }
+    unsigned grid = 1;
+    with_fixed_kernel<R>(model, product, [&](auto kernel) { grid = resident_grid(kernel, 0, n_pairs); });
+    double* partials = static_cast<double*>(scratch(kSlotPartials, static_cast<size_t>(grid) * 16 * sizeof(double)));
+    Timer timer(nullptr);
+    with_fixed_kernel<R>(model, product, [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, nullptr, c, n_pairs, k0, k1, partials);
+    });
+    hip_check(hipGetLastError(), "fixed kernel launch");
+    timer.mark();
+    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, nullptr, partials, static_cast<long long>(grid), 16, totals);
+    hip_check(hipGetLastError(), "reduce kernel launch");
+    return timer.stop();
+}
+
+template <typename R>
+double strip_on_device(const StripArgs<R>& s, long long n_pairs, std::uint64_t seed, double* totals) {
+    const std::uint32_t k0 = static_cast<std::uint32_t>(seed), k1 = static_cast<std::uint32_t>(seed >> 32);
+    const size_t lds = strip_lds_bytes<R>(s.n_mats);
+    const int n_slots = s.n_mats * 64;
+    int dev = 0, lds_max = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "attribute");
+    if (lds > static_cast<size_t>(lds_max))
+        invalid("too many maturities for one workgroup's LDS in this precision (" + std::to_string(lds) + " > " +
+                std::to_string(lds_max) + " bytes)");
+    unsigned grid = 1;
+    with_strip_kernel<R>(s.model, [&](auto kernel) { grid = resident_grid(kernel, lds, n_pairs); });
+    double* partials =
+        static_cast<double*>(scratch(kSlotPartials, static_cast<size_t>(grid) * n_slots * sizeof(double)));
+    Timer timer(nullptr);
+    with_strip_kernel<R>(s.model, [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), lds, nullptr, s, n_pairs, k0, k1, partials);
+    });
     hip_check(hipGetLastError(), "strip kernel launch");
+    timer.mark();
+    hipLaunchKernelGGL(reduce_kernel, dim3((n_slots + 63) / 64), dim3(256), 0, nullptr, partials,
+                       static_cast<long long>(grid), n_slots, totals);
+    hip_check(hipGetLastError(), "reduce kernel launch");
+    return timer.stop();
 }
 
 // The strip's constants on the host, and its (event, slot) table and event steps copied to device memory.
@@ -542,8 +620,19 @@ StripArgs<R> strip_args(const optlab_strip_request& rq) {
                 t = StripSlot<R>{R(0), R(0), R(1)};
         }
     }
-    Block* dev = static_cast<Block*>(scratch(kSlotSpec, sizeof(Block)));
-    hip_check(hipMemcpy(dev, &host, sizeof(Block), hipMemcpyHostToDevice), "copy strip");
+    // The strip's own buffer, sized for the FP64 table so it is never reallocated; the table is uploaded only when
+    // it differs from the last one sent to this device (repeated calls on one strip skip the copy).
+    static std::vector<std::vector<unsigned char>> last;  // per device
+    int device = 0;
+    hip_check(hipGetDevice(&device), "hipGetDevice");
+    if (static_cast<int>(last.size()) <= device) last.resize(device + 1);
+    Block* dev = static_cast<Block*>(scratch(kSlotStrip, kStripTableBytes));
+    const unsigned char* bytes = reinterpret_cast<const unsigned char*>(&host);
+    std::vector<unsigned char>& seen = last[device];
+    if (seen.size() != sizeof(Block) || std::memcmp(seen.data(), bytes, sizeof(Block)) != 0) {
+        hip_check(hipMemcpy(dev, &host, sizeof(Block), hipMemcpyHostToDevice), "copy strip");
+        seen.assign(bytes, bytes + sizeof(Block));
+    }
     s.slots = dev->slots;
     s.mat_step = dev->mat_step;
     return s;
@@ -571,19 +660,9 @@ void run_price(const optlab_request& c, int prec, optlab_result& out) {
     const HostConsts hc = make_consts(c.n_steps, c.S0, c.r, c.q, c.T, c.sigma, c.v0, c.kappa, c.theta, c.xi, c.rho,
                                       c.H, c.legs, c.strikes, 1.5);
     const long long n_pairs = c.n_paths / 2;
-    const unsigned grid = grid_for(n_pairs);
-    const long long n_waves = static_cast<long long>(grid) * (kBlock / 64);
-    double* partials = static_cast<double*>(scratch(kSlotPartials, static_cast<size_t>(n_waves) * 16 * sizeof(double)));
     double* totals = static_cast<double*>(scratch(kSlotTotals, 16 * sizeof(double)));
-    Timer timer(nullptr);
-    if (prec == 0)
-        launch_fixed<double>(hc, c.model, c.product, n_pairs, c.seed, grid, partials);
-    else
-        launch_fixed<float>(hc, c.model, c.product, n_pairs, c.seed, grid, partials);
-    timer.mark();
-    hipLaunchKernelGGL(reduce_kernel, dim3(16), dim3(256), 0, nullptr, partials, n_waves, 16, totals);
-    hip_check(hipGetLastError(), "reduce kernel launch");
-    const double kernel_ms = timer.stop();
+    const double kernel_ms = prec == 0 ? price_on_device<double>(hc, c.model, c.product, n_pairs, c.seed, totals)
+                                       : price_on_device<float>(hc, c.model, c.product, n_pairs, c.seed, totals);
     double t[16];
     hip_check(hipMemcpy(t, totals, sizeof t, hipMemcpyDeviceToHost), "copy totals");
     std::memset(&out, 0, sizeof out);
@@ -627,20 +706,10 @@ void check_strip_request(const optlab_strip_request& c) {
 void run_strip(optlab_strip_request& rq, int prec) {
     const auto t_start = std::chrono::steady_clock::now();
     const long long n_pairs = rq.n_paths / 2;
-    const unsigned grid = grid_for(n_pairs);
-    const long long n_waves = static_cast<long long>(grid) * (kBlock / 64);
     const int n_slots = rq.n_mats * 64;
-    double* partials = static_cast<double*>(scratch(kSlotPartials, static_cast<size_t>(n_waves) * n_slots * sizeof(double)));
     double* totals = static_cast<double*>(scratch(kSlotTotals, static_cast<size_t>(n_slots) * sizeof(double)));
-    Timer timer(nullptr);
-    if (prec == 0)
-        launch_strip(strip_args<double>(rq), n_pairs, rq.seed, grid, partials);
-    else
-        launch_strip(strip_args<float>(rq), n_pairs, rq.seed, grid, partials);
-    timer.mark();
-    hipLaunchKernelGGL(reduce_kernel, dim3(n_slots), dim3(256), 0, nullptr, partials, n_waves, n_slots, totals);
-    hip_check(hipGetLastError(), "reduce kernel launch");
-    rq.kernel_ms = timer.stop();
+    rq.kernel_ms = prec == 0 ? strip_on_device(strip_args<double>(rq), n_pairs, rq.seed, totals)
+                             : strip_on_device(strip_args<float>(rq), n_pairs, rq.seed, totals);
     std::vector<double> t(n_slots);
     hip_check(hipMemcpy(t.data(), totals, n_slots * sizeof(double), hipMemcpyDeviceToHost), "copy totals");
     const int width = rq.n_strikes + 1;

@@ -277,3 +277,16 @@ def test_bad_requests_raise_without_touching_the_device(lib):
     assert b"product" in lib.optlab_last_error()
     rq = _request(1, 0, (1, 0, 0), (100, 0, 0), 10)
     assert lib.optlab_price(C.byref(rq), 2, 0, C.byref(out)) == _abi.EINVAL
+
+
+def test_strip_table_cache_never_serves_a_stale_table(lib):
+    """Calls alternate strikes, precisions and maturities; each must equal a fresh call of the same request."""
+    a, (_, _, pa, _) = _strip(n_paths=8192, strikes=(80, 100, 120))
+    b, (_, _, pb, _) = _strip(n_paths=8192, strikes=(90, 95, 105, 110), mats=(26, 104))
+    results = []
+    for rq, price, backend in ((a, pa, 1), (b, pb, 1), (a, pa, 0), (a, pa, 1), (b, pb, 0), (b, pb, 1)):
+        assert lib.optlab_strip(C.byref(rq), backend) == 0
+        results.append(price.copy())
+    np.testing.assert_array_equal(results[0], results[3])
+    np.testing.assert_array_equal(results[1], results[5])
+    assert not np.array_equal(results[0], results[2])  # FP32 against FP64 of the same strip: not bit-equal

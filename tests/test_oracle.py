@@ -322,6 +322,7 @@ def _strip(model=1, n_paths=40_000, put=0, n_steps=104, mats=(13, 26, 52, 104), 
     rq.n_paths, rq.seed = n_paths, 42
     price, se = np.zeros((len(mats), len(strikes) + 1)), np.zeros((len(mats), len(strikes) + 1))
     rq.price, rq.se = _abi.ptr(price), _abi.ptr(se)
+    rq._keepalive = (strikes, mats, price, se)  # the request points into these arrays
     return rq, (strikes, mats, price, se)
 
 
