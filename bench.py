@@ -41,6 +41,62 @@ LANE_SLOTS_PER_S = 256 * 64 * 2.4e9
 PEAK_TFLOPS = 78.6
 
 
+# ---- the SLV particle workload (SSVI surface, Heston factor, uniform daily grid) --------------------------------
+
+SLV_PILLARS = [0.05, 0.1, 0.25, 0.5, 0.75, 1.0, 1.5, 2.0, 2.5, 3.0]
+SLV_STRIKES = [70.0 + 3.0 * k for k in range(21)]
+SLV_MATS = [0.375 * (j + 1) for j in range(8)]  # 8 maturities to 3 y
+
+
+def slv_inputs():
+    """The synthetic SLV case: power-law SSVI (theta_t = 0.04 t, rho -0.6, eta 1.2, gamma 0.5) on the pillars above,
+    Heston factor (v0 .04, kappa 2, theta .04, xi .5, rho -.7), flat market S0 100, r .03, q 0."""
+    import numpy as np
+
+    from optlab.core.market import Market
+    from optlab.core.models import HestonParams
+    from optlab.core.ssvi import SSVI
+
+    pillars = np.array(SLV_PILLARS)
+    surface = SSVI.power_law(-0.6, 1.2, 0.5, pillars, 0.04 * pillars)
+    return Market.flat(100.0, 0.03, 0.0), surface, HestonParams(0.04, 2.0, 0.04, 0.5, -0.7)
+
+
+def slv_calibrate(n_particles, T, backend="gpu", n_bins=256, rng=17):
+    """The leverage by particles on a uniform 365/y grid, 256 bins, bandwidth 0.25 std(ln S) N^-1/5."""
+    from optlab import native
+
+    market, surface, heston = slv_inputs()
+    return native.calibrate_leverage(heston, surface, market, T, 1.0, n_particles, dt_max=1 / 365, dt_min=None,
+                                     rng=rng, backend=backend, n_bins=n_bins, bandwidth=("std", 0.25))
+
+
+def slv_reprice(model, n_paths, maturities, strikes, seed=23, backend="gpu"):
+    """Calls on SLV paths (uniform daily steps) against the surface: price gaps, SEs and implied-vol gaps in bp."""
+    import numpy as np
+
+    from optlab import native
+    from optlab.core.blackscholes import black_price, implied_vol
+
+    market, surface, _ = slv_inputs()
+    S = native.simulate(model, market, maturities, n_paths, seed=seed, dt_max=1 / 365, dt_min=None, backend=backend)
+    K = np.asarray(strikes)
+    h = S.shape[0] // 2
+    out = {"price_gap": [], "se": [], "vol_bp": []}
+    for j, T in enumerate(maturities):
+        F, D = float(market.F(T)), float(market.D(T))
+        pay = D * np.maximum(S[:, j][:, None] - K, 0.0)
+        pay = 0.5 * (pay[:h] + pay[h:])
+        p, se = pay.mean(axis=0), pay.std(axis=0, ddof=1) / np.sqrt(h)
+        target_vol = surface.iv(np.log(K / F), T)
+        target = black_price(F, K, T, target_vol, D)
+        vol = np.array([implied_vol(pi, F, k, T, D) for pi, k in zip(p, K)])
+        out["price_gap"].append(p - target)
+        out["se"].append(se)
+        out["vol_bp"].append((vol - target_vol) * 1e4)
+    return {k: np.array(v) for k, v in out.items()}
+
+
 def _request(backend_model, n_paths):
     from optlab.core.market import Market
     from optlab.core.models import HestonModel, HestonParams

@@ -128,6 +128,14 @@ typedef struct {
     double total_ms;
 } optlab_leverage_request;
 
+/* ---- addition: options of the particle method beyond the reference request ------------------------------- */
+/* bandwidth_rule 0: the reference's bandwidth_scale * atm_vol * sqrt(max(t, 0.15)) * N^-0.2 (leverage.cpp:118);
+   1: bandwidth_coef * std(ln S) * N^-0.2, the standard deviation taken over the particles at the step. */
+typedef struct {
+    int bandwidth_rule;
+    double bandwidth_coef;
+} optlab_leverage_options;
+
 /* ---- addition: a strip of vanillas on one simulation (Heston QE-M or Black-Scholes) ------------------------ */
 /* Calls (or puts) at n_strikes strikes and n_mats maturities t_j = T * mat_step[j] / n_steps, all on the same
    paths; price[j * (n_strikes + 1) + k] is strike k at maturity j, discounted by exp(-r t_j), and column
@@ -174,6 +182,9 @@ OPTLAB_API int optlab_paths_dev(optlab_paths_request* rq, int backend, double* d
 
 /* requests.hpp:108 (api::leverage). */
 OPTLAB_API int optlab_leverage(optlab_leverage_request* rq, int backend, int n_threads);
+
+/* Addition: api::leverage with the options above (opt NULL: the reference's rule). */
+OPTLAB_API int optlab_leverage_ex(optlab_leverage_request* rq, const optlab_leverage_options* opt, int backend);
 
 /* Addition: the strip of vanillas (no counterpart in the reference's fixed products, which price three legs at
    one maturity: params.hpp:27-40). */

@@ -80,10 +80,10 @@ void* scratch(int slot, size_t bytes) {
     if (slots.empty()) slots.resize(kSlotCount);
     Buffer& b = slots[slot];
     if (b.bytes < bytes) {
+        const size_t want = std::max(bytes, b.bytes + b.bytes / 2);  // grow by half at least: fewer reallocations
         if (b.ptr) hip_check(hipFree(b.ptr), "hipFree");
         b.ptr = nullptr;
         b.bytes = 0;
-        const size_t want = std::max(bytes, b.bytes + b.bytes / 2);
         hip_check(hipMalloc(&b.ptr, want), "hipMalloc");
         b.bytes = want;
     }
@@ -241,7 +241,17 @@ int optlab_leverage(optlab_leverage_request* rq, int backend, int n_threads) {
         check_leverage_request(*rq);
         const int prec = precision_of(backend);
         need_device();
-        run_leverage(*rq, prec);
+        run_leverage(*rq, nullptr, prec);
+    });
+}
+
+int optlab_leverage_ex(optlab_leverage_request* rq, const optlab_leverage_options* opt, int backend) {
+    return guarded([&] {
+        if (!rq) invalid("null request");
+        check_leverage_request(*rq);
+        const int prec = precision_of(backend);
+        need_device();
+        run_leverage(*rq, opt, prec);
     });
 }
 

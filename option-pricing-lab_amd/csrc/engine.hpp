@@ -40,7 +40,7 @@ void run_strip(optlab_strip_request& rq, int prec);
 
 // ---- SLV particles (leverage.hip) -----------------------------------------------------------------------------
 void check_leverage_request(const optlab_leverage_request& rq);
-void run_leverage(optlab_leverage_request& rq, int prec);
+void run_leverage(optlab_leverage_request& rq, const optlab_leverage_options* opt, int prec);
 
 // ---- test entry points (paths.hip) -----------------------------------------------------------------------------
 void launch_philox(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1, long long n,

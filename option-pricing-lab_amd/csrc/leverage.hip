@@ -48,6 +48,8 @@ struct LevArgs {
     std::uint32_t k0, k1;
     double v0, kappa, theta, xi, rho, rc;
     double atm_vol, bandwidth_scale, l_min, l_max, q_lo, q_hi;
+    int bw_rule;     // 0: the reference's bandwidth; 1: bw_coef std(x) N^-0.2 over the particles
+    double bw_coef;
     const double* dt;
     const double* t0;
     const double* lv_lo;
@@ -253,9 +255,29 @@ __global__ void __launch_bounds__(kRowThreads) row_kernel(LevArgs s, Particles<R
         const double a = qv[0], b = qv[1];
         for (int g = tid; g < nx; g += blockDim.x) x_eval[g] = a + (b - a) * g / (nx - 1);
         // regression on the bin centres, one wave per grid point
-        const double bw = s.bandwidth_scale * s.atm_vol * sqrt(fmax(t0, 0.15)) * pow(n_particles, -0.2);
         const double we = (hi + 1e-12 - lo) / nb;
         const double inv_d = ldexp(1.0, -s.shared->shift_d), inv_v = ldexp(1.0, -s.shared->shift_v);
+        double bw = s.bandwidth_scale * s.atm_vol * sqrt(fmax(t0, 0.15)) * pow(n_particles, -0.2);
+        if (s.bw_rule == 1) {
+            // std(x) over the particles from the bins: sum x = n c + w S_d, sum x^2 = n c^2 + 2 c w S_d + w^2 S_dd
+            double sx = 0.0, sxx = 0.0;
+            for (int k = tid; k < nb; k += blockDim.x) {
+                const double centre = lo + (k + 0.5) * we, cnt = static_cast<double>(s.sum_tot[k]);
+                const double sd = static_cast<double>(s.sum_tot[2 * nb + k]) * inv_d;
+                const double sdd = static_cast<double>(s.sum_tot[3 * nb + k]) * inv_d;
+                sx += cnt * centre + we * sd;
+                sxx += cnt * centre * centre + 2.0 * centre * we * sd + we * we * sdd;
+            }
+            sx = wave_sum(sx);
+            sxx = wave_sum(sxx);
+            if (lane == 0) scan[wave] = sx, scan[n_waves + wave] = sxx;
+            __syncthreads();
+            double tx = 0.0, txx = 0.0;
+            for (int w = 0; w < n_waves; ++w) tx += scan[w], txx += scan[n_waves + w];
+            __syncthreads();
+            const double mean = tx / n_particles, var = fmax(txx / n_particles - mean * mean, 0.0);
+            bw = s.bw_coef * sqrt(var) * pow(n_particles, -0.2);
+        }
         double vs = 0.0;
         for (int k = lane; k < nb; k += 64) vs += static_cast<double>(s.sum_tot[nb + k]) * inv_v;
         const double fallback = wave_sum(vs) / n_particles;
@@ -397,7 +419,7 @@ __global__ void init_kernel(Particles<R> P, long long n, R v0) {
 }
 
 template <typename R>
-void run_typed(optlab_leverage_request& rq) {
+void run_typed(optlab_leverage_request& rq, int bw_rule, double bw_coef) {
     const auto t_start = std::chrono::steady_clock::now();
     const int cus = device_cu_count();
     const long long np = rq.n_pairs, n = 2 * np;
@@ -410,6 +432,7 @@ void run_typed(optlab_leverage_request& rq) {
     s.rc = std::sqrt(1 - rq.rho * rq.rho);
     s.atm_vol = rq.atm_vol, s.bandwidth_scale = rq.bandwidth_scale, s.l_min = rq.l_min, s.l_max = rq.l_max;
     s.q_lo = rq.q_lo, s.q_hi = rq.q_hi;
+    s.bw_rule = bw_rule, s.bw_coef = bw_coef;
     s.n_step_blocks = static_cast<int>(std::max<long long>(1, std::min<long long>((np + kBlock - 1) / kBlock, 4LL * cus)));
     s.n_bin_blocks = static_cast<int>(std::max<long long>(1, std::min<long long>((n + kBlock - 1) / kBlock, cus)));
     const size_t rows = static_cast<size_t>(rq.n_steps);
@@ -477,11 +500,15 @@ void check_leverage_request(const optlab_leverage_request& rq) {  // requests.cp
     if (2 * rq.n_pairs >= (1LL << 40)) invalid("too many particles");
 }
 
-void run_leverage(optlab_leverage_request& rq, int prec) {
+void run_leverage(optlab_leverage_request& rq, const optlab_leverage_options* opt, int prec) {
+    if (opt && (opt->bandwidth_rule < 0 || opt->bandwidth_rule > 1)) invalid("bandwidth_rule must be 0 or 1");
+    if (opt && opt->bandwidth_rule == 1 && !(opt->bandwidth_coef > 0)) invalid("bandwidth_coef must be positive");
+    const int rule = opt ? opt->bandwidth_rule : 0;
+    const double coef = opt ? opt->bandwidth_coef : 0.0;
     if (prec == 0)
-        run_typed<double>(rq);
+        run_typed<double>(rq, rule, coef);
     else
-        run_typed<float>(rq);
+        run_typed<float>(rq, rule, coef);
 }
 
 }  // namespace optlab_mi

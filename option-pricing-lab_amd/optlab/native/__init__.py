@@ -416,7 +416,7 @@ def simulate(model, market, times, n_paths: int, seed=None, dt_max=1 / 252, dt_m
 
 def leverage_request(params: HestonParams, surface, T_max: float, mix: float, n_particles: int, dt_max, dt_min,
                      n_x, bandwidth_scale, rng, l_bounds, regression, positive_part, n_bins=400, n_fine=8192,
-                     atm_vol=None):
+                     atm_vol=None, bandwidth=None):
     """(request, keep-alive arrays, t0) of the particle method, as native/__init__.py:460-495 fills it."""
     if regression not in ("nadaraya_watson", "local_linear"):
         raise ValueError(f"unknown regression {regression!r}, expected nadaraya_watson or local_linear")
@@ -446,19 +446,39 @@ def leverage_request(params: HestonParams, surface, T_max: float, mix: float, n_
     for name, a in arrays.items():
         setattr(rq, name, _abi.ptr(a))
     rq._keepalive = arrays
+    rq._options = leverage_options(bandwidth)
     return rq, arrays, t0
+
+
+def leverage_options(bandwidth=None) -> _abi.LeverageOptions:
+    """None: the reference's bandwidth (bandwidth_scale atm_vol sqrt(max(t, 0.15)) N^-0.2); ("std", c): c std(ln S)
+    N^-0.2 over the particles at each step."""
+    opt = _abi.LeverageOptions()
+    if bandwidth is None:
+        opt.bandwidth_rule, opt.bandwidth_coef = 0, 0.0
+    elif isinstance(bandwidth, tuple) and len(bandwidth) == 2 and bandwidth[0] == "std":
+        opt.bandwidth_rule, opt.bandwidth_coef = 1, float(bandwidth[1])
+    else:
+        raise ValueError(f"bandwidth must be None or ('std', coef), not {bandwidth!r}")
+    return opt
 
 
 def calibrate_leverage(params, surface, market, T_max: float, mix: float = 1.0, n_particles: int = 100_000,
                        dt_max=1 / 100, dt_min=1 / 2000, n_x=81, bandwidth_scale=1.0, rng=None, l_bounds=(0.05, 8.0),
-                       regression="nadaraya_watson", positive_part=False, backend: str = "auto", threads: int = 0):
+                       regression="nadaraya_watson", positive_part=False, backend: str = "auto", threads: int = 0,
+                       *, n_bins: int = 400, bandwidth=None):
     """`slv.calibrate_leverage` on the engine (native/__init__.py:429-496): 400 regression bins, quantiles from an
-    8192-bin histogram, local variance from uniform rows, Philox draws."""
+    8192-bin histogram, local variance from uniform rows, Philox draws. Additions (keyword only): `n_bins` (at most
+    512) and `bandwidth=("std", c)` for a bandwidth of c std(ln S) N^-0.2; dt_min=None gives a uniform grid."""
     lib = _need_lib()
     prec = precision_code(backend)
     rq, arrays, t0 = leverage_request(params, surface, T_max, mix, n_particles, dt_max, dt_min, n_x,
-                                      bandwidth_scale, rng, l_bounds, regression, positive_part)
-    _check(lib, lib.optlab_leverage(C.byref(rq), prec, int(threads)))
+                                      bandwidth_scale, rng, l_bounds, regression, positive_part, n_bins=n_bins,
+                                      bandwidth=bandwidth)
+    if bandwidth is None:
+        _check(lib, lib.optlab_leverage(C.byref(rq), prec, int(threads)))
+    else:
+        _check(lib, lib.optlab_leverage_ex(C.byref(rq), C.byref(rq._options), prec))
     return SLVModel(params, Leverage(t0, arrays["x_grids"], arrays["values"]), mix)
 
 

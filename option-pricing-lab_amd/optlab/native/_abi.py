@@ -142,6 +142,12 @@ class LeverageRequest(C.Structure):
     ]
 
 
+class LeverageOptions(C.Structure):
+    """optlab_leverage_options (addition: the bandwidth rule of the particle method)."""
+
+    _fields_ = [("bandwidth_rule", _I), ("bandwidth_coef", _D)]
+
+
 class StripRequest(C.Structure):
     """optlab_strip_request (addition: a strip of vanillas on one simulation)."""
 
@@ -199,6 +205,7 @@ _SIGNATURES = {
     "optlab_paths": (_I, [C.POINTER(PathsRequest), _I, _I]),
     "optlab_paths_dev": (_I, [C.POINTER(PathsRequest), _I, C.c_void_p, C.c_void_p]),
     "optlab_leverage": (_I, [C.POINTER(LeverageRequest), _I, _I]),
+    "optlab_leverage_ex": (_I, [C.POINTER(LeverageRequest), C.POINTER(LeverageOptions), _I]),
     "optlab_strip": (_I, [C.POINTER(StripRequest), _I]),
     "optlab_philox_dev": (_I, [C.c_uint] * 6 + [_LL, C.c_void_p, C.c_void_p]),
     "optlab_normals_dev": (_I, [_LL, _LL, _I, C.c_uint, _ULL, _I, C.c_void_p, C.c_void_p]),

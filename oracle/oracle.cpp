@@ -652,7 +652,14 @@ int oracle_strip(optlab_strip_request* rq, int n_threads) {
 }
 
 int oracle_leverage(optlab_leverage_request* rq, int n_threads, int n_parts) {
+    return oracle_leverage_ex(rq, nullptr, n_threads, n_parts);
+}
+
+int oracle_leverage_ex(optlab_leverage_request* rq, const optlab_leverage_options* opt, int n_threads, int n_parts) {
     try {
+        const int rule = opt ? opt->bandwidth_rule : 0;
+        if (rule < 0 || rule > 1) bad("bandwidth_rule must be 0 or 1");
+        if (rule == 1 && !(opt->bandwidth_coef > 0)) bad("bandwidth_coef must be positive");
         if (!rq) bad("null request");
         const optlab_leverage_request& s = *rq;
         if (!s.dt || !s.t0 || !s.lv_lo || !s.lv_inv_dx || !s.lv_rows || !s.x_grids || !s.values) bad("null pointer");
@@ -714,7 +721,13 @@ int oracle_leverage(optlab_leverage_request* rq, int n_threads, int n_parts) {
                 // leverage.cpp:116-139: quantile grid, kernel regression on the bin centres
                 const double a = quantile(bins, lo, hi, s.q_lo), b = quantile(bins, lo, hi, s.q_hi);
                 for (int g = 0; g < nx; ++g) x_eval[g] = a + (b - a) * g / (nx - 1);
-                const double bw = s.bandwidth_scale * s.atm_vol * std::sqrt(std::max(t0, 0.15)) * std::pow(n_particles, -0.2);
+                double bw = s.bandwidth_scale * s.atm_vol * std::sqrt(std::max(t0, 0.15)) * std::pow(n_particles, -0.2);
+                if (rule == 1) {  // coef * std(x) * N^-0.2 over the particles, from the bins' sums of x and x^2
+                    double sx = 0.0, sxx = 0.0;
+                    for (int k = 0; k < nb; ++k) sx += bins.acc[2 * nb + k], sxx += bins.acc[3 * nb + k];
+                    const double mean = sx / n_particles, var = std::max(sxx / n_particles - mean * mean, 0.0);
+                    bw = opt->bandwidth_coef * std::sqrt(var) * std::pow(n_particles, -0.2);
+                }
                 const double width = (hi + 1e-12 - lo) / nb;
                 for (int k = 0; k < nb; ++k) centres[k] = lo + (k + 0.5) * width;
                 const double fallback = bins.v_sum / n_particles;

@@ -33,6 +33,9 @@ int oracle_strip(optlab_strip_request* rq, int n_threads);
    (the reference sums one contiguous part per thread and adds the parts in order; 0: one part per thread). */
 int oracle_leverage(optlab_leverage_request* rq, int n_threads, int n_parts);
 
+/* The same with the options of include/optlab_mi.h (opt NULL: the reference's bandwidth). */
+int oracle_leverage_ex(optlab_leverage_request* rq, const optlab_leverage_options* opt, int n_threads, int n_parts);
+
 /* closed_forms.cpp:21-33: geometric Asian on n equally spaced fixings. */
 double oracle_geometric_asian(double S, double K, double T, double r, double q, double sigma, int n, int call);
 

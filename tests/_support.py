@@ -34,6 +34,8 @@ def oracle() -> C.CDLL:
         "oracle_price": (C.c_int, [C.POINTER(_abi.Request), C.c_int, C.POINTER(_abi.Result)]),
         "oracle_strip": (C.c_int, [C.POINTER(_abi.StripRequest), C.c_int]),
         "oracle_leverage": (C.c_int, [C.POINTER(_abi.LeverageRequest), C.c_int, C.c_int]),
+        "oracle_leverage_ex": (C.c_int, [C.POINTER(_abi.LeverageRequest), C.POINTER(_abi.LeverageOptions), C.c_int,
+                                         C.c_int]),
         "oracle_geometric_asian": (C.c_double, [C.c_double] * 6 + [C.c_int, C.c_int]),
     }
     return _abi.declare(lib, sig)
