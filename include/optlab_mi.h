@@ -159,6 +159,68 @@ typedef struct {
     double kernel_ms, total_ms;
 } optlab_strip_request;
 
+/* ---- addition: implied-vol surfaces held on the device ----------------------------------------------------- */
+/* One surface on the engine's table grid: row r (one per simulation step) at maturity T[r] (>= 1/365), on the log
+   forward moneyness k = lo[r] + x / inv_dx[r], x < n_x, with the total variance w and its derivatives w_k, w_kk,
+   w_T there: the rows packages/risk/src/optlab/risk/surfaces.py:180-205 keeps for a surface. Uploaded once; the
+   local-variance tables of the surface moved by node tents of the risk grid, a parallel shift or a skew tilt are
+   then built on the device (surfaces.py:95-170) in double. */
+typedef struct {
+    int n_rows, n_x;
+    const double* T;       /* host, n_rows */
+    const double* lo;      /* host, n_rows */
+    const double* inv_dx;  /* host, n_rows */
+    const double* w;       /* host, n_rows x n_x */
+    const double* w_k;
+    const double* w_kk;
+    const double* w_T;
+} optlab_surface_rows;
+
+/* The risk grid: 10 strikes in forward moneyness x 8 tenors (risk/grid.py:25-26). */
+#define OPTLAB_GRID_STRIKES 10
+#define OPTLAB_GRID_TENORS 8
+#define OPTLAB_GRID_NODES (OPTLAB_GRID_STRIKES * OPTLAB_GRID_TENORS)
+
+/* ---- addition: worst-of autocall (Phoenix) under Dupire local vol, with adjoint Greeks ------------------- */
+/* The product of packages/structured/src/optlab/structured/compiler.py:114-161, 270-281 with a daily knock-in:
+   at observation j the performance is perf_j = min_a S_a(t_j) / fixing_a; a coupon `coupon` (times 1 + missed
+   with memory) is paid when perf_j >= coupon_barrier; before the last date the note redeems at par and ends when
+   perf_j >= autocall; at the last date it pays 1 - ki max(put_strike - perf, 0), ki = [min over every close of
+   min_a S_a / fixing_a < ki_barrier]. Every comparison is the ramp clip(d / (smoothing |b|) + 1/2, 0, 1) of
+   packages/structured/src/optlab/structured/script/evaluator.py:93-108 (smoothing 0: hard). Paths: the
+   engine's local-vol step (paths.hpp:274-361, kind 1) with n_names normals per step on Philox stream 0 of
+   each antithetic pair, one table row per step. Greeks (optlab_adjoint): delta per name at sticky moneyness
+   (the spot scales the paths) and vega per node of the risk grid per name, per vol point, at the unbumped
+   surfaces, by a pathwise adjoint. Standard errors: price and deltas from pair averages, vegas by batch means
+   over 16 batches. */
+typedef struct {
+    int n_names;             /* 1 to 5 */
+    int n_steps;             /* uniform steps of dt; row s of every surface is read at step s */
+    double dt;
+    const int* surface;      /* host, n_names surface handles, each with n_steps rows of one n_x */
+    const double* root;      /* host, n_names x n_names: correlated normals z = root . independent normals */
+    const double* fwd;       /* host, n_names x (n_steps + 1): forward F_a(s dt); s = 0 is the spot */
+    const double* fixing;    /* host, n_names: the levels performances are read against */
+    int n_obs;               /* 1 to 64 */
+    const int* obs_step;     /* host, n_obs increasing in [1, n_steps]; the last is n_steps (maturity) */
+    const double* disc;      /* host, n_obs discount factors of the payments */
+    double autocall, coupon_barrier, coupon;
+    int memory;
+    double ki_barrier;       /* 0: no knock-in (the note redeems at par) */
+    double put_strike;
+    double smoothing;
+    long long n_pairs, first_pair;
+    unsigned long long seed;
+    const double* nodes;     /* host or NULL: n_names x OPTLAB_GRID_NODES node bumps (vol units); price only */
+    const double* spot;      /* host or NULL: n_names factors on fwd (a spot bump at sticky moneyness) */
+    double price, se;
+    double batch[16];        /* the price of each of 16 batches of consecutive pairs (CRN differences) */
+    double delta[5], delta_se[5];
+    double* vega;            /* host or NULL: n_names x OPTLAB_GRID_NODES, per vol point (optlab_adjoint) */
+    double* vega_se;         /* host or NULL: the same shape */
+    double forward_ms, backward_ms, total_ms;
+} optlab_autocall_request;
+
 /* ---- entry points ------------------------------------------------------------------------------------------- */
 
 /* requests.hpp:97-101 */
@@ -189,6 +251,26 @@ OPTLAB_API int optlab_leverage_ex(optlab_leverage_request* rq, const optlab_leve
 /* Addition: the strip of vanillas (no counterpart in the reference's fixed products, which price three legs at
    one maturity: params.hpp:27-40). */
 OPTLAB_API int optlab_strip(optlab_strip_request* rq, int backend);
+
+/* Addition: upload a surface's rows to the current device; *handle names it in later calls (the reference keeps
+   the same rows per surface in a host cache, risk/surfaces.py:180-205). */
+OPTLAB_API int optlab_surface_upload(const optlab_surface_rows* rows, int* handle);
+OPTLAB_API int optlab_surface_free(int handle);
+
+/* Addition: the local-variance table of surface `handle` moved by node bumps (OPTLAB_GRID_NODES, strike-major,
+   vol units, or NULL), a parallel shift and a skew tilt (surfaces.py:141-165): out host, n_rows x n_x, double. */
+OPTLAB_API int optlab_surface_table(int handle, const double* nodes, double parallel, double skew, double* out);
+
+/* Addition: sum over entries of tbar[row][x] dLV[row][x] / dH_node at the unbumped surface, for every node of the
+   risk grid: tbar host n_rows x n_x, out host OPTLAB_GRID_NODES. The contraction the adjoint Greeks end with. */
+OPTLAB_API int optlab_surface_node_grad(int handle, const double* tbar, double* out);
+
+/* Addition: the worst-of autocall above, price only (the bump-and-revalue comparator calls this; structured/
+   greeks.py:164-313 revalues through the script evaluator). */
+OPTLAB_API int optlab_autocall(optlab_autocall_request* rq, int backend);
+
+/* Addition: price, deltas and node vegas by the pathwise adjoint (nodes must be NULL). */
+OPTLAB_API int optlab_adjoint(optlab_autocall_request* rq, int backend);
 
 /* Addition, for tests of the integer stream: the Philox4x32-10 words of counters (c0 + i, c1, c2, c3), i < n,
    key (k0, k1), computed by the device function the kernels use; out_dev holds 4 n uint32 (device memory). */

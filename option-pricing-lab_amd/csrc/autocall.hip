@@ -1,0 +1,723 @@
+// Worst-of autocall (Phoenix) on 1 to 5 names under Dupire local vol: price by Monte Carlo and Greeks by a pathwise
+// adjoint (deltas per name, vegas per node of the risk grid per surface).
+// Product: packages/structured/src/optlab/structured/compiler.py:114-161, 270-295 (coupon with memory, autocall,
+// knock-in put at maturity on the worst performance, a daily knock-in on the lowest close), every comparison
+// smoothed as in packages/structured/src/optlab/structured/script/evaluator.py:93-108. Paths: the local-vol step of
+// packages/native/cuda/include/optlab_cuda/paths.hpp:274-361 (kind 1: table variance, n_names normals per step
+// on stream 0 of the pair, z = +-root . zs). Bump Greeks it replaces: structured/greeks.py:164-313 and
+// packages/risk/src/optlab/risk/greeks.py:199-244.
+//
+// MI355X design:
+//   forward   one leg per lane (lanes 2p, 2p + 1 are the two legs of pair p, so a wave reduces pair averages with
+//             one shuffle), path state in VGPRs, the step's table rows read through the L1/L2 (every lane of the
+//             launch reads the same rows at the same step); with Greeks on, the state every kSeg steps goes to a
+//             checkpoint array (structure of arrays, coalesced) and the payoff's adjoint seeds are stored per leg.
+//   backward  persistent workgroups; segments of kSeg steps from the last to the first; within a segment each
+//             workgroup recomputes its legs' kSeg steps from the checkpoint, walks them backwards and adds the
+//             adjoint of every table entry it read into an LDS copy of the segment's rows as 64-bit fixed point
+//             (integer adds: the result does not depend on the order, so the Greeks are bit-reproducible); the
+//             rows are then written once to the workgroup's private table, and a reduction sums the workgroups
+//             of each of 16 batches. The tables' adjoints are contracted with dLV/dH on the device (surface.hip).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device.hpp"
+#include "engine.hpp"
+#include "surface.hpp"
+
+namespace optlab_mi {
+
+namespace {
+
+constexpr int kFwdBlock = 256;
+// Backward workgroup: 1024 lanes in FP32 (one workgroup a CU, so few private tables), 256 in FP64 (its state
+// needs twice the registers).
+template <typename R>
+constexpr int bwd_block() {
+    return sizeof(R) == 4 ? 1024 : 256;
+}
+constexpr int kSeg = 4;  // steps between checkpoints: with n normals a step, kSeg n normals fill whole Philox blocks
+constexpr int kBatches = 16;
+constexpr int kMaxNames = 5;
+constexpr int kMaxObs = 64;
+constexpr double kFixed = 4294967296.0;  // 2^32: the table adjoint in units of 2^-32
+
+template <typename R>
+struct Terms {  // compiler.py:75-111 parameters, in the path precision
+    R autocall, coupon_barrier, coupon, ki_barrier, put_strike, smoothing;
+    int memory;
+};
+
+template <typename R>
+struct AcArgs {
+    int n, n_steps, n_obs, n_x;
+    long long n_legs, first_pair;
+    std::uint32_t key0, key1;
+    R dt;
+    const R* root;                 // n x n
+    const R* table[kMaxNames];     // per name: n_steps x n_x
+    const double* lo[kMaxNames];   // per name: n_steps
+    const double* inv[kMaxNames];  // per name: n_steps
+    const R* lf;                   // n x (n_steps + 1): ln(F_a(s dt) / fixing_a)
+    const int* obs_step;           // n_obs
+    const int* obs_of;             // n_steps + 1: the observation at that step, or -1
+    const R* disc;                 // n_obs
+    Terms<R> pr;
+    R spot[kMaxNames];
+    int greeks;
+    R* ck;                // ((c n + a) n_legs + leg)
+    R* perf;              // n_obs x n_legs: perf_j, then dV / dx of the worst name at date j
+    unsigned char* worst; // n_obs x n_legs
+    R* lseed;             // n_legs: dV / dx of the lowest close's name at that close
+    int* lat;             // n_legs: 8 step + name of the lowest close
+    double* partials;     // forward workgroups x 16
+};
+
+// Smoothed comparisons (evaluator.py:93-108): the value and its derivative in the left operand.
+template <typename R>
+struct Ramp {
+    R v, d;
+};
+
+template <typename R>
+__device__ __forceinline__ Ramp<R> ramp_ge(R a, R b, R eps) {  // a >= b
+    const R scale = eps * (b < R(0) ? -b : b);
+    if (!(scale > R(0))) return Ramp<R>{a >= b ? R(1) : R(0), R(0)};
+    const R r = (a - b) / scale + R(0.5);
+    return Ramp<R>{r_min(r_max(r, R(0)), R(1)), (r > R(0) && r < R(1)) ? R(1) / scale : R(0)};
+}
+
+template <typename R>
+__device__ __forceinline__ Ramp<R> ramp_lt(R a, R b, R eps) {  // a < b
+    const R scale = eps * (b < R(0) ? -b : b);
+    if (!(scale > R(0))) return Ramp<R>{a < b ? R(1) : R(0), R(0)};
+    const R r = (b - a) / scale + R(0.5);
+    return Ramp<R>{r_min(r_max(r, R(0)), R(1)), (r > R(0) && r < R(1)) ? R(-1) / scale : R(0)};
+}
+
+template <typename R>
+struct Note {  // the script's state: alive (1 until called), missed coupons, value paid so far
+    R alive, missed, value;
+};
+
+// One observation date of the script (compiler.py:114-127 coupon, 270-281 call, 141-161 redemption).
+template <typename R>
+__device__ __forceinline__ void observe(const Terms<R>& t, bool last, R p, R low, R D, Note<R>& s) {
+    const R c = ramp_ge(p, t.coupon_barrier, t.smoothing).v;
+    const R f = t.memory ? R(1) + s.missed : R(1);
+    s.value += s.alive * c * t.coupon * f * D;
+    if (t.memory) s.missed = (R(1) - c) * (s.missed + R(1));
+    if (!last) {
+        const R u = ramp_ge(p, t.autocall, t.smoothing).v;
+        s.value += s.alive * u * D;
+        s.alive *= R(1) - u;
+    } else {
+        const R ki = t.ki_barrier > R(0) ? ramp_lt(low, t.ki_barrier, t.smoothing).v : R(0);
+        s.value += s.alive * D * (R(1) - ki * r_max(t.put_strike - p, R(0)));
+    }
+}
+
+// The lerp of paths.hpp:196-205 with its slope in x (zero where u is clamped) and the weight of entry i0 + 1.
+template <typename R>
+struct Lerp {
+    int i0;
+    R f, var, slope;
+};
+
+template <typename R>
+__device__ __forceinline__ Lerp<R> lerp_at(const R* y, R lo, R inv, int n_x, R x) {
+    const R u0 = (x - lo) * inv;
+    const R top = static_cast<R>(n_x - 1) - static_cast<R>(1e-4);
+    const R u = r_min(r_max(u0, R(0)), top);
+    const int i = static_cast<int>(u);
+    const R f = u - static_cast<R>(i);
+    const R y0 = y[i], y1 = y[i + 1];
+    const bool inside = u0 > R(0) && u0 < top;
+    return Lerp<R>{i, f, y0 * (R(1) - f) + y1 * f, inside ? (y1 - y0) * inv : R(0)};
+}
+
+template <typename R, int N>
+__device__ __forceinline__ void correlate(const R* root, R sign, const R (&zs)[N], R (&z)[N]) {
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+        R acc = R(0);
+#pragma unroll
+        for (int b = 0; b < N; ++b) acc += root[a * N + b] * zs[b];
+        z[a] = sign * acc;
+    }
+}
+
+template <typename R, int N>
+__global__ void __launch_bounds__(kFwdBlock) autocall_forward(AcArgs<R> s) {
+    __shared__ double wave_part[kFwdBlock / 64][16];
+    const long long i = static_cast<long long>(blockIdx.x) * kFwdBlock + threadIdx.x;
+    const bool active = i < s.n_legs;
+    const long long leg = active ? i : 0;
+    const R sign = (i & 1) ? R(-1) : R(1);
+    const int stride = s.n_steps + 1;
+    NormalStream<R> rng;
+    rng.init(static_cast<std::uint64_t>(s.first_pair + (leg >> 1)), s.key0, s.key1, 0u);
+
+    R x[N];
+#pragma unroll
+    for (int a = 0; a < N; ++a) x[a] = R(0);
+    R ymin = s.lf[0];
+    int at = 0;
+#pragma unroll
+    for (int a = 1; a < N; ++a)
+        if (s.lf[a * stride] < ymin) ymin = s.lf[a * stride], at = a;
+    Note<R> note{R(1), R(0), R(0)};
+    int j = 0, next = s.obs_step[0];
+
+    for (int step = 0; step < s.n_steps; ++step) {
+        if (s.greeks && step % kSeg == 0 && active) {
+            const long long c = step / kSeg;
+#pragma unroll
+            for (int a = 0; a < N; ++a) s.ck[(c * N + a) * s.n_legs + i] = x[a];
+        }
+        R zs[N], z[N];
+#pragma unroll
+        for (int a = 0; a < N; ++a) zs[a] = rng.next();
+        correlate<R, N>(s.root, sign, zs, z);
+#pragma unroll
+        for (int a = 0; a < N; ++a) {
+            const R* y = s.table[a] + static_cast<long long>(step) * s.n_x;
+            const Lerp<R> l = lerp_at(y, static_cast<R>(s.lo[a][step]), static_cast<R>(s.inv[a][step]), s.n_x, x[a]);
+            x[a] += R(-0.5) * l.var * s.dt + r_sqrt(l.var * s.dt) * z[a];
+        }
+        const int t1 = step + 1;
+        R yw = s.lf[t1] + x[0];
+        int w = 0;
+#pragma unroll
+        for (int a = 1; a < N; ++a) {
+            const R ya = s.lf[a * stride + t1] + x[a];
+            if (ya < yw) yw = ya, w = a;
+        }
+        if (yw < ymin) ymin = yw, at = t1 * 8 + w;
+        if (t1 == next) {
+            const R p = r_exp(yw);
+            const bool last = j == s.n_obs - 1;
+            if (s.greeks && active) {
+                s.perf[static_cast<long long>(j) * s.n_legs + i] = p;
+                s.worst[static_cast<long long>(j) * s.n_legs + i] = static_cast<unsigned char>(w);
+            }
+            observe(s.pr, last, p, r_exp(ymin), s.disc[j], note);
+            ++j;
+            next = j < s.n_obs ? s.obs_step[j] : -1;
+        }
+    }
+
+    // Reverse sweep of the script for dV / dperf_j and dV / dlow (the state before date j recomputed from the
+    // stored performances: n_obs^2 / 2 cheap steps instead of n_obs saved states).
+    R dsum[N];
+#pragma unroll
+    for (int a = 0; a < N; ++a) dsum[a] = R(0);
+    if (s.greeks && active) {
+        const Terms<R>& t = s.pr;
+        const R low = r_exp(ymin);
+        R abar = R(0), mbar = R(0), lbar = R(0);
+        for (int jj = s.n_obs - 1; jj >= 0; --jj) {
+            R alive = R(1), missed = R(0);
+            for (int q = 0; q < jj; ++q) {
+                const R pq = s.perf[static_cast<long long>(q) * s.n_legs + i];
+                const R c = ramp_ge(pq, t.coupon_barrier, t.smoothing).v;
+                if (t.memory) missed = (R(1) - c) * (missed + R(1));
+                alive *= R(1) - ramp_ge(pq, t.autocall, t.smoothing).v;
+            }
+            const long long o = static_cast<long long>(jj) * s.n_legs + i;
+            const R p = s.perf[o], D = s.disc[jj];
+            const Ramp<R> c = ramp_ge(p, t.coupon_barrier, t.smoothing);
+            R pbar = R(0);
+            if (jj == s.n_obs - 1) {
+                const R h = r_max(t.put_strike - p, R(0)), dh = t.put_strike - p > R(0) ? R(-1) : R(0);
+                const Ramp<R> ki = t.ki_barrier > R(0) ? ramp_lt(low, t.ki_barrier, t.smoothing) : Ramp<R>{R(0), R(0)};
+                abar += D * (R(1) - ki.v * h);
+                pbar += alive * D * (-ki.v * dh);
+                lbar += -alive * D * h * ki.d;
+            } else {
+                const Ramp<R> u = ramp_ge(p, t.autocall, t.smoothing);
+                const R ubar = alive * (D - abar);
+                abar = abar * (R(1) - u.v) + u.v * D;
+                pbar += ubar * u.d;
+            }
+            R cbar = R(0);
+            if (t.memory) {
+                cbar -= mbar * (missed + R(1));
+                mbar *= R(1) - c.v;
+            }
+            const R f = t.memory ? R(1) + missed : R(1);
+            abar += c.v * t.coupon * f * D;
+            cbar += alive * t.coupon * f * D;
+            if (t.memory) mbar += alive * c.v * t.coupon * D;
+            pbar += cbar * c.d;
+            const R seed = pbar * p;
+            s.perf[o] = seed;
+            const int w = s.worst[o];
+#pragma unroll
+            for (int a = 0; a < N; ++a)
+                if (a == w) dsum[a] += seed;
+        }
+        const R lseed = lbar * low;
+        s.lseed[i] = lseed;
+        s.lat[i] = at;
+#pragma unroll
+        for (int a = 0; a < N; ++a)
+            if (a == (at & 7)) dsum[a] += lseed;
+    }
+
+    // pair averages (lanes 2p and 2p + 1), then sums and sums of squares over the wave and the workgroup
+    double v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = 0.0;
+    {
+        const double mine = active ? static_cast<double>(note.value) : 0.0;
+        const double pair = 0.5 * (mine + shfl_xor_d(mine, 1));
+        const bool even = (threadIdx.x & 1) == 0;
+        v[0] = even ? pair : 0.0;
+        v[1] = even ? pair * pair : 0.0;
+#pragma unroll
+        for (int a = 0; a < N; ++a) {
+            const double d = active ? static_cast<double>(dsum[a] / s.spot[a]) : 0.0;
+            const double pd = 0.5 * (d + shfl_xor_d(d, 1));
+            v[2 + 2 * a] = even ? pd : 0.0;
+            v[3 + 2 * a] = even ? pd * pd : 0.0;
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    reduce_scatter16(v, lane);
+    if (lane < 16) wave_part[wave][lane] = v[0];
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kFwdBlock / 64; ++w) t += wave_part[w][threadIdx.x];
+        s.partials[static_cast<long long>(blockIdx.x) * 16 + threadIdx.x] = t;
+    }
+}
+
+// batches[b * 16 + k] = sum over the forward workgroups of batch b (in order) of value k.
+__global__ void batch_sums_kernel(const double* partials, long long n_blocks, double* batches) {
+    const int b = threadIdx.x / 16, k = threadIdx.x % 16;
+    const long long b0 = b * n_blocks / kBatches, b1 = (b + 1) * n_blocks / kBatches;
+    double t = 0.0;
+    for (long long blk = b0; blk < b1; ++blk) t += partials[blk * 16 + k];
+    batches[threadIdx.x] = t;
+}
+
+template <typename R>
+__device__ __forceinline__ long long to_fixed(R v) {
+    return static_cast<long long>(rint(v * static_cast<R>(kFixed)));
+}
+
+template <typename R, int N>
+__global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s, long long n_chunks, R* xbar,
+                                                               long long* partial) {
+    constexpr int kBwdBlock = bwd_block<R>();
+    extern __shared__ unsigned char lds_raw[];
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds_raw);  // kSeg x N x n_x
+    const long long G = gridDim.x, blk = blockIdx.x;
+    const long long c0 = blk * n_chunks / G, c1 = (blk + 1) * n_chunks / G;
+    const int n_seg = (s.n_steps + kSeg - 1) / kSeg;
+    const int row_len = N * s.n_x;
+    for (int seg = n_seg - 1; seg >= 0; --seg) {
+        const int s0 = seg * kSeg, len = min(kSeg, s.n_steps - s0);
+        for (int e = threadIdx.x; e < kSeg * row_len; e += kBwdBlock) acc[e] = 0ull;
+        __syncthreads();
+        for (long long chunk = c0; chunk < c1; ++chunk) {
+            const long long i = chunk * kBwdBlock + threadIdx.x;
+            const bool active = i < s.n_legs;
+            const long long leg = active ? i : 0;
+            const R sign = (i & 1) ? R(-1) : R(1);
+            R x[N], xb[N];
+#pragma unroll
+            for (int a = 0; a < N; ++a) {
+                x[a] = s.ck[(static_cast<long long>(seg) * N + a) * s.n_legs + leg];
+                xb[a] = seg == n_seg - 1 ? R(0) : xbar[a * s.n_legs + leg];
+            }
+            NormalStream<R> rng;
+            rng.init(static_cast<std::uint64_t>(s.first_pair + (leg >> 1)), s.key0, s.key1, 0u);
+            rng.block = static_cast<std::uint32_t>(seg * (kSeg * N / 4));
+            R xs[kSeg][N], zz[kSeg][N];
+#pragma unroll
+            for (int k = 0; k < kSeg; ++k) {
+                if (k >= len) break;
+                const int step = s0 + k;
+                R zs[N], z[N];
+#pragma unroll
+                for (int a = 0; a < N; ++a) zs[a] = rng.next();
+                correlate<R, N>(s.root, sign, zs, z);
+#pragma unroll
+                for (int a = 0; a < N; ++a) {
+                    xs[k][a] = x[a];
+                    zz[k][a] = z[a];
+                    const R* y = s.table[a] + static_cast<long long>(step) * s.n_x;
+                    const Lerp<R> l =
+                        lerp_at(y, static_cast<R>(s.lo[a][step]), static_cast<R>(s.inv[a][step]), s.n_x, x[a]);
+                    x[a] += R(-0.5) * l.var * s.dt + r_sqrt(l.var * s.dt) * z[a];
+                }
+            }
+            const int lat = s.lat[leg];
+            const R lseed = s.lseed[leg];
+#pragma unroll
+            for (int k = kSeg - 1; k >= 0; --k) {
+                if (k >= len) continue;
+                const int step = s0 + k, t1 = step + 1;
+                const int j = s.obs_of[t1];
+                if (j >= 0) {
+                    const long long o = static_cast<long long>(j) * s.n_legs + leg;
+                    const int w = s.worst[o];
+                    const R g = s.perf[o];
+#pragma unroll
+                    for (int a = 0; a < N; ++a)
+                        if (a == w) xb[a] += g;
+                }
+                if ((lat >> 3) == t1) {
+#pragma unroll
+                    for (int a = 0; a < N; ++a)
+                        if (a == (lat & 7)) xb[a] += lseed;
+                }
+#pragma unroll
+                for (int a = 0; a < N; ++a) {
+                    const R* y = s.table[a] + static_cast<long long>(step) * s.n_x;
+                    const Lerp<R> l =
+                        lerp_at(y, static_cast<R>(s.lo[a][step]), static_cast<R>(s.inv[a][step]), s.n_x, xs[k][a]);
+                    const R sd = r_sqrt(l.var * s.dt);
+                    const R vbar = xb[a] * s.dt * (R(-0.5) + R(0.5) * zz[k][a] / sd);
+                    if (active) {
+                        unsigned long long* r = acc + (k * N + a) * s.n_x + l.i0;
+                        const long long q0 = to_fixed(vbar * (R(1) - l.f)), q1 = to_fixed(vbar * l.f);
+                        if (q0) atomicAdd(r, static_cast<unsigned long long>(q0));
+                        if (q1) atomicAdd(r + 1, static_cast<unsigned long long>(q1));
+                    }
+                    xb[a] += vbar * l.slope;
+                }
+            }
+            if (seg > 0 && active) {
+#pragma unroll
+                for (int a = 0; a < N; ++a) xbar[a * s.n_legs + i] = xb[a];
+            }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < len * row_len; e += kBwdBlock) {
+            const int k = e / row_len, rest = e - k * row_len, a = rest / s.n_x, xx = rest - a * s.n_x;
+            partial[((blk * N + a) * s.n_steps + s0 + k) * s.n_x + xx] = static_cast<long long>(acc[e]);
+        }
+        __syncthreads();
+    }
+}
+
+// batch[b][e] = sum over the backward workgroups of batch b (in order) of partial[w][e], e < entries.
+__global__ void table_batches_kernel(const long long* partial, long long entries, long long n_blocks, long long* batch) {
+    for (long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; e < entries;
+         e += static_cast<long long>(gridDim.x) * blockDim.x) {
+        for (int b = 0; b < kBatches; ++b) {
+            const long long w0 = b * n_blocks / kBatches, w1 = (b + 1) * n_blocks / kBatches;
+            unsigned long long t = 0;
+            for (long long w = w0; w < w1; ++w) t += static_cast<unsigned long long>(partial[w * entries + e]);
+            batch[b * entries + e] = static_cast<long long>(t);
+        }
+    }
+}
+
+// ---- host -------------------------------------------------------------------------------------------------------
+
+struct Arena {
+    std::vector<unsigned char> host;
+    size_t put(const void* src, size_t bytes) {
+        const size_t at = (host.size() + 255) & ~size_t(255);
+        host.resize(at + bytes);
+        if (bytes) std::memcpy(host.data() + at, src, bytes);
+        return at;
+    }
+    template <typename R>
+    size_t reals(const std::vector<double>& v) {
+        std::vector<R> tmp(v.begin(), v.end());
+        return put(tmp.data(), tmp.size() * sizeof(R));
+    }
+};
+
+struct Events {
+    hipEvent_t e[4];
+    Events() {
+        for (auto& x : e) hip_check(hipEventCreate(&x), "hipEventCreate");
+    }
+    ~Events() {
+        for (auto& x : e) (void)hipEventDestroy(x);
+    }
+    float ms(int a, int b) const {
+        float t = 0.f;
+        hip_check(hipEventElapsedTime(&t, e[a], e[b]), "hipEventElapsedTime");
+        return t;
+    }
+};
+
+void mean_se(double sum, double sum2, double n, double& mean, double& se) {  // cpu_mc.cpp:14-18
+    mean = sum / n;
+    const double var = n > 1 ? std::max(sum2 - sum * sum / n, 0.0) / (n - 1) : 0.0;
+    se = std::sqrt(var / n);
+}
+
+template <typename R, int N>
+void run_typed(optlab_autocall_request& rq, bool adjoint) {
+    const auto wall0 = std::chrono::steady_clock::now();
+    const int n = rq.n_names, ns = rq.n_steps;
+    const long long n_legs = 2 * rq.n_pairs;
+    const TentGrid grid = make_tent_grid();
+    const int n_nodes = grid.n_strikes * grid.n_tenors;
+    constexpr int kBwdBlock = bwd_block<R>();
+    hipStream_t stream = nullptr;
+
+    // tables: cached unmoved ones, or bumped ones built here
+    const SurfaceRows* rows[kMaxNames];
+    const R* tables[kMaxNames];
+    int n_x = 0;
+    int n_bumped = 0;
+    for (int a = 0; a < n; ++a) {
+        rows[a] = &surface_rows(rq.surface[a]);
+        n_x = static_cast<int>(rows[a]->n_x);
+        bool moved = false;
+        if (rq.nodes)
+            for (int k = 0; k < n_nodes; ++k) moved |= rq.nodes[a * n_nodes + k] != 0.0;
+        n_bumped += moved;
+    }
+    const size_t table_bytes = static_cast<size_t>(ns) * n_x * sizeof(R);
+    const size_t node_bytes = static_cast<size_t>(n_nodes) * sizeof(double);
+    unsigned char* bump_mem = n_bumped ? static_cast<unsigned char*>(scratch(
+                                             kSlotTables, n_bumped * (((table_bytes + 255) & ~size_t(255)) + node_bytes)))
+                                       : nullptr;
+    for (int a = 0, b = 0; a < n; ++a) {
+        bool moved = false;
+        if (rq.nodes)
+            for (int k = 0; k < n_nodes; ++k) moved |= rq.nodes[a * n_nodes + k] != 0.0;
+        if (!moved) {
+            tables[a] = static_cast<const R*>(surface_base_table(rq.surface[a], std::is_same<R, double>::value ? 0 : 1,
+                                                                 stream));
+            continue;
+        }
+        unsigned char* at = bump_mem + b++ * (((table_bytes + 255) & ~size_t(255)) + node_bytes);
+        double* d_nodes = reinterpret_cast<double*>(at + ((table_bytes + 255) & ~size_t(255)));
+        hip_check(hipMemcpyAsync(d_nodes, rq.nodes + a * n_nodes, node_bytes, hipMemcpyHostToDevice, stream),
+                  "copy nodes");
+        launch_bumped_table(*rows[a], grid, d_nodes, SurfaceMove{0.0, 0.0}, std::is_same<R, double>::value ? 0 : 1,
+                            at, stream);
+        tables[a] = reinterpret_cast<const R*>(at);
+    }
+
+    // the request's small arrays
+    std::vector<double> root(rq.root, rq.root + n * n), lf(static_cast<size_t>(n) * (ns + 1)), disc(rq.disc,
+                                                                                                  rq.disc + rq.n_obs);
+    double spot[kMaxNames];
+    for (int a = 0; a < n; ++a) {
+        const double factor = rq.spot ? rq.spot[a] : 1.0;
+        spot[a] = rq.fwd[a * (ns + 1)] * factor;
+        for (int t = 0; t <= ns; ++t) lf[a * (ns + 1) + t] = std::log(rq.fwd[a * (ns + 1) + t] * factor / rq.fixing[a]);
+    }
+    std::vector<int> obs_of(ns + 1, -1);
+    for (int j = 0; j < rq.n_obs; ++j) obs_of[rq.obs_step[j]] = j;
+    Arena ar;
+    const size_t o_root = ar.reals<R>(root), o_lf = ar.reals<R>(lf), o_disc = ar.reals<R>(disc);
+    const size_t o_obs = ar.put(rq.obs_step, rq.n_obs * sizeof(int)), o_of = ar.put(obs_of.data(), obs_of.size() * 4);
+    ar.put(nullptr, 0);
+    unsigned char* dev = static_cast<unsigned char*>(scratch(kSlotSpec, std::max<size_t>(ar.host.size(), 256)));
+    hip_check(hipMemcpyAsync(dev, ar.host.data(), ar.host.size(), hipMemcpyHostToDevice, stream), "copy spec");
+
+    AcArgs<R> s{};
+    s.n = n, s.n_steps = ns, s.n_obs = rq.n_obs, s.n_x = n_x;
+    s.n_legs = n_legs, s.first_pair = rq.first_pair;
+    s.key0 = static_cast<std::uint32_t>(rq.seed), s.key1 = static_cast<std::uint32_t>(rq.seed >> 32);
+    s.dt = static_cast<R>(rq.dt);
+    s.root = reinterpret_cast<const R*>(dev + o_root);
+    for (int a = 0; a < n; ++a) {
+        s.table[a] = tables[a];
+        s.lo[a] = rows[a]->lo;
+        s.inv[a] = rows[a]->inv_dx;
+        s.spot[a] = static_cast<R>(spot[a]);
+    }
+    s.lf = reinterpret_cast<const R*>(dev + o_lf);
+    s.obs_step = reinterpret_cast<const int*>(dev + o_obs);
+    s.obs_of = reinterpret_cast<const int*>(dev + o_of);
+    s.disc = reinterpret_cast<const R*>(dev + o_disc);
+    s.pr = Terms<R>{static_cast<R>(rq.autocall), static_cast<R>(rq.coupon_barrier), static_cast<R>(rq.coupon),
+                    static_cast<R>(rq.ki_barrier), static_cast<R>(rq.put_strike), static_cast<R>(rq.smoothing),
+                    rq.memory};
+    s.greeks = adjoint ? 1 : 0;
+
+    const long long fwd_blocks = (n_legs + kFwdBlock - 1) / kFwdBlock;
+    const int n_seg = (ns + kSeg - 1) / kSeg;
+    const size_t align = 256;
+    auto up = [&](size_t b) { return (b + align - 1) & ~(align - 1); };
+    const size_t part_bytes = up(static_cast<size_t>(fwd_blocks) * 16 * sizeof(double));
+    double* partials = static_cast<double*>(scratch(kSlotPartials, part_bytes + 256 * sizeof(double)));
+    double* batches = partials + part_bytes / sizeof(double);
+    s.partials = partials;
+    R* xbar = nullptr;
+    if (adjoint) {
+        s.ck = static_cast<R*>(scratch(kSlotCheckpoints, static_cast<size_t>(n_seg) * n * n_legs * sizeof(R)));
+        const size_t perf_b = up(static_cast<size_t>(rq.n_obs) * n_legs * sizeof(R));
+        const size_t worst_b = up(static_cast<size_t>(rq.n_obs) * n_legs);
+        const size_t leg_b = up(static_cast<size_t>(n_legs) * sizeof(R));
+        const size_t lat_b = up(static_cast<size_t>(n_legs) * sizeof(int));
+        const size_t xbar_b = up(static_cast<size_t>(n) * n_legs * sizeof(R));
+        unsigned char* m = static_cast<unsigned char*>(scratch(kSlotSeeds, perf_b + worst_b + leg_b + lat_b + xbar_b));
+        s.perf = reinterpret_cast<R*>(m);
+        s.worst = m + perf_b;
+        s.lseed = reinterpret_cast<R*>(m + perf_b + worst_b);
+        s.lat = reinterpret_cast<int*>(m + perf_b + worst_b + leg_b);
+        xbar = reinterpret_cast<R*>(m + perf_b + worst_b + leg_b + lat_b);
+    }
+
+    Events ev;
+    hip_check(hipEventRecord(ev.e[0], stream), "event");
+    hipLaunchKernelGGL((autocall_forward<R, N>), dim3(static_cast<unsigned>(fwd_blocks)), dim3(kFwdBlock), 0, stream,
+                       s);
+    hip_check(hipGetLastError(), "autocall forward launch");
+    hipLaunchKernelGGL(batch_sums_kernel, dim3(1), dim3(kBatches * 16), 0, stream, partials, fwd_blocks, batches);
+    hip_check(hipGetLastError(), "batch sums launch");
+    hip_check(hipEventRecord(ev.e[1], stream), "event");
+
+    std::vector<double> node_out;
+    long long bwd_blocks = 0, n_chunks = 0;
+    if (adjoint) {
+        const size_t lds = static_cast<size_t>(kSeg) * N * n_x * sizeof(unsigned long long);
+        int max_lds = 0;
+        int dev_id = 0;
+        hip_check(hipGetDevice(&dev_id), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_id), "attribute");
+        if (lds > static_cast<size_t>(max_lds)) invalid("n_x too large for the adjoint's LDS rows");
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(autocall_backward<R, N>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
+                  "LDS attribute");
+        int per_cu = 0;
+        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, autocall_backward<R, N>, kBwdBlock, lds),
+                  "occupancy");
+        n_chunks = (n_legs + kBwdBlock - 1) / kBwdBlock;
+        bwd_blocks = std::min<long long>(n_chunks, static_cast<long long>(std::max(per_cu, 1)) * device_cu_count());
+        bwd_blocks = std::max<long long>(bwd_blocks, 1);
+        const long long entries = static_cast<long long>(n) * ns * n_x;
+        long long* partial = static_cast<long long*>(
+            scratch(kSlotAdjoint, static_cast<size_t>(bwd_blocks) * entries * sizeof(long long)));
+        long long* batch = static_cast<long long*>(
+            scratch(kSlotBatches, static_cast<size_t>(kBatches) * entries * sizeof(long long) +
+                                      static_cast<size_t>(n) * kBatches * n_nodes * sizeof(double)));
+        double* vega_dev = reinterpret_cast<double*>(batch + kBatches * entries);
+        hipLaunchKernelGGL((autocall_backward<R, N>), dim3(static_cast<unsigned>(bwd_blocks)), dim3(kBwdBlock), lds,
+                           stream, s, n_chunks, xbar, partial);
+        hip_check(hipGetLastError(), "autocall backward launch");
+        const unsigned rb = static_cast<unsigned>(std::min<long long>((entries + 255) / 256, 8192));
+        hipLaunchKernelGGL(table_batches_kernel, dim3(rb), dim3(256), 0, stream, partial, entries, bwd_blocks, batch);
+        hip_check(hipGetLastError(), "table batches launch");
+        for (int a = 0; a < n; ++a)
+            launch_node_contraction(*rows[a], grid, batch + static_cast<long long>(a) * ns * n_x, entries, kBatches,
+                                    1.0 / kFixed, vega_dev + static_cast<size_t>(a) * kBatches * n_nodes, stream);
+        node_out.resize(static_cast<size_t>(n) * kBatches * n_nodes);
+        hip_check(hipEventRecord(ev.e[2], stream), "event");
+        hip_check(hipMemcpyAsync(node_out.data(), vega_dev, node_out.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                 stream),
+                  "copy vegas");
+    } else {
+        hip_check(hipEventRecord(ev.e[2], stream), "event");
+    }
+    std::vector<double> bsum(kBatches * 16);
+    hip_check(hipMemcpyAsync(bsum.data(), batches, bsum.size() * sizeof(double), hipMemcpyDeviceToHost, stream),
+              "copy sums");
+    hip_check(hipStreamSynchronize(stream), "synchronize");
+
+    // price and deltas from pair averages; batch prices over the forward workgroups of each batch
+    double tot[16] = {0};
+    for (int b = 0; b < kBatches; ++b)
+        for (int k = 0; k < 16; ++k) tot[k] += bsum[b * 16 + k];
+    const double pairs = static_cast<double>(rq.n_pairs);
+    mean_se(tot[0], tot[1], pairs, rq.price, rq.se);
+    for (int b = 0; b < kBatches; ++b) {
+        const long long l0 = std::min(n_legs, (b * fwd_blocks / kBatches) * kFwdBlock);
+        const long long l1 = std::min(n_legs, ((b + 1) * fwd_blocks / kBatches) * kFwdBlock);
+        rq.batch[b] = l1 > l0 ? bsum[b * 16] / (0.5 * static_cast<double>(l1 - l0)) : 0.0;
+    }
+    for (int a = 0; a < kMaxNames; ++a) rq.delta[a] = rq.delta_se[a] = 0.0;
+    if (adjoint) {
+        for (int a = 0; a < n; ++a) mean_se(tot[2 + 2 * a], tot[3 + 2 * a], pairs, rq.delta[a], rq.delta_se[a]);
+        // vegas: sum over legs / n_legs, per vol point; SE by batch means weighted by each batch's legs
+        double legs_b[kBatches];
+        for (int b = 0; b < kBatches; ++b) {
+            const long long w0 = b * bwd_blocks / kBatches, w1 = (b + 1) * bwd_blocks / kBatches;
+            const long long l0 = std::min(n_legs, (w0 * n_chunks / bwd_blocks) * kBwdBlock);
+            const long long l1 = std::min(n_legs, (w1 * n_chunks / bwd_blocks) * kBwdBlock);
+            legs_b[b] = static_cast<double>(l1 - l0);
+        }
+        int used = 0;
+        for (int b = 0; b < kBatches; ++b) used += legs_b[b] > 0;
+        for (int a = 0; a < n; ++a)
+            for (int k = 0; k < n_nodes; ++k) {
+                double total = 0.0;
+                for (int b = 0; b < kBatches; ++b) total += node_out[(static_cast<size_t>(a) * kBatches + b) * n_nodes + k];
+                const double mean = 0.01 * total / static_cast<double>(n_legs);
+                double var = 0.0;
+                for (int b = 0; b < kBatches; ++b) {
+                    if (legs_b[b] <= 0) continue;
+                    const double vb = 0.01 * node_out[(static_cast<size_t>(a) * kBatches + b) * n_nodes + k] / legs_b[b];
+                    const double wgt = legs_b[b] / static_cast<double>(n_legs);
+                    var += wgt * wgt * (vb - mean) * (vb - mean);
+                }
+                if (rq.vega) rq.vega[a * n_nodes + k] = mean;
+                if (rq.vega_se) rq.vega_se[a * n_nodes + k] = used > 1 ? std::sqrt(var * used / (used - 1.0)) : 0.0;
+            }
+    }
+    rq.forward_ms = ev.ms(0, 1);
+    rq.backward_ms = adjoint ? ev.ms(1, 2) : 0.0;
+    rq.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+}
+
+template <typename R>
+void run_prec(optlab_autocall_request& rq, bool adjoint) {
+    switch (rq.n_names) {
+        case 1: return run_typed<R, 1>(rq, adjoint);
+        case 2: return run_typed<R, 2>(rq, adjoint);
+        case 3: return run_typed<R, 3>(rq, adjoint);
+        case 4: return run_typed<R, 4>(rq, adjoint);
+        default: return run_typed<R, 5>(rq, adjoint);
+    }
+}
+
+}  // namespace
+
+void check_autocall_request(const optlab_autocall_request& c, bool adjoint) {
+    if (c.n_names < 1 || c.n_names > kMaxNames) invalid("n_names must be 1 to 5");
+    if (c.n_steps < 1 || !(c.dt > 0)) invalid("n_steps must be >= 1 and dt > 0");
+    if (!c.surface || !c.root || !c.fwd || !c.fixing || !c.obs_step || !c.disc) invalid("null array");
+    if (c.n_obs < 1 || c.n_obs > kMaxObs) invalid("n_obs must be 1 to 64");
+    int prev = 0;
+    for (int j = 0; j < c.n_obs; ++j) {
+        if (c.obs_step[j] <= prev || c.obs_step[j] > c.n_steps) invalid("obs_step must increase within [1, n_steps]");
+        prev = c.obs_step[j];
+    }
+    if (prev != c.n_steps) invalid("the last observation must be at n_steps");
+    for (int a = 0; a < c.n_names; ++a) {
+        if (!(c.fixing[a] > 0)) invalid("fixings must be > 0");
+        for (int t = 0; t <= c.n_steps; ++t)
+            if (!(c.fwd[a * (c.n_steps + 1) + t] > 0)) invalid("forwards must be > 0");
+        if (c.spot && !(c.spot[a] > 0)) invalid("spot factors must be > 0");
+    }
+    if (!(c.smoothing >= 0)) invalid("smoothing must be >= 0");
+    if (c.n_pairs < 1 || c.first_pair < 0) invalid("n_pairs must be >= 1 and first_pair >= 0");
+    if (c.n_pairs > (1ll << 40)) invalid("n_pairs too large");
+    if (adjoint && c.nodes) invalid("adjoint Greeks are taken at the unbumped surfaces: nodes must be NULL");
+    int n_x = -1;
+    for (int a = 0; a < c.n_names; ++a) {
+        const SurfaceRows& r = surface_rows(c.surface[a]);
+        if (r.n_rows != c.n_steps) invalid("each surface needs one row per step");
+        if (n_x >= 0 && r.n_x != n_x) invalid("the surfaces must share n_x");
+        n_x = r.n_x;
+    }
+}
+
+void run_autocall(optlab_autocall_request& rq, int prec, bool adjoint) {
+    if (prec == 0)
+        run_prec<double>(rq, adjoint);
+    else
+        run_prec<float>(rq, adjoint);
+}
+
+}  // namespace optlab_mi

@@ -265,6 +265,57 @@ int optlab_strip(optlab_strip_request* rq, int backend) {
     });
 }
 
+int optlab_surface_upload(const optlab_surface_rows* rows, int* handle) {
+    return guarded([&] {
+        if (!rows || !handle) invalid("null argument");
+        need_device();
+        *handle = surface_upload(*rows);
+    });
+}
+
+int optlab_surface_free(int handle) {
+    return guarded([&] {
+        need_device();
+        surface_free(handle);
+    });
+}
+
+int optlab_surface_table(int handle, const double* nodes, double parallel, double skew, double* out) {
+    return guarded([&] {
+        if (!out) invalid("null output");
+        need_device();
+        surface_table(handle, nodes, parallel, skew, out);
+    });
+}
+
+int optlab_surface_node_grad(int handle, const double* tbar, double* out) {
+    return guarded([&] {
+        if (!tbar || !out) invalid("null argument");
+        need_device();
+        surface_node_grad(handle, tbar, out);
+    });
+}
+
+int optlab_autocall(optlab_autocall_request* rq, int backend) {
+    return guarded([&] {
+        if (!rq) invalid("null request");
+        const int prec = precision_of(backend);
+        need_device();
+        check_autocall_request(*rq, false);
+        run_autocall(*rq, prec, false);
+    });
+}
+
+int optlab_adjoint(optlab_autocall_request* rq, int backend) {
+    return guarded([&] {
+        if (!rq) invalid("null request");
+        const int prec = precision_of(backend);
+        need_device();
+        check_autocall_request(*rq, true);
+        run_autocall(*rq, prec, true);
+    });
+}
+
 int optlab_philox_dev(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1, long long n,
                       unsigned* out_dev, void* stream) {
     return guarded([&] {

@@ -22,7 +22,22 @@ void hip_check(hipError_t e, const char* what);
 
 // Grow-only device buffers reused across calls (calls on one device are serialised by contract).
 void* scratch(int slot, size_t bytes);
-enum Slot { kSlotSpec = 0, kSlotOut, kSlotPartials, kSlotTotals, kSlotParticles, kSlotStrip, kSlotAux, kSlotCount };
+enum Slot {
+    kSlotSpec = 0,
+    kSlotOut,
+    kSlotPartials,
+    kSlotTotals,
+    kSlotParticles,
+    kSlotStrip,
+    kSlotAux,
+    kSlotNodes,        // per-row node contractions
+    kSlotTables,       // bumped local-variance tables of one call
+    kSlotCheckpoints,  // adjoint: path state every few steps
+    kSlotSeeds,        // adjoint: payoff seeds and adjoint state
+    kSlotAdjoint,      // adjoint: per-workgroup table adjoints
+    kSlotBatches,      // adjoint: per-batch table adjoints
+    kSlotCount
+};
 
 int device_cu_count();
 
@@ -41,6 +56,14 @@ void run_strip(optlab_strip_request& rq, int prec);
 // ---- SLV particles (leverage.hip) -----------------------------------------------------------------------------
 void check_leverage_request(const optlab_leverage_request& rq);
 void run_leverage(optlab_leverage_request& rq, const optlab_leverage_options* opt, int prec);
+
+// ---- surfaces on the device (surface.hip) and the worst-of autocall (autocall.hip) -----------------------------
+int surface_upload(const optlab_surface_rows& rows);
+void surface_free(int handle);
+void surface_table(int handle, const double* nodes, double parallel, double skew, double* out_host);
+void surface_node_grad(int handle, const double* tbar_host, double* out_host);
+void check_autocall_request(const optlab_autocall_request& rq, bool adjoint);
+void run_autocall(optlab_autocall_request& rq, int prec, bool adjoint);
 
 // ---- test entry points (paths.hip) -----------------------------------------------------------------------------
 void launch_philox(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1, long long n,

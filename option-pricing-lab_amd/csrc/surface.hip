@@ -1,0 +1,417 @@
+// Local-variance tables of implied-vol surfaces bumped by the tents of the risk grid (and by a parallel shift and a
+// skew rotation), built on the device in FP64, and the exact derivative of every table entry with respect to the
+// grid's nodes contracted with a table adjoint.
+// Semantics: packages/risk/src/optlab/risk/grid.py:32-166 (strike tents: Gaussian-smoothed hats in log forward
+// moneyness; tenor tents: a smootherstep partition of unity), packages/risk/src/optlab/risk/surfaces.py:30-165
+// (sigma' = sigma + a, a = parallel + skew + sum_ij H_ij phi_i(k, T) psi_j(T); w = T sigma'^2 with exact w_k, w_kk,
+// w_T; Dupire on the base surface's table grid) and packages/structured/src/optlab/structured/surfaces.py:10-50
+// (the skew rotation).
+//
+// Layout: the base surface arrives as rows (one per simulation step) of w, w_k, w_kk, w_T on the engine's table
+// grid; the table kernel runs one thread per entry; the contraction runs one workgroup per row, its threads split
+// over (active node, x phase), each accumulating every batch of the adjoint.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "device.hpp"
+#include "engine.hpp"
+#include "surface.hpp"
+
+namespace optlab_mi {
+
+namespace {
+
+constexpr double kSkewWidth = 0.3, kSkewFloor = 1.0 / 12;
+constexpr int kMaxBatches = 16;
+
+struct Width {  // l(k, T) = c(k) sqrt(max(T, floor)) and its derivatives (grid.py:100-112)
+    double l, lk, lkk, lT;
+};
+
+__device__ Width smoothing_width(const TentGrid& g, double k, double T) {
+    const double root = sqrt(fmax(T, g.t_floor));
+    const double span = g.wing - g.atm, a2 = g.scale * g.scale;
+    const double e = exp(-k * k / a2);
+    const double c = g.wing - span * e;
+    const double ck = span * e * 2.0 * k / a2;
+    const double ckk = span * e * (2.0 / a2 - 4.0 * k * k / (a2 * a2));
+    return Width{c * root, ck * root, ckk * root, T < g.t_floor ? 0.0 : 0.5 * c / root};
+}
+
+struct Tent {
+    double f, fk, fkk, fT;
+};
+
+// Strike tent i at k: const + sum_m c_m R(k - a_m), R = u Phi(z) + l n(z), z = u / l, R_k = Phi + n l_k,
+// R_kk = n ((1 - z l_k)^2 / l + l_kk), R_T = n l_T (grid.py:115-140).
+__device__ Tent strike_tent(const TentGrid& g, int i, double k, const Width& w) {
+    Tent t{g.hat_const[i], 0.0, 0.0, 0.0};
+    for (int m = 0; m < g.hat_n[i]; ++m) {
+        const double u = k - g.hat_a[i][m], c = g.hat_c[i][m];
+        const double z = u / w.l;
+        const double pdf = exp(-0.5 * z * z) * 0.3989422804014327, cdf = normcdf(z);
+        const double q = 1.0 - z * w.lk;
+        t.f += c * (u * cdf + w.l * pdf);
+        t.fk += c * (cdf + pdf * w.lk);
+        t.fkk += c * (pdf * (q * q / w.l + w.lkk));
+        t.fT += c * (pdf * w.lT);
+    }
+    return t;
+}
+
+// The (at most two) tenor tents that are not zero at T, j0 and j0 + 1, and their T-derivatives (grid.py:143-162).
+struct Tenors {
+    int j0;
+    double psi[2], dpsi[2];
+};
+
+__device__ Tenors tenor_tents(const TentGrid& g, double T) {
+    Tenors r{0, {0.0, 0.0}, {0.0, 0.0}};
+    const int n = g.n_tenors;
+    if (T <= g.tenors[0]) {
+        r.psi[0] = 1.0;
+        return r;
+    }
+    if (T >= g.tenors[n - 1]) {
+        r.j0 = n - 1, r.psi[0] = 1.0;
+        return r;
+    }
+    int i = 0;
+    while (i + 1 < n && g.tenors[i + 1] <= T) ++i;
+    r.j0 = i;
+    if (T == g.tenors[i]) {
+        r.psi[0] = 1.0;
+        return r;
+    }
+    const double a = g.tenors[i], b = g.tenors[i + 1], u = (T - a) / (b - a);
+    const double s = u * u * u * (10.0 - 15.0 * u + 6.0 * u * u), ds = 30.0 * u * u * (1.0 - u) * (1.0 - u);
+    r.psi[0] = 1.0 - s, r.psi[1] = s;
+    r.dpsi[0] = -ds / (b - a), r.dpsi[1] = ds / (b - a);
+    return r;
+}
+
+struct Point {
+    double k, T, w, wk, wkk, wT;
+};
+
+__device__ Point base_point(const SurfaceRows& s, long long row, int x) {
+    const long long at = row * s.n_x + x;
+    return Point{s.lo[row] + x / s.inv_dx[row], s.T[row], s.w[at], s.wk[at], s.wkk[at], s.wT[at]};
+}
+
+struct Sigma {  // sigma and its derivatives from w (surfaces.py:37-43)
+    double s, sk, skk, sT;
+};
+
+__device__ Sigma sigma_of(const Point& p) {
+    const double s = sqrt(fmax(p.w, 1e-300) / p.T);
+    const double sk = p.wk / (2.0 * p.T * s);
+    const double skk = (p.wkk - 2.0 * p.T * sk * sk) / (2.0 * p.T * s);
+    const double sT = (p.wT - s * s) / (2.0 * p.T * s);
+    return Sigma{s, sk, skk, sT};
+}
+
+// Durrleman's g (core/svi.py:31-35); the local variance is clip(w_T / max(g, 1e-6), floor, cap).
+__device__ double g_function(double k, double w, double wk, double wkk) {
+    const double A = 1.0 - k * wk / (2.0 * w);
+    return A * A - wk * wk / 4.0 * (1.0 / w + 0.25) + wkk / 2.0;
+}
+
+// The skew rotation and its derivatives in k, k twice and T (structured/surfaces.py:29-50).
+__device__ Tent skew_bump(double skew, double k, double T) {
+    if (skew == 0.0) return Tent{0.0, 0.0, 0.0, 0.0};
+    const double norm = 1.0 / (tanh(-log(0.9) / kSkewWidth) + tanh(log(1.1) / kSkewWidth));
+    const double Tf = fmax(T, kSkewFloor);
+    const double scale = sqrt(1.0 / Tf), dscale = T < kSkewFloor ? 0.0 : -0.5 * scale / Tf;
+    const double a = -skew * norm, th = tanh(k / kSkewWidth), sech2 = 1.0 - th * th;
+    return Tent{a * scale * th, a * scale * sech2 / kSkewWidth,
+                -2.0 * a * scale * th * sech2 / (kSkewWidth * kSkewWidth), a * dscale * th};
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) bumped_table_kernel(SurfaceRows s, TentGrid g, const double* H, SurfaceMove mv,
+                                                           R* out, double floor, double cap) {
+    const long long n = s.n_rows * static_cast<long long>(s.n_x);
+    for (long long at = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; at < n;
+         at += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const long long row = at / s.n_x;
+        const int x = static_cast<int>(at - row * s.n_x);
+        const Point p = base_point(s, row, x);
+        const Sigma q = sigma_of(p);
+        const Tent sk = skew_bump(mv.skew, p.k, p.T);
+        double a = mv.parallel + sk.f, ak = sk.fk, akk = sk.fkk, aT = sk.fT;
+        if (H) {
+            const Width wd = smoothing_width(g, p.k, p.T);
+            const Tenors tn = tenor_tents(g, p.T);
+            for (int i = 0; i < g.n_strikes; ++i) {
+                double h = 0.0, hT = 0.0;
+                for (int jj = 0; jj < 2; ++jj) {
+                    const int j = tn.j0 + jj;
+                    if (j >= g.n_tenors) break;
+                    h += H[i * g.n_tenors + j] * tn.psi[jj];
+                    hT += H[i * g.n_tenors + j] * tn.dpsi[jj];
+                }
+                if (h == 0.0 && hT == 0.0) continue;
+                const Tent t = strike_tent(g, i, p.k, wd);
+                a += t.f * h, ak += t.fk * h, akk += t.fkk * h, aT += t.f * hT + t.fT * h;
+            }
+        }
+        const double s2 = q.s + a, sk2 = q.sk + ak, skk2 = q.skk + akk, sT2 = q.sT + aT;
+        const double w2 = p.T * s2 * s2, wk2 = 2.0 * p.T * s2 * sk2;
+        const double wkk2 = 2.0 * p.T * (sk2 * sk2 + s2 * skk2), wT2 = s2 * s2 + 2.0 * p.T * s2 * sT2;
+        const double gg = g_function(p.k, w2, wk2, wkk2);
+        out[at] = static_cast<R>(fmin(fmax(wT2 / fmax(gg, 1e-6), floor), cap));
+    }
+}
+
+// One workgroup per row. Thread t works on active node na = t / phases (strike na / 2, tenor slot na % 2) and on
+// the x with x % phases == t % phases; every batch's adjoint is read at each x, so the tent derivatives are
+// evaluated once per (x, node).
+template <typename Adj>
+__global__ void __launch_bounds__(256) node_contraction_kernel(SurfaceRows s, TentGrid g, const Adj* tbar,
+                                                               long long tbar_stride, int n_batches, double unit,
+                                                               double* out, double floor, double cap) {
+    __shared__ double part[256][kMaxBatches];
+    const long long row = blockIdx.x;
+    const int n_active = 2 * g.n_strikes;
+    const int phases = blockDim.x / n_active;
+    const int na = threadIdx.x / phases, phase = threadIdx.x % phases;
+    const bool live = na < n_active;
+    const int i = na / 2, jj = na % 2;
+    const Tenors tn = tenor_tents(g, s.T[row]);
+    const double psi = live ? tn.psi[jj] : 0.0, dpsi = live ? tn.dpsi[jj] : 0.0;
+    double acc[kMaxBatches];
+#pragma unroll
+    for (int b = 0; b < kMaxBatches; ++b) acc[b] = 0.0;
+    if (live && (psi != 0.0 || dpsi != 0.0)) {
+        for (int x = phase; x < s.n_x; x += phases) {
+            const Point p = base_point(s, row, x);
+            const double gg = g_function(p.k, p.w, p.wk, p.wkk);
+            const double lv_raw = p.wT / fmax(gg, 1e-6);
+            if (lv_raw <= floor || lv_raw >= cap) continue;  // clipped: the entry does not move
+            const Sigma q = sigma_of(p);
+            const Width wd = smoothing_width(g, p.k, p.T);
+            const Tent t = strike_tent(g, i, p.k, wd);
+            const double da = t.f * psi, dak = t.fk * psi, dakk = t.fkk * psi, daT = t.fT * psi + t.f * dpsi;
+            const double dw = 2.0 * p.T * q.s * da;
+            const double dwk = 2.0 * p.T * (q.sk * da + q.s * dak);
+            const double dwkk = 2.0 * p.T * (2.0 * q.sk * dak + q.skk * da + q.s * dakk);
+            const double dwT = 2.0 * q.s * da + 2.0 * p.T * (q.sT * da + q.s * daT);
+            double d;
+            if (gg > 1e-6) {
+                const double A = 1.0 - p.k * p.wk / (2.0 * p.w);
+                const double g_w = A * p.k * p.wk / (p.w * p.w) + p.wk * p.wk / (4.0 * p.w * p.w);
+                const double g_wk = -A * p.k / p.w - 0.5 * p.wk * (1.0 / p.w + 0.25);
+                d = (dwT - lv_raw * (g_w * dw + g_wk * dwk + 0.5 * dwkk)) / gg;
+            } else {
+                d = dwT / 1e-6;
+            }
+            const Adj* tb = tbar + row * s.n_x + x;
+#pragma unroll
+            for (int b = 0; b < kMaxBatches; ++b)
+                if (b < n_batches) acc[b] += static_cast<double>(tb[b * tbar_stride]) * unit * d;
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < kMaxBatches; ++b) part[threadIdx.x][b] = acc[b];
+    __syncthreads();
+    if (live && phase == 0) {
+        for (int b = 0; b < n_batches; ++b) {
+            double v = 0.0;
+            for (int ph = 0; ph < phases; ++ph) v += part[threadIdx.x + ph][b];
+            out[(static_cast<long long>(b) * s.n_rows + row) * n_active + na] = v;
+        }
+    }
+}
+
+// vega[b][i * n_tenors + j] = the sum over rows (in order) of each row's part of node (i, j).
+__global__ void node_sum_kernel(SurfaceRows s, TentGrid g, const double* per_row, int n_batches, double* vega) {
+    const int n_nodes = g.n_strikes * g.n_tenors, n_active = 2 * g.n_strikes;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_batches * n_nodes) return;
+    const int b = t / n_nodes, node = t % n_nodes, i = node / g.n_tenors, j = node % g.n_tenors;
+    double v = 0.0;
+    for (long long row = 0; row < s.n_rows; ++row) {
+        const Tenors tn = tenor_tents(g, s.T[row]);
+        const int jj = j - tn.j0;
+        if (jj < 0 || jj > 1) continue;
+        v += per_row[(static_cast<long long>(b) * s.n_rows + row) * n_active + i * 2 + jj];
+    }
+    vega[t] = v;
+}
+
+}  // namespace
+
+TentGrid make_tent_grid() {
+    // grid.py:25-26 and 73-77: strikes in forward moneyness, tenors in years, the smoothing constants
+    static const double strikes[] = {0.6, 0.7, 0.8, 0.9, 0.95, 1.0, 1.05, 1.1, 1.2, 1.4};
+    static const double tenors[] = {1.0 / 12, 0.25, 0.5, 1.0, 1.5, 2.0, 3.0, 5.0};
+    TentGrid g{};
+    g.n_strikes = 10, g.n_tenors = 8;
+    for (int i = 0; i < g.n_strikes; ++i) g.log_strikes[i] = std::log(strikes[i]);
+    for (int j = 0; j < g.n_tenors; ++j) g.tenors[j] = tenors[j];
+    g.atm = 0.04, g.wing = 0.12, g.scale = 0.15, g.t_floor = 1.0 / 52;
+    // each hat as const + sum of ramps (x - a)_+ (grid.py:80-97)
+    const double* x = g.log_strikes;
+    const int n = g.n_strikes;
+    for (int i = 0; i < n; ++i) {
+        if (i == 0) {
+            const double d = x[1] - x[0];
+            g.hat_const[i] = 1.0, g.hat_n[i] = 2;
+            g.hat_a[i][0] = x[0], g.hat_c[i][0] = -1 / d, g.hat_a[i][1] = x[1], g.hat_c[i][1] = 1 / d;
+        } else if (i == n - 1) {
+            const double d = x[n - 1] - x[n - 2];
+            g.hat_const[i] = 0.0, g.hat_n[i] = 2;
+            g.hat_a[i][0] = x[n - 2], g.hat_c[i][0] = 1 / d, g.hat_a[i][1] = x[n - 1], g.hat_c[i][1] = -1 / d;
+        } else {
+            const double dl = x[i] - x[i - 1], dr = x[i + 1] - x[i];
+            g.hat_const[i] = 0.0, g.hat_n[i] = 3;
+            g.hat_a[i][0] = x[i - 1], g.hat_c[i][0] = 1 / dl;
+            g.hat_a[i][1] = x[i], g.hat_c[i][1] = -1 / dl - 1 / dr;
+            g.hat_a[i][2] = x[i + 1], g.hat_c[i][2] = 1 / dr;
+        }
+    }
+    return g;
+}
+
+void launch_bumped_table(const SurfaceRows& s, const TentGrid& g, const double* H, SurfaceMove move, int prec,
+                         void* out, hipStream_t stream) {
+    const long long n = s.n_rows * static_cast<long long>(s.n_x);
+    const unsigned grid = static_cast<unsigned>(std::min<long long>((n + 255) / 256, 65536));
+    if (prec == 0)
+        hipLaunchKernelGGL(bumped_table_kernel<double>, dim3(grid), dim3(256), 0, stream, s, g, H, move,
+                           static_cast<double*>(out), 1e-4, 4.0);
+    else
+        hipLaunchKernelGGL(bumped_table_kernel<float>, dim3(grid), dim3(256), 0, stream, s, g, H, move,
+                           static_cast<float*>(out), 1e-4, 4.0);
+    hip_check(hipGetLastError(), "bumped table kernel launch");
+}
+
+template <typename A>
+void contract(const SurfaceRows& s, const TentGrid& g, const A* tbar, long long tbar_stride, int n_batches,
+              double unit, double* out, hipStream_t stream) {
+    if (n_batches < 1 || n_batches > kMaxBatches) invalid("n_batches must be 1 to 16");
+    const int n_active = 2 * g.n_strikes;
+    double* per_row = static_cast<double*>(
+        scratch(kSlotNodes, static_cast<size_t>(n_batches) * s.n_rows * n_active * sizeof(double)));
+    hipLaunchKernelGGL(node_contraction_kernel<A>, dim3(static_cast<unsigned>(s.n_rows)), dim3(256), 0, stream, s, g,
+                       tbar, tbar_stride, n_batches, unit, per_row, 1e-4, 4.0);
+    hip_check(hipGetLastError(), "node contraction kernel launch");
+    const int n_out = n_batches * g.n_strikes * g.n_tenors;
+    hipLaunchKernelGGL(node_sum_kernel, dim3((n_out + 127) / 128), dim3(128), 0, stream, s, g, per_row, n_batches,
+                       out);
+    hip_check(hipGetLastError(), "node sum kernel launch");
+}
+
+void launch_node_contraction(const SurfaceRows& s, const TentGrid& g, const long long* tbar, long long tbar_stride,
+                             int n_batches, double unit, double* out, hipStream_t stream) {
+    contract(s, g, tbar, tbar_stride, n_batches, unit, out, stream);
+}
+
+// ---- the registry of uploaded surfaces ----------------------------------------------------------------------------
+
+namespace {
+
+struct Held {
+    int device = -1;
+    SurfaceRows rows{};
+    void* mem = nullptr;
+    void* table[2] = {nullptr, nullptr};  // the unmoved local-variance table in double and in float, built once
+};
+std::vector<Held> g_surfaces;
+
+int current_device() {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    return dev;
+}
+
+}  // namespace
+
+int surface_upload(const optlab_surface_rows& r) {
+    if (r.n_rows < 1 || r.n_x < 2) invalid("a surface needs n_rows >= 1 and n_x >= 2");
+    if (!r.T || !r.lo || !r.inv_dx || !r.w || !r.w_k || !r.w_kk || !r.w_T) invalid("null surface array");
+    for (int i = 0; i < r.n_rows; ++i)
+        if (!(r.T[i] > 0) || !(r.inv_dx[i] > 0)) invalid("surface rows need T > 0 and inv_dx > 0");
+    const size_t nr = static_cast<size_t>(r.n_rows), ne = nr * r.n_x;
+    const size_t bytes = (3 * nr + 4 * ne) * sizeof(double);
+    Held h;
+    h.device = current_device();
+    hip_check(hipMalloc(&h.mem, bytes), "hipMalloc surface");
+    double* d = static_cast<double*>(h.mem);
+    const double* src[7] = {r.T, r.lo, r.inv_dx, r.w, r.w_k, r.w_kk, r.w_T};
+    const size_t len[7] = {nr, nr, nr, ne, ne, ne, ne};
+    const double* at[7];
+    for (int i = 0; i < 7; ++i) {
+        hip_check(hipMemcpy(d, src[i], len[i] * sizeof(double), hipMemcpyHostToDevice), "copy surface");
+        at[i] = d;
+        d += len[i];
+    }
+    h.rows = SurfaceRows{r.n_rows, r.n_x, at[0], at[1], at[2], at[3], at[4], at[5], at[6]};
+    for (size_t i = 0; i < g_surfaces.size(); ++i)
+        if (!g_surfaces[i].mem) {
+            g_surfaces[i] = h;
+            return static_cast<int>(i) + 1;
+        }
+    g_surfaces.push_back(h);
+    return static_cast<int>(g_surfaces.size());
+}
+
+static Held& held(int handle) {
+    if (handle < 1 || handle > static_cast<int>(g_surfaces.size()) || !g_surfaces[handle - 1].mem)
+        invalid("no such surface handle");
+    Held& h = g_surfaces[handle - 1];
+    if (h.device != current_device()) invalid("the surface was uploaded to another device");
+    return h;
+}
+
+void surface_free(int handle) {
+    Held& h = held(handle);
+    hip_check(hipDeviceSynchronize(), "synchronize before free");
+    for (void*& t : h.table)
+        if (t) hip_check(hipFree(t), "hipFree table"), t = nullptr;
+    hip_check(hipFree(h.mem), "hipFree surface");
+    h = Held{};
+}
+
+const SurfaceRows& surface_rows(int handle) { return held(handle).rows; }
+
+const void* surface_base_table(int handle, int prec, hipStream_t stream) {
+    Held& h = held(handle);
+    if (!h.table[prec]) {
+        const size_t n = static_cast<size_t>(h.rows.n_rows) * h.rows.n_x;
+        hip_check(hipMalloc(&h.table[prec], n * (prec == 0 ? sizeof(double) : sizeof(float))), "hipMalloc table");
+        launch_bumped_table(h.rows, make_tent_grid(), nullptr, SurfaceMove{0.0, 0.0}, prec, h.table[prec], stream);
+    }
+    return h.table[prec];
+}
+
+void surface_table(int handle, const double* nodes, double parallel, double skew, double* out_host) {
+    const SurfaceRows& s = held(handle).rows;
+    const size_t n = static_cast<size_t>(s.n_rows) * s.n_x;
+    const TentGrid g = make_tent_grid();
+    const size_t n_nodes = static_cast<size_t>(g.n_strikes) * g.n_tenors;
+    double* d_nodes = nullptr;
+    unsigned char* buf = static_cast<unsigned char*>(scratch(kSlotTables, n * sizeof(double) + n_nodes * sizeof(double)));
+    double* d_out = reinterpret_cast<double*>(buf);
+    if (nodes) {
+        d_nodes = d_out + n;
+        hip_check(hipMemcpy(d_nodes, nodes, n_nodes * sizeof(double), hipMemcpyHostToDevice), "copy nodes");
+    }
+    launch_bumped_table(s, g, d_nodes, SurfaceMove{parallel, skew}, 0, d_out, nullptr);
+    hip_check(hipMemcpy(out_host, d_out, n * sizeof(double), hipMemcpyDeviceToHost), "copy table");
+}
+
+void surface_node_grad(int handle, const double* tbar_host, double* out_host) {
+    const SurfaceRows& s = held(handle).rows;
+    const size_t n = static_cast<size_t>(s.n_rows) * s.n_x;
+    const TentGrid g = make_tent_grid();
+    const size_t n_nodes = static_cast<size_t>(g.n_strikes) * g.n_tenors;
+    double* buf = static_cast<double*>(scratch(kSlotTables, (n + n_nodes) * sizeof(double)));
+    hip_check(hipMemcpy(buf, tbar_host, n * sizeof(double), hipMemcpyHostToDevice), "copy tbar");
+    contract(s, g, buf, static_cast<long long>(n), 1, 1.0, buf + n, nullptr);
+    hip_check(hipMemcpy(out_host, buf + n, n_nodes * sizeof(double), hipMemcpyDeviceToHost), "copy node gradient");
+}
+
+}  // namespace optlab_mi

@@ -179,6 +179,65 @@ class StripRequest(C.Structure):
     ]
 
 
+class SurfaceRows(C.Structure):
+    """optlab_surface_rows (addition: a surface's rows on the engine's table grid, risk/surfaces.py:180-205)."""
+
+    _fields_ = [
+        ("n_rows", _I),
+        ("n_x", _I),
+        ("T", _PD),
+        ("lo", _PD),
+        ("inv_dx", _PD),
+        ("w", _PD),
+        ("w_k", _PD),
+        ("w_kk", _PD),
+        ("w_T", _PD),
+    ]
+
+
+GRID_STRIKES, GRID_TENORS = 10, 8
+GRID_NODES = GRID_STRIKES * GRID_TENORS
+
+
+class AutocallRequest(C.Structure):
+    """optlab_autocall_request (addition: the worst-of autocall and its adjoint Greeks)."""
+
+    _fields_ = [
+        ("n_names", _I),
+        ("n_steps", _I),
+        ("dt", _D),
+        ("surface", _PI),
+        ("root", _PD),
+        ("fwd", _PD),
+        ("fixing", _PD),
+        ("n_obs", _I),
+        ("obs_step", _PI),
+        ("disc", _PD),
+        ("autocall", _D),
+        ("coupon_barrier", _D),
+        ("coupon", _D),
+        ("memory", _I),
+        ("ki_barrier", _D),
+        ("put_strike", _D),
+        ("smoothing", _D),
+        ("n_pairs", _LL),
+        ("first_pair", _LL),
+        ("seed", _ULL),
+        ("nodes", _PD),
+        ("spot", _PD),
+        ("price", _D),
+        ("se", _D),
+        ("batch", _D * 16),
+        ("delta", _D * 5),
+        ("delta_se", _D * 5),
+        ("vega", _PD),
+        ("vega_se", _PD),
+        ("forward_ms", _D),
+        ("backward_ms", _D),
+        ("total_ms", _D),
+    ]
+
+
 def ptr(a, kind=_PD):
     """A ctypes pointer to a NumPy array's data (None for None or an empty array)."""
     if a is None or a.size == 0:
@@ -207,6 +266,12 @@ _SIGNATURES = {
     "optlab_leverage": (_I, [C.POINTER(LeverageRequest), _I, _I]),
     "optlab_leverage_ex": (_I, [C.POINTER(LeverageRequest), C.POINTER(LeverageOptions), _I]),
     "optlab_strip": (_I, [C.POINTER(StripRequest), _I]),
+    "optlab_surface_upload": (_I, [C.POINTER(SurfaceRows), _PI]),
+    "optlab_surface_free": (_I, [_I]),
+    "optlab_surface_table": (_I, [_I, _PD, _D, _D, _PD]),
+    "optlab_surface_node_grad": (_I, [_I, _PD, _PD]),
+    "optlab_autocall": (_I, [C.POINTER(AutocallRequest), _I]),
+    "optlab_adjoint": (_I, [C.POINTER(AutocallRequest), _I]),
     "optlab_philox_dev": (_I, [C.c_uint] * 6 + [_LL, C.c_void_p, C.c_void_p]),
     "optlab_normals_dev": (_I, [_LL, _LL, _I, C.c_uint, _ULL, _I, C.c_void_p, C.c_void_p]),
 }

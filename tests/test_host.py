@@ -50,6 +50,8 @@ def test_ctypes_layouts_match_the_c_header(tmp_path):
         "optlab_device": _abi.Device,
         "optlab_leverage_request": _abi.LeverageRequest,
         "optlab_strip_request": _abi.StripRequest,
+        "optlab_surface_rows": _abi.SurfaceRows,
+        "optlab_autocall_request": _abi.AutocallRequest,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for cname, py in structs.items():
@@ -122,6 +124,12 @@ def test_without_a_device_the_engine_refuses_loudly():
     with pytest.raises(engines.EngineError):
         engines.choose("gpu", BlackScholesModel(0.2))
     assert engines.choose("auto", BlackScholesModel(0.2)) == ("numpy", "no HIP device")
+    lib = _abi.load()
+    handle = C.c_int(0)
+    assert lib.optlab_surface_upload(C.byref(_abi.SurfaceRows()), C.byref(handle)) == _abi.EDEVICE
+    assert lib.optlab_autocall(C.byref(_abi.AutocallRequest()), 1) == _abi.EDEVICE
+    assert lib.optlab_adjoint(C.byref(_abi.AutocallRequest()), 0) == _abi.EDEVICE
+    assert lib.optlab_adjoint(C.byref(_abi.AutocallRequest()), 2) == _abi.EINVAL  # backend
 
 
 def test_the_engine_is_found_by_name():
