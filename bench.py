@@ -5,11 +5,13 @@ at 8 maturities (0.25..2 y) on 2^20 Heston paths (Andersen QE-M, psi_c 1.5; S0 1
 xi .6, rho -.7, r .03, q 0) of 504 uniform steps to T = 2 y, FP32 path state with compensated FP64 payoff sums.
 Inputs are parameters (nothing to stage in HBM); the outputs are 9 x 22 estimates.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--fp64-steps 3] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--fp64-steps 3] [--no-cpu] [--adjoint-paths 524288]
 
 Prints one JSON line: value = FP32 path-steps/s of the whole call (wall clock around the timed steps), the FP64
 figure and the FP32/FP64 ratio beside it, the roofline of the strip kernel (its average duration from HIP events
-recorded around it on its stream, by the library) and the CPU oracle timed on a bounded sample on this host."""
+recorded around it on its stream, by the library) and the CPU oracle timed on a bounded sample on this host; then the
+second half of the metric, the worst-of autocall's adjoint Greeks against bump-and-revalue at matched standard
+error ("adjoint")."""
 
 from __future__ import annotations
 
@@ -143,6 +145,78 @@ def _time_cpu(threads: int):
     return n * N_STEPS / reps[1], n
 
 
+# ---- the worst-of autocall adjoint workload ----------------------------------------------------------------------
+
+AC_SCALES, AC_RHOS = (0.03, 0.04, 0.05), (-0.5, -0.6, -0.7)
+
+
+def autocall_note():
+    """Worst-of Phoenix on 3 names under Dupire local vol from SSVI surfaces (theta_t = s t, s .03/.04/.05, rho
+    -.5/-.6/-.7, eta 1.2, gamma .5, pillars 0.05..3 y), correlation 0.6, 3 y quarterly, 756 daily steps, autocall
+    100 %, coupon 8 % p.a. at a 70 % barrier with memory, knock-in put at 60 % on every close, ramps of 0.5 %;
+    flat markets S0 100, r .03, q 0."""
+    import numpy as np
+
+    from optlab.core.market import Market
+    from optlab.core.ssvi import SSVI
+    from optlab.native.adjoint import WorstOfAutocall
+
+    pillars = np.array(SLV_PILLARS)
+    surfaces = [SSVI.power_law(r, 1.2, 0.5, pillars, s * pillars) for s, r in zip(AC_SCALES, AC_RHOS)]
+    corr = np.full((3, 3), 0.6) + 0.4 * np.eye(3)
+    return WorstOfAutocall(surfaces, [Market.flat(100.0, 0.03, 0.0)] * 3, corr)
+
+
+def bench_adjoint(n_paths: int, seed: int = 42, reps: int = 3) -> dict:
+    """Adjoint Greeks (3 deltas + 240 node vegas) against bump-and-revalue of all 486 bumps on common random numbers,
+    both in FP32 on the same engine; the speedup at matched standard error is t_bump / t_adjoint times the median over
+    nodes of (SE_bump / SE_adjoint)^2 (the paths the bumps need for the adjoint's SE)."""
+    import numpy as np
+
+    note = autocall_note()
+    note.adjoint(n_paths, seed)  # warm-up: tables built, buffers sized
+    note.price(n_paths, seed)
+    walls, fwd, bwd = [], [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        g = note.adjoint(n_paths, seed)
+        walls.append(time.perf_counter() - t0)
+        fwd.append(g.seconds["forward_ms"])
+        bwd.append(g.seconds["backward_ms"])
+    t_adj = sorted(walls)[reps // 2]
+    t0 = time.perf_counter()
+    b = note.bump(n_paths, seed)
+    t_bump = time.perf_counter() - t0
+    live = np.abs(g.vega) > 3 * g.vega_se
+    ratio = float(np.median((b.vega_se[live] / g.vega_se[live]) ** 2)) if live.any() else float("nan")
+    moving = np.hypot(g.vega_se, b.vega_se) > 0  # nodes beyond the last date move nothing: both vegas are 0
+    z = np.abs(g.vega - b.vega)[moving] / np.hypot(g.vega_se, b.vega_se)[moving]
+    zd = np.abs(g.delta - b.delta) / np.hypot(g.delta_se, b.delta_se)
+    name_steps = n_paths * note.n_steps * note.n_names
+    note.close()
+    return {
+        "workload": "worst-of Phoenix, 3 names, local vol from 3 SSVI surfaces, corr 0.6, 3 y quarterly, 756 daily "
+                    f"steps, {n_paths} paths, FP32 paths; 3 deltas + 3 x 80 node vegas",
+        "price": g.price,
+        "se": g.se,
+        "adjoint_ms": t_adj * 1e3,
+        "adjoint_forward_kernel_ms": float(np.median(fwd)),
+        "adjoint_backward_ms": float(np.median(bwd)),
+        "bump_ms": t_bump * 1e3,
+        "revaluations": b.seconds["revaluations"],
+        "speedup_time": t_bump / t_adj,
+        "se_ratio_sq_median": ratio,
+        "speedup_matched_se": t_bump / t_adj * ratio,
+        "moving_nodes": int(moving.sum()),
+        "moving_nodes_within_4se": float(np.mean(z < 4)),
+        "max_z_vega": float(z.max()),
+        "max_z_delta": float(zd.max()),
+        "delta": g.delta.tolist(),
+        "delta_se": g.delta_se.tolist(),
+        "forward_name_steps_per_s": name_steps / (float(np.median(fwd)) * 1e-3),
+    }
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +224,8 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--adjoint-paths", type=int, default=1 << 19,
+                    help="paths of the worst-of autocall adjoint-vs-bump comparison (0: skip it)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the timed region, one device-to-device copy of this many MiB: the known-size kernel "
                          "that scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -229,6 +305,8 @@ def main(argv=None) -> int:
                       f"oracle/oracle.cpp on {threads} threads",
         }
         line["gpu_over_cpu"] = v32 / cpu_rate
+    if args.adjoint_paths:
+        line["adjoint"] = bench_adjoint(args.adjoint_paths)
     print(json.dumps(line), flush=True)
     if args.copy_mib:
         src = torch.ones(args.copy_mib << 20, dtype=torch.uint8, device="cuda")

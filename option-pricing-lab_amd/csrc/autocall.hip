@@ -409,16 +409,28 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
     }
 }
 
-// batch[b][e] = sum over the backward workgroups of batch b (in order) of partial[w][e], e < entries.
-__global__ void table_batches_kernel(const long long* partial, long long entries, long long n_blocks, long long* batch) {
-    for (long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; e < entries;
-         e += static_cast<long long>(gridDim.x) * blockDim.x) {
-        for (int b = 0; b < kBatches; ++b) {
-            const long long w0 = b * n_blocks / kBatches, w1 = (b + 1) * n_blocks / kBatches;
-            unsigned long long t = 0;
-            for (long long w = w0; w < w1; ++w) t += static_cast<unsigned long long>(partial[w * entries + e]);
-            batch[b * entries + e] = static_cast<long long>(t);
+// batch[e][b] = sum over the backward workgroups of batch b (in order) of partial[w][e], e < entries: the 16
+// batches of an entry are contiguous (one 128-byte line), as the node contraction reads them. A workgroup takes 256
+// consecutive entries: reads are coalesced over e, the tile is transposed in LDS and written out contiguously.
+__global__ void __launch_bounds__(256) table_batches_kernel(const long long* partial, long long entries,
+                                                           long long n_blocks, long long* batch) {
+    __shared__ long long tile[256 * kBatches];
+    for (long long base = static_cast<long long>(blockIdx.x) * 256; base < entries;
+         base += static_cast<long long>(gridDim.x) * 256) {
+        const long long e = base + threadIdx.x;
+        if (e < entries) {
+            for (int b = 0; b < kBatches; ++b) {
+                const long long w0 = b * n_blocks / kBatches, w1 = (b + 1) * n_blocks / kBatches;
+                unsigned long long t = 0;  // integer sums: any order gives the same bits, so loads may overlap
+#pragma unroll 8
+                for (long long w = w0; w < w1; ++w) t += static_cast<unsigned long long>(partial[w * entries + e]);
+                tile[threadIdx.x * kBatches + b] = static_cast<long long>(t);
+            }
         }
+        __syncthreads();
+        const long long n_here = std::min<long long>(256, entries - base) * kBatches;
+        for (long long k = threadIdx.x; k < n_here; k += 256) batch[base * kBatches + k] = tile[k];
+        __syncthreads();
     }
 }
 
@@ -609,9 +621,14 @@ void run_typed(optlab_autocall_request& rq, bool adjoint) {
         const unsigned rb = static_cast<unsigned>(std::min<long long>((entries + 255) / 256, 8192));
         hipLaunchKernelGGL(table_batches_kernel, dim3(rb), dim3(256), 0, stream, partial, entries, bwd_blocks, batch);
         hip_check(hipGetLastError(), "table batches launch");
-        for (int a = 0; a < n; ++a)
-            launch_node_contraction(*rows[a], grid, batch + static_cast<long long>(a) * ns * n_x, entries, kBatches,
-                                    1.0 / kFixed, vega_dev + static_cast<size_t>(a) * kBatches * n_nodes, stream);
+        SurfaceSet set{};
+        DerivSet dset{};
+        set.n = n;
+        for (int a = 0; a < n; ++a) {
+            set.s[a] = *rows[a];
+            dset.d[a] = surface_node_derivatives(rq.surface[a], stream);
+        }
+        launch_node_contraction(set, dset, grid, batch, kBatches, 1.0 / kFixed, vega_dev, stream);
         node_out.resize(static_cast<size_t>(n) * kBatches * n_nodes);
         hip_check(hipEventRecord(ev.e[2], stream), "event");
         hip_check(hipMemcpyAsync(node_out.data(), vega_dev, node_out.size() * sizeof(double), hipMemcpyDeviceToHost,

@@ -44,11 +44,27 @@ struct Tent {
 };
 
 // Strike tent i at k: const + sum_m c_m R(k - a_m), R = u Phi(z) + l n(z), z = u / l, R_k = Phi + n l_k,
-// R_kk = n ((1 - z l_k)^2 / l + l_kk), R_T = n l_T (grid.py:115-140).
-__device__ Tent strike_tent(const TentGrid& g, int i, double k, const Width& w) {
-    Tent t{g.hat_const[i], 0.0, 0.0, 0.0};
-    for (int m = 0; m < g.hat_n[i]; ++m) {
-        const double u = k - g.hat_a[i][m], c = g.hat_c[i][m];
+// R_kk = n ((1 - z l_k)^2 / l + l_kk), R_T = n l_T (grid.py:115-140); the hat's constants read into registers once.
+struct Hat {
+    double c0;
+    int n;
+    double a[3], c[3];
+};
+
+__device__ __forceinline__ Hat hat_of(const TentGrid& g, int i) {
+    Hat h{g.hat_const[i], g.hat_n[i], {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+        if (m < h.n) h.a[m] = g.hat_a[i][m], h.c[m] = g.hat_c[i][m];
+    return h;
+}
+
+__device__ __forceinline__ Tent hat_tent(const Hat& h, double k, const Width& w) {
+    Tent t{h.c0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        if (m >= h.n) break;
+        const double u = k - h.a[m], c = h.c[m];
         const double z = u / w.l;
         const double pdf = exp(-0.5 * z * z) * 0.3989422804014327, cdf = normcdf(z);
         const double q = 1.0 - z * w.lk;
@@ -153,7 +169,7 @@ __global__ void __launch_bounds__(256) bumped_table_kernel(SurfaceRows s, TentGr
                     hT += H[i * g.n_tenors + j] * tn.dpsi[jj];
                 }
                 if (h == 0.0 && hT == 0.0) continue;
-                const Tent t = strike_tent(g, i, p.k, wd);
+                const Tent t = hat_tent(hat_of(g, i), p.k, wd);
                 a += t.f * h, ak += t.fk * h, akk += t.fkk * h, aT += t.f * hT + t.fT * h;
             }
         }
@@ -165,80 +181,97 @@ __global__ void __launch_bounds__(256) bumped_table_kernel(SurfaceRows s, TentGr
     }
 }
 
-// One workgroup per row. Thread t works on active node na = t / phases (strike na / 2, tenor slot na % 2) and on
-// the x with x % phases == t % phases; every batch's adjoint is read at each x, so the tent derivatives are
-// evaluated once per (x, node).
-template <typename Adj>
-__global__ void __launch_bounds__(256) node_contraction_kernel(SurfaceRows s, TentGrid g, const Adj* tbar,
-                                                               long long tbar_stride, int n_batches, double unit,
-                                                               double* out, double floor, double cap) {
-    __shared__ double part[256][kMaxBatches];
-    const long long row = blockIdx.x;
+// One workgroup per (row, surface), one wave per strike tent; lane l takes the x with x % 64 == l. At each x the
+// wave evaluates its strike's tent once and the entry's derivative for the two tenor tents that are not zero at the
+// row's maturity, then multiplies it into every batch of the adjoint (the batches of one entry are contiguous).
+// The wave sums its lanes with shuffles in a fixed order.
+// D[(row * n_x + x) * n_active + 2 i + jj] = dLV[row][x] / dH at H = 0 for strike tent i and tenor tent j0(row) + jj
+// (the two tenor tents that are not zero at the row's maturity); zero where the entry is clipped. Computed once per
+// surface and kept with it: the base surface does not move between adjoint calls.
+__global__ void __launch_bounds__(256) node_derivative_kernel(SurfaceRows s, TentGrid g, double* D, double floor,
+                                                              double cap) {
+    const long long n = s.n_rows * static_cast<long long>(s.n_x);
     const int n_active = 2 * g.n_strikes;
-    const int phases = blockDim.x / n_active;
-    const int na = threadIdx.x / phases, phase = threadIdx.x % phases;
-    const bool live = na < n_active;
-    const int i = na / 2, jj = na % 2;
-    const Tenors tn = tenor_tents(g, s.T[row]);
-    const double psi = live ? tn.psi[jj] : 0.0, dpsi = live ? tn.dpsi[jj] : 0.0;
-    double acc[kMaxBatches];
-#pragma unroll
-    for (int b = 0; b < kMaxBatches; ++b) acc[b] = 0.0;
-    if (live && (psi != 0.0 || dpsi != 0.0)) {
-        for (int x = phase; x < s.n_x; x += phases) {
-            const Point p = base_point(s, row, x);
-            const double gg = g_function(p.k, p.w, p.wk, p.wkk);
-            const double lv_raw = p.wT / fmax(gg, 1e-6);
-            if (lv_raw <= floor || lv_raw >= cap) continue;  // clipped: the entry does not move
-            const Sigma q = sigma_of(p);
-            const Width wd = smoothing_width(g, p.k, p.T);
-            const Tent t = strike_tent(g, i, p.k, wd);
-            const double da = t.f * psi, dak = t.fk * psi, dakk = t.fkk * psi, daT = t.fT * psi + t.f * dpsi;
-            const double dw = 2.0 * p.T * q.s * da;
-            const double dwk = 2.0 * p.T * (q.sk * da + q.s * dak);
-            const double dwkk = 2.0 * p.T * (2.0 * q.sk * dak + q.skk * da + q.s * dakk);
-            const double dwT = 2.0 * q.s * da + 2.0 * p.T * (q.sT * da + q.s * daT);
-            double d;
-            if (gg > 1e-6) {
-                const double A = 1.0 - p.k * p.wk / (2.0 * p.w);
-                const double g_w = A * p.k * p.wk / (p.w * p.w) + p.wk * p.wk / (4.0 * p.w * p.w);
-                const double g_wk = -A * p.k / p.w - 0.5 * p.wk * (1.0 / p.w + 0.25);
-                d = (dwT - lv_raw * (g_w * dw + g_wk * dwk + 0.5 * dwkk)) / gg;
-            } else {
-                d = dwT / 1e-6;
-            }
-            const Adj* tb = tbar + row * s.n_x + x;
-#pragma unroll
-            for (int b = 0; b < kMaxBatches; ++b)
-                if (b < n_batches) acc[b] += static_cast<double>(tb[b * tbar_stride]) * unit * d;
+    for (long long at = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; at < n;
+         at += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const long long row = at / s.n_x;
+        const int x = static_cast<int>(at - row * s.n_x);
+        double* out = D + at * n_active;
+        const Point p = base_point(s, row, x);
+        const double gg = g_function(p.k, p.w, p.wk, p.wkk);
+        const double lv_raw = p.wT / fmax(gg, 1e-6);
+        if (lv_raw <= floor || lv_raw >= cap) {  // clipped: the entry does not move
+            for (int na = 0; na < n_active; ++na) out[na] = 0.0;
+            continue;
         }
-    }
-#pragma unroll
-    for (int b = 0; b < kMaxBatches; ++b) part[threadIdx.x][b] = acc[b];
-    __syncthreads();
-    if (live && phase == 0) {
-        for (int b = 0; b < n_batches; ++b) {
-            double v = 0.0;
-            for (int ph = 0; ph < phases; ++ph) v += part[threadIdx.x + ph][b];
-            out[(static_cast<long long>(b) * s.n_rows + row) * n_active + na] = v;
+        const Sigma q = sigma_of(p);
+        const Width wd = smoothing_width(g, p.k, p.T);
+        const Tenors tn = tenor_tents(g, p.T);
+        const double A = 1.0 - p.k * p.wk / (2.0 * p.w);
+        const double g_w = A * p.k * p.wk / (p.w * p.w) + p.wk * p.wk / (4.0 * p.w * p.w);
+        const double g_wk = -A * p.k / p.w - 0.5 * p.wk * (1.0 / p.w + 0.25);
+        for (int i = 0; i < g.n_strikes; ++i) {
+            const Tent t = hat_tent(hat_of(g, i), p.k, wd);
+            for (int jj = 0; jj < 2; ++jj) {
+                const double psi = tn.psi[jj], dpsi = tn.dpsi[jj];
+                const double da = t.f * psi, dak = t.fk * psi, dakk = t.fkk * psi, daT = t.fT * psi + t.f * dpsi;
+                const double dw = 2.0 * p.T * q.s * da;
+                const double dwk = 2.0 * p.T * (q.sk * da + q.s * dak);
+                const double dwkk = 2.0 * p.T * (2.0 * q.sk * dak + q.skk * da + q.s * dakk);
+                const double dwT = 2.0 * q.s * da + 2.0 * p.T * (q.sT * da + q.s * daT);
+                out[2 * i + jj] =
+                    gg > 1e-6 ? (dwT - lv_raw * (g_w * dw + g_wk * dwk + 0.5 * dwkk)) / gg : dwT / 1e-6;
+            }
         }
     }
 }
 
-// vega[b][i * n_tenors + j] = the sum over rows (in order) of each row's part of node (i, j).
-__global__ void node_sum_kernel(SurfaceRows s, TentGrid g, const double* per_row, int n_batches, double* vega) {
-    const int n_nodes = g.n_strikes * g.n_tenors, n_active = 2 * g.n_strikes;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_batches * n_nodes) return;
-    const int b = t / n_nodes, node = t % n_nodes, i = node / g.n_tenors, j = node % g.n_tenors;
-    double v = 0.0;
-    for (long long row = 0; row < s.n_rows; ++row) {
-        const Tenors tn = tenor_tents(g, s.T[row]);
-        const int jj = j - tn.j0;
-        if (jj < 0 || jj > 1) continue;
-        v += per_row[(static_cast<long long>(b) * s.n_rows + row) * n_active + i * 2 + jj];
+// One workgroup per (row, surface); thread (b, na) sums over x, in order, the batch-b adjoint of the entry times its
+// derivative for active node na. Lanes of one batch read one adjoint value (a broadcast) and consecutive
+// derivatives (coalesced).
+template <typename Adj>
+__global__ void __launch_bounds__(256) node_contraction_kernel(SurfaceSet set, DerivSet dset, int n_active,
+                                                               const Adj* tbar, int n_batches, double unit,
+                                                               double* out) {
+    const long long row = blockIdx.x;
+    const int a = blockIdx.y;
+    const int n_x = set.s[a].n_x;
+    const long long n_rows = set.s[a].n_rows;
+    for (int t = threadIdx.x; t < n_batches * n_active; t += blockDim.x) {
+        const int b = t / n_active, na = t - b * n_active;
+        const double* d = dset.d[a] + row * n_x * n_active + na;
+        const Adj* tb = tbar + (static_cast<long long>(a) * n_rows + row) * n_x * n_batches + b;
+        // four partial sums over x % 4 (loads in flight), added in a fixed order
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        int x = 0;
+        for (; x + 4 <= n_x; x += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                acc[u] += static_cast<double>(tb[static_cast<long long>(x + u) * n_batches]) * d[(x + u) * n_active];
+        }
+        for (; x < n_x; ++x) acc[x & 3] += static_cast<double>(tb[static_cast<long long>(x) * n_batches]) * d[x * n_active];
+        out[((static_cast<long long>(a) * n_batches + b) * n_rows + row) * n_active + na] =
+            ((acc[0] + acc[1]) + (acc[2] + acc[3])) * unit;
     }
-    vega[t] = v;
+}
+
+// vega[(a * n_batches + b) * n_nodes + node] = the sum over rows of each row's part of node (i, j): one wave per
+// (node, batch, surface), lanes striding the rows, then a fixed-order shuffle sum.
+__global__ void __launch_bounds__(64) node_sum_kernel(SurfaceSet set, TentGrid g, const double* per_row,
+                                                      int n_batches, double* vega) {
+    const int node = blockIdx.x, b = blockIdx.y, a = blockIdx.z, lane = threadIdx.x;
+    const int n_nodes = g.n_strikes * g.n_tenors, n_active = 2 * g.n_strikes;
+    const int i = node / g.n_tenors, j = node % g.n_tenors;
+    const SurfaceRows s = set.s[a];
+    const double* pr = per_row + (static_cast<long long>(a) * n_batches + b) * s.n_rows * n_active;
+    double v = 0.0;
+    for (long long row = lane; row < s.n_rows; row += 64) {
+        const int jj = j - tenor_tents(g, s.T[row]).j0;
+        if (jj == 0 || jj == 1) v += pr[row * n_active + 2 * i + jj];
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += shfl_xor_d(v, m);
+    if (lane == 0) vega[(static_cast<long long>(a) * n_batches + b) * n_nodes + node] = v;
 }
 
 }  // namespace
@@ -289,24 +322,27 @@ void launch_bumped_table(const SurfaceRows& s, const TentGrid& g, const double* 
 }
 
 template <typename A>
-void contract(const SurfaceRows& s, const TentGrid& g, const A* tbar, long long tbar_stride, int n_batches,
+void contract(const SurfaceSet& set, const DerivSet& dset, const TentGrid& g, const A* tbar, int n_batches,
               double unit, double* out, hipStream_t stream) {
     if (n_batches < 1 || n_batches > kMaxBatches) invalid("n_batches must be 1 to 16");
+    if (set.n < 1 || set.n > kMaxSurfaces) invalid("1 to 5 surfaces");
+    for (int a = 1; a < set.n; ++a)
+        if (set.s[a].n_rows != set.s[0].n_rows || set.s[a].n_x != set.s[0].n_x) invalid("surfaces of one shape");
     const int n_active = 2 * g.n_strikes;
+    const long long n_rows = set.s[0].n_rows;
     double* per_row = static_cast<double*>(
-        scratch(kSlotNodes, static_cast<size_t>(n_batches) * s.n_rows * n_active * sizeof(double)));
-    hipLaunchKernelGGL(node_contraction_kernel<A>, dim3(static_cast<unsigned>(s.n_rows)), dim3(256), 0, stream, s, g,
-                       tbar, tbar_stride, n_batches, unit, per_row, 1e-4, 4.0);
+        scratch(kSlotNodes, static_cast<size_t>(set.n) * n_batches * n_rows * n_active * sizeof(double)));
+    hipLaunchKernelGGL(node_contraction_kernel<A>, dim3(static_cast<unsigned>(n_rows), set.n), dim3(256), 0, stream,
+                       set, dset, n_active, tbar, n_batches, unit, per_row);
     hip_check(hipGetLastError(), "node contraction kernel launch");
-    const int n_out = n_batches * g.n_strikes * g.n_tenors;
-    hipLaunchKernelGGL(node_sum_kernel, dim3((n_out + 127) / 128), dim3(128), 0, stream, s, g, per_row, n_batches,
-                       out);
+    hipLaunchKernelGGL(node_sum_kernel, dim3(g.n_strikes * g.n_tenors, n_batches, set.n), dim3(64), 0, stream, set, g,
+                       per_row, n_batches, out);
     hip_check(hipGetLastError(), "node sum kernel launch");
 }
 
-void launch_node_contraction(const SurfaceRows& s, const TentGrid& g, const long long* tbar, long long tbar_stride,
+void launch_node_contraction(const SurfaceSet& set, const DerivSet& dset, const TentGrid& g, const long long* tbar,
                              int n_batches, double unit, double* out, hipStream_t stream) {
-    contract(s, g, tbar, tbar_stride, n_batches, unit, out, stream);
+    contract(set, dset, g, tbar, n_batches, unit, out, stream);
 }
 
 // ---- the registry of uploaded surfaces ----------------------------------------------------------------------------
@@ -318,6 +354,7 @@ struct Held {
     SurfaceRows rows{};
     void* mem = nullptr;
     void* table[2] = {nullptr, nullptr};  // the unmoved local-variance table in double and in float, built once
+    double* deriv = nullptr;              // its node derivatives, built once
 };
 std::vector<Held> g_surfaces;
 
@@ -371,6 +408,7 @@ void surface_free(int handle) {
     hip_check(hipDeviceSynchronize(), "synchronize before free");
     for (void*& t : h.table)
         if (t) hip_check(hipFree(t), "hipFree table"), t = nullptr;
+    if (h.deriv) hip_check(hipFree(h.deriv), "hipFree derivatives"), h.deriv = nullptr;
     hip_check(hipFree(h.mem), "hipFree surface");
     h = Held{};
 }
@@ -385,6 +423,19 @@ const void* surface_base_table(int handle, int prec, hipStream_t stream) {
         launch_bumped_table(h.rows, make_tent_grid(), nullptr, SurfaceMove{0.0, 0.0}, prec, h.table[prec], stream);
     }
     return h.table[prec];
+}
+
+const double* surface_node_derivatives(int handle, hipStream_t stream) {
+    Held& h = held(handle);
+    if (!h.deriv) {
+        const TentGrid g = make_tent_grid();
+        const long long n = static_cast<long long>(h.rows.n_rows) * h.rows.n_x;
+        hip_check(hipMalloc(&h.deriv, static_cast<size_t>(n) * 2 * g.n_strikes * sizeof(double)), "hipMalloc deriv");
+        const unsigned grid = static_cast<unsigned>(std::min<long long>((n + 255) / 256, 65536));
+        hipLaunchKernelGGL(node_derivative_kernel, dim3(grid), dim3(256), 0, stream, h.rows, g, h.deriv, 1e-4, 4.0);
+        hip_check(hipGetLastError(), "node derivative kernel launch");
+    }
+    return h.deriv;
 }
 
 void surface_table(int handle, const double* nodes, double parallel, double skew, double* out_host) {
@@ -410,7 +461,12 @@ void surface_node_grad(int handle, const double* tbar_host, double* out_host) {
     const size_t n_nodes = static_cast<size_t>(g.n_strikes) * g.n_tenors;
     double* buf = static_cast<double*>(scratch(kSlotTables, (n + n_nodes) * sizeof(double)));
     hip_check(hipMemcpy(buf, tbar_host, n * sizeof(double), hipMemcpyHostToDevice), "copy tbar");
-    contract(s, g, buf, static_cast<long long>(n), 1, 1.0, buf + n, nullptr);
+    SurfaceSet set{};
+    set.s[0] = s;
+    set.n = 1;
+    DerivSet dset{};
+    dset.d[0] = surface_node_derivatives(handle, nullptr);
+    contract(set, dset, g, buf, 1, 1.0, buf + n, nullptr);
     hip_check(hipMemcpy(out_host, buf + n, n_nodes * sizeof(double), hipMemcpyDeviceToHost), "copy node gradient");
 }
 

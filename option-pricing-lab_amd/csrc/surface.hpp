@@ -47,14 +47,29 @@ TentGrid make_tent_grid();
 void launch_bumped_table(const SurfaceRows& s, const TentGrid& g, const double* H, SurfaceMove move, int prec,
                          void* out, hipStream_t stream);
 
-// out[((b * n_rows) + row) * n_nodes + node] = sum_x tbar_b[row][x] dLV[row][x] / dH_node at H = 0, for the
-// n_batches tables tbar_b = tbar + b * tbar_stride, each n_rows x n_x fixed-point integers of weight `unit`.
-void launch_node_contraction(const SurfaceRows& s, const TentGrid& g, const long long* tbar, long long tbar_stride,
+constexpr int kMaxSurfaces = 5;
+
+struct SurfaceSet {  // surfaces of one shape (n_rows, n_x), contracted in one launch
+    SurfaceRows s[kMaxSurfaces];
+    int n;
+};
+
+struct DerivSet {  // each surface's node derivatives (surface_node_derivatives)
+    const double* d[kMaxSurfaces];
+};
+
+// out[(a * n_batches + b) * n_nodes + node] = sum over (row, x) of tbar[a][row][x][b] * unit * dLV_a[row][x] /
+// dH_node at H = 0: the adjoint of surface a's table (n_rows x n_x entries, each with its n_batches batches
+// contiguous, fixed-point integers of weight `unit`) contracted with the table's node derivatives.
+void launch_node_contraction(const SurfaceSet& set, const DerivSet& dset, const TentGrid& g, const long long* tbar,
                              int n_batches, double unit, double* out, hipStream_t stream);
 
 // The rows of an uploaded surface (handle from surface_upload) and its unmoved local-variance table in precision
 // prec (0 double, 1 float), built on first use and kept with the surface.
 const SurfaceRows& surface_rows(int handle);
 const void* surface_base_table(int handle, int prec, hipStream_t stream);
+// dLV / dH for the two active tenor tents of each row and every strike tent, (n_rows x n_x) x 2 n_strikes doubles,
+// built on first use and kept with the surface.
+const double* surface_node_derivatives(int handle, hipStream_t stream);
 
 }  // namespace optlab_mi
