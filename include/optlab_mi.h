@@ -221,6 +221,57 @@ typedef struct {
     double forward_ms, backward_ms, total_ms;
 } optlab_autocall_request;
 
+/* ---- addition: a book of trades revalued under scenarios, in one persistent launch --------------------------- */
+/* Trades on the names of one book (each name: a surface handle on the book's grid of uniform steps, and its
+   forwards); paths by the local-vol step of paths.hpp:274-361 on stream 0 of each trade's own seed, so every
+   scenario of a trade sees the same draws. Scenario s multiplies name a's spot by 1 + spot_shock[s][a] (the paths
+   scale: sticky moneyness) and moves every implied vol by vol_shift[s] plus the skew tilt skew[s]
+   (structured/surfaces.py:18-50); the local-variance tables of every (scenario, name) are built on the device
+   (risk/surfaces.py:141-165). Every scenario is simulated in full: the revaluation of
+   packages/risk/src/optlab/risk/engine.py:600-645, the products of structured/engine.py:199-330. Strikes, barriers
+   and fixings are in units of each name's base spot fwd[a][0]. */
+typedef struct {
+    int product;             /* 0 European call, 1 arithmetic Asian call, 2 up-and-out call on every close,
+                                3 worst-of autocall (the phoenix of optlab_autocall_request, hard comparisons) */
+    int n_names;             /* 1 for products 0-2; 1 to 5 for product 3 */
+    int names[5];            /* indices into the book's names */
+    double root[25];         /* n_names x n_names correlation root of the trade's names (row-major) */
+    int n_steps;             /* maturity, in steps of the book's grid (1 to the book's n_steps) */
+    int obs_every;           /* Asian fixings / autocall dates every obs_every steps; n_steps % obs_every == 0 */
+    double strike;           /* products 0-2: strike; 3: put strike (performance) */
+    double barrier;          /* 2: the up-and-out level; 3: the knock-in level (0: none) */
+    double coupon, coupon_barrier, autocall;  /* product 3 */
+    int memory;              /* product 3 */
+    double quantity;         /* units held: the weight of the trade in the book's P&L */
+    long long n_pairs;
+    unsigned long long seed;
+} optlab_trade;
+
+typedef struct {
+    int n_names, n_steps;    /* the book's grid: n_steps uniform steps of dt from the valuation date */
+    double dt;
+    const int* surface;      /* host, n_names surface handles with n_steps rows each, one n_x */
+    const double* fwd;       /* host, n_names x (n_steps + 1) */
+    const double* disc;      /* host, n_steps + 1 discount factors to each step's date */
+    int n_trades;            /* >= 1 */
+    const optlab_trade* trades;
+    int n_scen;              /* >= 1; scenario 0 is the base when its moves are zero */
+    const double* spot_shock;/* host, n_scen x n_names */
+    const double* vol_shift; /* host, n_scen */
+    const double* skew;      /* host, n_scen */
+    int mode;                /* 0: one persistent launch drains the queue of (trade, scenario, chunk) items, captured
+                                with its reset and reduction in a HIP graph; 1: one launch per trade (the loop) */
+    int block;               /* threads per workgroup: 128, 256 or 512; 0: the autotuner chooses */
+    int pairs_per_lane;      /* 1, 2 or 4; 0: the autotuner chooses */
+    double* value;           /* host, n_trades x n_scen prices */
+    double* se;              /* host, n_trades x n_scen standard errors (pair averages) */
+    double* finish_ms;       /* host or NULL, n_trades: mode 0, when the trade's last item finished after the
+                                launch began (device clock); mode 1, the trade's own launch time */
+    double tables_ms, kernel_ms, total_ms;
+    long long name_steps;    /* simulated: sum over items of legs x steps x names */
+    int block_used, pairs_per_lane_used;
+} optlab_book_request;
+
 /* ---- entry points ------------------------------------------------------------------------------------------- */
 
 /* requests.hpp:97-101 */
@@ -271,6 +322,9 @@ OPTLAB_API int optlab_autocall(optlab_autocall_request* rq, int backend);
 
 /* Addition: price, deltas and node vegas by the pathwise adjoint (nodes must be NULL). */
 OPTLAB_API int optlab_adjoint(optlab_autocall_request* rq, int backend);
+
+/* Addition: the book above (backend 0 double paths, 1 float paths; sums in double either way). */
+OPTLAB_API int optlab_book(optlab_book_request* rq, int backend);
 
 /* Addition, for tests of the integer stream: the Philox4x32-10 words of counters (c0 + i, c1, c2, c3), i < n,
    key (k0, k1), computed by the device function the kernels use; out_dev holds 4 n uint32 (device memory). */

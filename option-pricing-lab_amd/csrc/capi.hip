@@ -316,6 +316,16 @@ int optlab_adjoint(optlab_autocall_request* rq, int backend) {
     });
 }
 
+int optlab_book(optlab_book_request* rq, int backend) {
+    return guarded([&] {
+        if (!rq) invalid("null request");
+        const int prec = precision_of(backend);
+        need_device();
+        check_book_request(*rq);
+        run_book(*rq, prec);
+    });
+}
+
 int optlab_philox_dev(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1, long long n,
                       unsigned* out_dev, void* stream) {
     return guarded([&] {

@@ -36,6 +36,8 @@ enum Slot {
     kSlotSeeds,        // adjoint: payoff seeds and adjoint state
     kSlotAdjoint,      // adjoint: per-workgroup table adjoints
     kSlotBatches,      // adjoint: per-batch table adjoints
+    kSlotBook,         // book: the tables of every (scenario, name) and the book's small arrays
+    kSlotBookQueue,    // book: the work queue, trades, per-item sums and clocks
     kSlotCount
 };
 
@@ -64,6 +66,8 @@ void surface_table(int handle, const double* nodes, double parallel, double skew
 void surface_node_grad(int handle, const double* tbar_host, double* out_host);
 void check_autocall_request(const optlab_autocall_request& rq, bool adjoint);
 void run_autocall(optlab_autocall_request& rq, int prec, bool adjoint);
+void check_book_request(const optlab_book_request& rq);
+void run_book(optlab_book_request& rq, int prec);
 
 // ---- test entry points (paths.hip) -----------------------------------------------------------------------------
 void launch_philox(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1, long long n,

@@ -238,6 +238,59 @@ class AutocallRequest(C.Structure):
     ]
 
 
+class Trade(C.Structure):
+    """optlab_trade (addition: one trade of a book)."""
+
+    _fields_ = [
+        ("product", _I),
+        ("n_names", _I),
+        ("names", _I * 5),
+        ("root", _D * 25),
+        ("n_steps", _I),
+        ("obs_every", _I),
+        ("strike", _D),
+        ("barrier", _D),
+        ("coupon", _D),
+        ("coupon_barrier", _D),
+        ("autocall", _D),
+        ("memory", _I),
+        ("quantity", _D),
+        ("n_pairs", _LL),
+        ("seed", _ULL),
+    ]
+
+
+class BookRequest(C.Structure):
+    """optlab_book_request (addition: a book of trades revalued under scenarios in one persistent launch)."""
+
+    _fields_ = [
+        ("n_names", _I),
+        ("n_steps", _I),
+        ("dt", _D),
+        ("surface", _PI),
+        ("fwd", _PD),
+        ("disc", _PD),
+        ("n_trades", _I),
+        ("trades", C.POINTER(Trade)),
+        ("n_scen", _I),
+        ("spot_shock", _PD),
+        ("vol_shift", _PD),
+        ("skew", _PD),
+        ("mode", _I),
+        ("block", _I),
+        ("pairs_per_lane", _I),
+        ("value", _PD),
+        ("se", _PD),
+        ("finish_ms", _PD),
+        ("tables_ms", _D),
+        ("kernel_ms", _D),
+        ("total_ms", _D),
+        ("name_steps", _LL),
+        ("block_used", _I),
+        ("pairs_per_lane_used", _I),
+    ]
+
+
 def ptr(a, kind=_PD):
     """A ctypes pointer to a NumPy array's data (None for None or an empty array)."""
     if a is None or a.size == 0:
@@ -272,6 +325,7 @@ _SIGNATURES = {
     "optlab_surface_node_grad": (_I, [_I, _PD, _PD]),
     "optlab_autocall": (_I, [C.POINTER(AutocallRequest), _I]),
     "optlab_adjoint": (_I, [C.POINTER(AutocallRequest), _I]),
+    "optlab_book": (_I, [C.POINTER(BookRequest), _I]),
     "optlab_philox_dev": (_I, [C.c_uint] * 6 + [_LL, C.c_void_p, C.c_void_p]),
     "optlab_normals_dev": (_I, [_LL, _LL, _I, C.c_uint, _ULL, _I, C.c_void_p, C.c_void_p]),
 }

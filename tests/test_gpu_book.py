@@ -1,0 +1,110 @@
+"""The book of trades under scenarios on the GPU: every product and scenario move in FP64 against the oracle on the
+same paths (1e-12); the persistent batched launch against the per-trade loop (bit-identical); the book's autocall
+against the autocall engine; results independent of the work split; the benchmark book's VaR run end to end at a
+reduced path count."""
+
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+from _support import ROOT, oracle
+
+sys.path.insert(0, str(ROOT / "oracle"))
+import book_oracle as bo  # noqa: E402
+import surface_oracle as so  # noqa: E402
+
+from optlab.core.market import Market  # noqa: E402
+from optlab.core.ssvi import SSVI  # noqa: E402
+from optlab.native import _abi  # noqa: E402
+from optlab.risk.book import Book, Scenarios, Trade, pnl, revalue, synthetic_book, synthetic_scenarios, var_es  # noqa: E402
+from optlab.structured.multiasset import correlation_root  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+PILLARS = np.array([0.05, 0.1, 0.25, 0.5, 0.75, 1.0, 1.5, 2.0, 3.0])
+
+
+def small_book(n_steps=189, n_paths=512):
+    surfaces = [SSVI.power_law(r, 1.2, 0.5, PILLARS, s * PILLARS) for s, r in ((0.03, -0.5), (0.04, -0.6), (0.05, -0.7))]
+    markets = [Market.flat(100.0, 0.03, q) for q in (0.0, 0.01, 0.02)]
+    corr = np.full((3, 3), 0.6) + 0.4 * np.eye(3)
+    trades = [
+        Trade("european", (1,), 150, 1, 0.95, n_paths=n_paths, seed=11),
+        Trade("asian", (2,), 168, 21, 1.02, n_paths=n_paths, seed=12),
+        Trade("barrier", (0,), 189, 1, 1.0, 1.25, n_paths=n_paths, seed=13),
+        Trade("autocall", (0, 1, 2), 189, 63, 1.0, 0.6, 0.02, 0.7, 1.0, True, n_paths=n_paths, seed=14),
+        Trade("autocall", (2,), 126, 63, 1.0, 0.8, 0.03, 0.9, 1.05, False, n_paths=n_paths, seed=15),
+    ]
+    return Book(surfaces, markets, corr, trades, n_steps=n_steps)
+
+
+SCEN = Scenarios(np.array([[0.0, 0.0, 0.0], [0.03, -0.02, 0.01], [-0.05, 0.0, 0.04]]),
+                 np.array([0.0, 0.015, -0.02]), np.array([0.0, -0.005, 0.01]))
+
+
+def test_fp64_book_is_the_oracle_on_the_same_paths():
+    book = small_book()
+    rev = revalue(book, SCEN, backend="fp64", mode="batched", block=128, pairs_per_lane=2)
+    t = np.arange(book.n_steps + 1) * book.dt
+    rows = [so.base_rows(s, t[:-1]) for s in book.surfaces]
+    ob = {"rows": rows, "fwd": np.stack([m.F(t) for m in book.markets]), "disc": book.markets[0].D(t), "dt": book.dt}
+    for i, tr in enumerate(book.trades):
+        sub = book.corr[np.ix_(tr.names, tr.names)]
+        root = correlation_root(sub) if len(tr.names) > 1 else np.ones((1, 1))
+        td = dict(product=tr.product, names=tr.names, root=root, n_steps=tr.n_steps, obs_every=tr.obs_every,
+                  strike=tr.strike, barrier=tr.barrier, coupon=tr.coupon, coupon_barrier=tr.coupon_barrier,
+                  autocall=tr.autocall, memory=tr.memory, n_pairs=tr.n_paths // 2, seed=tr.seed)
+        for s in range(len(SCEN.vol_shift)):
+            sc = {"spot": SCEN.spot_shock[s], "vol": SCEN.vol_shift[s], "skew": SCEN.skew[s]}
+            want, want_se = bo.trade_value(oracle(), _abi.PathsRequest, td, ob, sc)
+            assert rev.value[i, s] == pytest.approx(want, rel=1e-12, abs=1e-13), (tr.product, s)
+            assert rev.se[i, s] == pytest.approx(want_se, rel=1e-8, abs=1e-13), (tr.product, s)
+
+
+def test_batched_launch_equals_the_per_trade_loop():
+    book = small_book(n_paths=4096)
+    a = revalue(book, SCEN, mode="batched", block=256, pairs_per_lane=1)
+    b = revalue(book, SCEN, mode="loop", block=256, pairs_per_lane=1)
+    np.testing.assert_array_equal(a.value, b.value)
+    np.testing.assert_array_equal(a.se, b.se)
+    assert np.all(a.finish_ms > 0) and np.all(b.finish_ms > 0)
+
+
+def test_work_split_does_not_move_the_prices():
+    book = small_book(n_paths=4096)
+    ref = revalue(book, SCEN, block=256, pairs_per_lane=1)
+    for block, ppl in ((128, 4), (512, 2), (0, 0)):
+        got = revalue(book, SCEN, block=block, pairs_per_lane=ppl)
+        np.testing.assert_allclose(got.value, ref.value, rtol=1e-12, atol=1e-15)
+        assert got.block in (128, 256, 512) and got.pairs_per_lane in (1, 2, 4)
+
+
+def test_book_autocall_is_the_autocall_engine():
+    from optlab.native.adjoint import Phoenix, WorstOfAutocall
+
+    book = small_book(n_paths=8192)
+    base = Scenarios(np.zeros((1, 3)), np.zeros(1), np.zeros(1))
+    rev = revalue(book, base)
+    tr = book.trades[3]
+    note = WorstOfAutocall(book.surfaces, book.markets, book.corr,
+                           Phoenix(autocall=1.0, coupon_barrier=0.7, coupon_rate=0.08, frequency_months=3,
+                                   maturity_months=9, memory=True, ki_barrier=0.6, put_strike=1.0, smoothing=0.0))
+    assert note.n_steps == tr.n_steps
+    price, se, _, _ = note.price(tr.n_paths, tr.seed)
+    assert rev.value[3, 0] == pytest.approx(price, rel=2e-6)
+    assert rev.se[3, 0] == pytest.approx(se, rel=1e-3)
+    note.close()
+
+
+def test_benchmark_book_var_runs_end_to_end():
+    book = synthetic_book(path_scale=64)
+    scen = synthetic_scenarios(len(book.surfaces), n=40)
+    rev = revalue(book, scen)
+    assert rev.value.shape == (512, 41) and np.all(np.isfinite(rev.value))
+    assert np.all(rev.value >= 0)
+    p = pnl(book, rev)
+    r = var_es(p)
+    assert r["var"] > 0 and r["es"] >= r["var"] - 1e-12
+    assert rev.name_steps > 0 and rev.kernel_ms > 0

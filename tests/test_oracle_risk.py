@@ -151,3 +151,13 @@ def test_phoenix_smoothing_is_the_evaluator_ramp():
     # outside the ramp the smoothed script is the hard one
     for path in ([1.0, 0.65, 0.66, 0.67, 0.8], [1.0, 0.55, 0.9, 0.9, 0.5]):
         assert _value(path, eps=eps) == pytest.approx(_value(path), abs=1e-15)
+
+
+def test_var_es_reads_an_actual_scenario():
+    from optlab.risk.book import var_es
+
+    x = np.arange(-50.0, 50.0)  # 100 scenarios: losses down to -50
+    r = var_es(x)
+    assert r["var"] == 50.0  # the 1 % "lower" quantile of 100 points is the worst one
+    assert r["es"] == pytest.approx(np.mean([50.0, 49.0, 48.0]))  # beyond the 2.5 % quantile: 3 scenarios
+    assert x[r["var_index"]] == -50.0
