@@ -263,7 +263,7 @@ typedef struct {
                                 1: one launch per trade (the loop); 2: mode 0 captured in a HIP graph (switched
                                 off: refused with OPTLAB_EINVAL) */
     int block;               /* threads per workgroup: 128, 256 or 512; 0: the autotuner chooses */
-    int pairs_per_lane;      /* 1, 2 or 4; 0: the autotuner chooses */
+    int pairs_per_lane;      /* 1, 2, 4 or 8: an item is one wave's 64 x pairs_per_lane pairs; 0: the autotuner */
     double* value;           /* host, n_trades x n_scen prices */
     double* se;              /* host, n_trades x n_scen standard errors (pair averages) */
     double* finish_ms;       /* host or NULL, n_trades: mode 0, when the trade's last item finished after the

@@ -9,9 +9,9 @@
 // MI355X design: the queue holds (trade, scenario, chunk of pairs) items ordered longest first; a grid of exactly
 // the resident workgroups takes items with one atomic add each until the queue is empty (every wave leaves
 // through the same test), so trades of 1 to 5 names, 63 to 1260 steps and 2^14 to 2^18 paths share the machine
-// without tail effects. One pair per lane (both legs in registers), pairs_per_lane pairs in turn; each item's pair
-// sums go to its own slot, and a reduction adds the slots of a (trade, scenario) in chunk order, so results do not
-// depend on which workgroup ran what. The counter is cleared by a kernel, not a memset, so the reset, the clock
+// without tail effects. An item is 64 x pairs_per_lane pairs, taken by one wave: one pair per lane (both legs in
+// registers), pairs_per_lane pairs in turn; each item's pair sums go to its own slot, and a reduction adds the slots
+// of a (trade, scenario) in chunk order, so results do not depend on which wave ran what. The counter is cleared by a kernel, not a memset, so the reset, the clock
 // stamp, the persistent kernel and the reduction form a sequence a HIP graph can hold; capturing it is switched
 // off (mode 2 is refused) until its first run, which went silent, is explained.
 #include <algorithm>
@@ -168,23 +168,25 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, int sce
     return 0.5 * v;
 }
 
+// Persistent and wave-granular: each wave takes the next item for itself (every lane adds lane == 0 to the counter,
+// so the wave's old value is lane 0's, read as a scalar), values the item's 64 x pairs_per_lane pairs, sums them
+// with shuffles and lane 0 writes the item's slot. No LDS and no barrier: the loop's only exit is the scalar test
+// on the item index, the same in every lane. (An earlier form took items per workgroup: thread 0's atomic, an LDS
+// broadcast and barriers. The compiler split that loop so that the barrier and the index read repeated without
+// thread 0's atomic, and the workgroup took one item forever.)
 template <typename R>
 __global__ void __launch_bounds__(kMaxBlock) book_kernel(BookArgs<R> a) {
-    __shared__ int s_item;
-    __shared__ double part[kMaxBlock / 64][2];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const long long per_item = 64ll * a.pairs_per_lane;
     for (;;) {
-        if (threadIdx.x == 0) s_item = static_cast<int>(atomicAdd(a.counter, 1u));
-        __syncthreads();
-        const int it = s_item;
-        __syncthreads();
-        if (it >= a.n_items) break;  // every wave of the workgroup reads the same value: all leave together
+        const unsigned got = atomicAdd(a.counter, lane == 0 ? 1u : 0u);
+        const int it = __builtin_amdgcn_readfirstlane(static_cast<int>(got));
+        if (it >= a.n_items) break;
         const Item item = a.items[it];
         const TradeDev<R>& t = a.trades[item.trade];
-        const long long per_item = static_cast<long long>(blockDim.x) * a.pairs_per_lane;
         double s1 = 0.0, s2 = 0.0;
         for (int k = 0; k < a.pairs_per_lane; ++k) {
-            const long long p = item.chunk * per_item + static_cast<long long>(k) * blockDim.x + threadIdx.x;
+            const long long p = item.chunk * per_item + 64ll * k + lane;
             if (p < t.n_pairs) {
                 const double v = pair_value(a, t, item.scen, p);
                 s1 += v;
@@ -196,13 +198,10 @@ __global__ void __launch_bounds__(kMaxBlock) book_kernel(BookArgs<R> a) {
             s1 += shfl_xor_d(s1, m);
             s2 += shfl_xor_d(s2, m);
         }
-        if (lane == 0) part[wave][0] = s1, part[wave][1] = s2;
-        __syncthreads();
-        if (threadIdx.x < 2) {
-            double v = 0.0;
-            for (int w = 0; w < n_waves; ++w) v += part[w][threadIdx.x];
-            a.partial[item.slot * 2 + threadIdx.x] = v;
-            if (threadIdx.x == 0) a.finish[item.slot] = wall_clock64();
+        if (lane == 0) {
+            a.partial[item.slot * 2] = s1;
+            a.partial[item.slot * 2 + 1] = s2;
+            a.finish[item.slot] = wall_clock64();
         }
     }
 }
@@ -251,7 +250,7 @@ struct Prepared {
 template <typename R>
 Prepared<R> prepare(const optlab_book_request& rq, const std::vector<double>& spot, int block, int ppl) {
     Prepared<R> pr;
-    const long long per_item = static_cast<long long>(block) * ppl;
+    const long long per_item = 64ll * ppl;  // an item is one wave's: 64 lanes x ppl pairs
     for (int i = 0; i < rq.n_trades; ++i) {
         const optlab_trade& c = rq.trades[i];
         TradeDev<R> t{};
@@ -447,7 +446,7 @@ void run_typed(optlab_book_request& rq) {
             block = hit->second.first, ppl = hit->second.second;
         } else {
             // the sample: every trade under one scenario (the queue's shape without its length)
-            const int cand[][2] = {{256, 1}, {128, 1}, {512, 1}, {256, 2}, {256, 4}, {128, 2}};
+            const int cand[][2] = {{256, 2}, {256, 4}, {512, 4}, {256, 8}, {128, 4}, {512, 8}};
             float best = 1e30f;
             for (const auto& c : cand) {
                 float ms = 0.f;
@@ -459,7 +458,7 @@ void run_typed(optlab_book_request& rq) {
         }
     }
     if (block != 128 && block != 256 && block != 512) invalid("block must be 128, 256 or 512");
-    if (ppl != 1 && ppl != 2 && ppl != 4) invalid("pairs_per_lane must be 1, 2 or 4");
+    if (ppl != 1 && ppl != 2 && ppl != 4 && ppl != 8) invalid("pairs_per_lane must be 1, 2, 4 or 8");
     rq.block_used = block, rq.pairs_per_lane_used = ppl;
 
     std::vector<double> sums(static_cast<size_t>(rq.n_trades) * nsc * 2);

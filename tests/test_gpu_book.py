@@ -97,10 +97,10 @@ def test_batched_launch_equals_the_per_trade_loop():
 def test_work_split_does_not_move_the_prices():
     book = small_book(n_paths=4096)
     ref = revalue(book, SCEN, block=256, pairs_per_lane=1)
-    for block, ppl in ((128, 4), (512, 2), (0, 0)):
+    for block, ppl in ((128, 4), (512, 2), (256, 8), (0, 0)):
         got = revalue(book, SCEN, block=block, pairs_per_lane=ppl)
         np.testing.assert_allclose(got.value, ref.value, rtol=1e-12, atol=1e-15)
-        assert got.block in (128, 256, 512) and got.pairs_per_lane in (1, 2, 4)
+        assert got.block in (128, 256, 512) and got.pairs_per_lane in (1, 2, 4, 8)
 
 
 def test_book_autocall_is_the_autocall_engine():
