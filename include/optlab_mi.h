@@ -259,9 +259,9 @@ typedef struct {
     const double* spot_shock;/* host, n_scen x n_names */
     const double* vol_shift; /* host, n_scen */
     const double* skew;      /* host, n_scen */
-    int mode;                /* 0: one persistent launch drains the queue of (trade, scenario, chunk) items;
-                                1: one launch per trade (the loop); 2: mode 0 captured in a HIP graph (switched
-                                off: refused with OPTLAB_EINVAL) */
+    int mode;                /* 0: one persistent launch drains the queue of (trade, scenario, chunk) items,
+                                captured with its reset and reduction in a HIP graph; 1: one launch per trade
+                                (the loop); 2: mode 0 launched directly, without the graph */
     int block;               /* threads per workgroup: 128, 256 or 512; 0: the autotuner chooses */
     int pairs_per_lane;      /* 1, 2, 4 or 8: an item is one wave's 64 x pairs_per_lane pairs; 0: the autotuner */
     double* value;           /* host, n_trades x n_scen prices */

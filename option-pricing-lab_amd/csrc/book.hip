@@ -307,10 +307,11 @@ int resident_blocks(int block) {
     return std::max(per_cu, 1) * device_cu_count();
 }
 
-// One launch of the queue `items` with its counter reset and its reduction, synchronised.
+// One launch of the queue `items` with its counter reset and its reduction (captured in a HIP graph and the graph
+// launched, when `graph`), synchronised.
 template <typename R>
 float launch_queue(BookArgs<R> args, const std::vector<Item>& items, Item* d_items, int block, int n_trades,
-                   int n_scen, const TradeDev<R>* d_trades, double* d_sums) {
+                   int n_scen, const TradeDev<R>* d_trades, double* d_sums, bool graph) {
     hipStream_t st = book_stream();
     hip_check(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, st),
               "copy queue");
@@ -327,11 +328,27 @@ float launch_queue(BookArgs<R> args, const std::vector<Item>& items, Item* d_ite
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "event");
     hip_check(hipEventCreate(&e1), "event");
-    hip_check(hipEventRecord(e0, st), "event");
-    body();
-    hip_check(hipGetLastError(), "book launch");
-    hip_check(hipEventRecord(e1, st), "event");
-    hip_check(hipStreamSynchronize(st), "synchronize");
+    if (graph) {
+        hipGraph_t g;
+        hipGraphExec_t ge;
+        hip_check(hipStreamSynchronize(st), "synchronize");
+        hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+        body();
+        hip_check(hipStreamEndCapture(st, &g), "end capture");
+        hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "instantiate");
+        hip_check(hipEventRecord(e0, st), "event");
+        hip_check(hipGraphLaunch(ge, st), "graph launch");
+        hip_check(hipEventRecord(e1, st), "event");
+        hip_check(hipStreamSynchronize(st), "synchronize");
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+    } else {
+        hip_check(hipEventRecord(e0, st), "event");
+        body();
+        hip_check(hipGetLastError(), "book launch");
+        hip_check(hipEventRecord(e1, st), "event");
+        hip_check(hipStreamSynchronize(st), "synchronize");
+    }
     float ms = 0.f;
     hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
     (void)hipEventDestroy(e0);
@@ -405,7 +422,8 @@ void run_typed(optlab_book_request& rq) {
     int block = rq.block, ppl = rq.pairs_per_lane;
     const auto key = std::make_tuple(rq.n_trades, nsc, total_pairs, prec);
 
-    auto run = [&](int blk, int pp, const std::vector<int>& which, int n_scen_run, float* ms_out, Prepared<R>* keep) {
+    auto run = [&](int blk, int pp, const std::vector<int>& which, int n_scen_run, bool graph, float* ms_out,
+                   Prepared<R>* keep) {
         Prepared<R> pr = prepare<R>(rq, spot, blk, pp);
         std::vector<Item> items = queue_of(pr, n_scen_run, which);
         const size_t slots = static_cast<size_t>(pr.n_slots);
@@ -433,7 +451,7 @@ void run_typed(optlab_book_request& rq) {
         BookArgs<R> ar = args;
         ar.trades = d_trades, ar.pairs_per_lane = pp, ar.counter = d_counter, ar.partial = d_partial;
         ar.finish = d_finish, ar.start = d_start;
-        *ms_out = launch_queue(ar, items, d_items, blk, rq.n_trades, n_scen_run, d_trades, d_sums);
+        *ms_out = launch_queue(ar, items, d_items, blk, rq.n_trades, n_scen_run, d_trades, d_sums, graph);
         if (keep) *keep = pr;
         return std::make_tuple(d_sums, d_finish, d_start);
     };
@@ -450,8 +468,8 @@ void run_typed(optlab_book_request& rq) {
             float best = 1e30f;
             for (const auto& c : cand) {
                 float ms = 0.f;
-                run(c[0], c[1], all, 1, &ms, nullptr);  // first use of a shape pays its warm-up
-                run(c[0], c[1], all, 1, &ms, nullptr);
+                run(c[0], c[1], all, 1, false, &ms, nullptr);  // first use of a shape pays its warm-up
+                run(c[0], c[1], all, 1, false, &ms, nullptr);
                 if (ms < best) best = ms, block = c[0], ppl = c[1];
             }
             g_tuned[key] = {block, ppl};
@@ -470,9 +488,9 @@ void run_typed(optlab_book_request& rq) {
         hip_check(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "wall clock rate");
         return static_cast<double>(khz > 0 ? khz : 100000);
     }();
-    if (rq.mode == 0) {
+    if (rq.mode != 1) {
         float ms = 0.f;
-        auto [d_sums, d_finish, d_start] = run(block, ppl, all, nsc, &ms, &pr);
+        auto [d_sums, d_finish, d_start] = run(block, ppl, all, nsc, rq.mode == 0, &ms, &pr);
         kernel_ms = ms;
         hip_check(hipMemcpy(sums.data(), d_sums, sums.size() * sizeof(double), hipMemcpyDeviceToHost), "copy sums");
         if (rq.finish_ms) {
@@ -492,7 +510,7 @@ void run_typed(optlab_book_request& rq) {
     } else {
         for (int i = 0; i < rq.n_trades; ++i) {
             float ms = 0.f;
-            auto [d_sums, d_finish, d_start] = run(block, ppl, std::vector<int>{i}, nsc, &ms, &pr);
+            auto [d_sums, d_finish, d_start] = run(block, ppl, std::vector<int>{i}, nsc, false, &ms, &pr);
             (void)d_finish, (void)d_start;
             kernel_ms += ms;
             if (rq.finish_ms) rq.finish_ms[i] = ms;
@@ -531,8 +549,7 @@ void check_book_request(const optlab_book_request& c) {
     if (!c.surface || !c.fwd || !c.disc || !c.trades || !c.spot_shock || !c.vol_shift || !c.skew || !c.value || !c.se)
         invalid("null array");
     if (c.n_trades < 1 || c.n_scen < 1) invalid("n_trades and n_scen must be >= 1");
-    if (c.mode == 2) invalid("mode 2 (the launch captured in a HIP graph) is switched off");
-    if (c.mode != 0 && c.mode != 1) invalid("mode must be 0 (one launch) or 1 (a launch per trade)");
+    if (c.mode < 0 || c.mode > 2) invalid("mode must be 0 (one graph launch), 1 (a launch per trade) or 2 (one launch)");
     int n_x = -1;
     for (int a = 0; a < c.n_names; ++a) {
         const SurfaceRows& r = surface_rows(c.surface[a]);

@@ -137,8 +137,9 @@ class Revaluation:
 
 def revalue(book: Book, scen: Scenarios, backend: str = "gpu", mode: str = "batched", block: int = 0,
             pairs_per_lane: int = 0) -> Revaluation:
-    """Every trade under every scenario: mode "batched" (one persistent launch draining the work queue) or "loop"
-    (one launch per trade). ("graph", the batched launch captured in a HIP graph, is switched off in the engine.)"""
+    """Every trade under every scenario: mode "batched" (one persistent launch draining the work queue, captured in a
+    HIP graph with its reset and reduction), "direct" (the same launch without the graph) or "loop" (one launch per
+    trade)."""
     lib = _abi.load()
     dev = book.device_surfaces()
     n_names, ns = len(dev), book.n_steps
@@ -171,7 +172,7 @@ def revalue(book: Book, scen: Scenarios, backend: str = "gpu", mode: str = "batc
     rq.surface, rq.fwd, rq.disc = _abi.ptr(handles, _abi._PI), _abi.ptr(fwd), _abi.ptr(disc)
     rq.n_trades, rq.trades = len(book.trades), trades
     rq.n_scen, rq.spot_shock, rq.vol_shift, rq.skew = n_scen, _abi.ptr(spot), _abi.ptr(vol), _abi.ptr(skew)
-    rq.mode = {"batched": 0, "loop": 1, "graph": 2}[mode]
+    rq.mode = {"batched": 0, "loop": 1, "direct": 2}[mode]
     rq.block, rq.pairs_per_lane = block, pairs_per_lane
     rq.value, rq.se, rq.finish_ms = _abi.ptr(value), _abi.ptr(se), _abi.ptr(finish)
     code = lib.optlab_book(C.byref(rq), precision_code(backend))
