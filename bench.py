@@ -217,6 +217,76 @@ def bench_adjoint(n_paths: int, seed: int = 42, reps: int = 3) -> dict:
     }
 
 
+def bench_book(path_scale: int = 16, n_scen: int = 250) -> dict:
+    """The batched book: 512 trades (seed 7) x 250 scenarios plus the base, every trade's paths divided by
+    `path_scale`; one persistent launch captured in a HIP graph against the per-trade loop, same numbers."""
+    import numpy as np
+
+    from optlab.risk.book import pnl, revalue, synthetic_book, synthetic_scenarios, var_es
+
+    book = synthetic_book(path_scale=path_scale)
+    scen = synthetic_scenarios(len(book.surfaces), n_scen)
+    book.device_surfaces()
+    revalue(book, scen)  # warm-up: the autotuner's choice and the buffers
+    t0 = time.perf_counter()
+    rb = revalue(book, scen, mode="batched")
+    wall_b = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rl = revalue(book, scen, mode="loop", block=rb.block, pairs_per_lane=rb.pairs_per_lane)
+    wall_l = time.perf_counter() - t0
+    same = bool(np.array_equal(rb.value, rl.value))
+    risk = var_es(pnl(book, rb))
+    return {
+        "workload": f"512 trades (40 % European, 20 % Asian, 20 % daily up-and-out, 20 % worst-of autocall on 1-5 "
+                    f"names; 0.25-5 y; 2^14-2^17 paths / {path_scale}) x {n_scen} scenarios + base, local vol from 5 "
+                    "SSVI surfaces, FP32 paths",
+        "name_steps": rb.name_steps,
+        "batched_ms": rb.kernel_ms,
+        "batched_wall_ms": wall_b * 1e3,
+        "batched_name_steps_per_s": rb.name_steps / (rb.kernel_ms * 1e-3),
+        "loop_ms": rl.kernel_ms,
+        "loop_wall_ms": wall_l * 1e3,
+        "loop_name_steps_per_s": rl.name_steps / (rl.kernel_ms * 1e-3),
+        "batched_over_loop": rl.kernel_ms / rb.kernel_ms,
+        "tables_ms": rb.tables_ms,
+        "block": rb.block,
+        "pairs_per_lane": rb.pairs_per_lane,
+        "latency_ms_batched": {"p50": float(np.percentile(rb.finish_ms, 50)), "p99": float(np.percentile(rb.finish_ms, 99))},
+        "latency_ms_loop": {"p50": float(np.percentile(rl.finish_ms, 50)), "p99": float(np.percentile(rl.finish_ms, 99))},
+        "var99": risk["var"],
+        "es975": risk["es"],
+        "batched_equals_loop": same,
+    }
+
+
+def bench_slv(n_particles: int = 1 << 20, T: float = 3.0, n_paths: int = 1 << 20) -> dict:
+    """The SLV leverage by particles (2^20 particles, 365 steps a year to 3 y, 256 bins, bandwidth 0.25 std(ln S)
+    N^-1/5, FP64 leverage division) and the repricing of the 21 x 8 grid on fresh FP32 paths."""
+    import numpy as np
+
+    slv_calibrate(1 << 14, 0.25)  # warm-up
+    t0 = time.perf_counter()
+    model = slv_calibrate(n_particles, T)
+    t_cal = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r = slv_reprice(model, n_paths, SLV_MATS, SLV_STRIKES)
+    t_rep = time.perf_counter() - t0
+    n_steps = int(round(T * 365))
+    inner = slice(3, 18)  # K 79..121
+    return {
+        "workload": f"SLV particles: SSVI theta .04 t, rho -.6, eta 1.2, gamma .5; Heston .04/2/.04/.5/-.7; {n_particles} "
+                    f"particles x {n_steps} steps, 256 bins, bandwidth 0.25 std(ln S) N^-1/5; reprice 21 x 8 calls on "
+                    f"{n_paths} FP32 paths",
+        "calibration_s": t_cal,
+        "particle_steps_per_s": n_particles * n_steps / t_cal,
+        "reprice_s": t_rep,
+        "max_abs_vol_bp_k79_121": float(np.nanmax(np.abs(r["vol_bp"][:, inner]))),
+        "max_abs_vol_bp_all": float(np.nanmax(np.abs(r["vol_bp"]))),
+        "max_price_gap_in_se": float(np.nanmax(np.abs(r["price_gap"]) / r["se"])),
+        "vol_bp_by_maturity_k79_121": [float(np.nanmax(np.abs(v[inner]))) for v in r["vol_bp"]],
+    }
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,6 +296,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--adjoint-paths", type=int, default=1 << 19,
                     help="paths of the worst-of autocall adjoint-vs-bump comparison (0: skip it)")
+    ap.add_argument("--book-scale", type=int, default=16,
+                    help="divide every trade's paths of the batched book by this (0: skip the book)")
+    ap.add_argument("--slv-particles", type=int, default=1 << 20, help="SLV particles (0: skip the SLV block)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the timed region, one device-to-device copy of this many MiB: the known-size kernel "
                          "that scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -307,6 +380,13 @@ def main(argv=None) -> int:
         line["gpu_over_cpu"] = v32 / cpu_rate
     if args.adjoint_paths:
         line["adjoint"] = bench_adjoint(args.adjoint_paths)
+        print("bench: adjoint block done", file=sys.stderr, flush=True)
+    if args.book_scale:
+        line["book"] = bench_book(args.book_scale)
+        print("bench: book block done", file=sys.stderr, flush=True)
+    if args.slv_particles:
+        line["slv"] = bench_slv(args.slv_particles)
+        print("bench: slv block done", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
     if args.copy_mib:
         src = torch.ones(args.copy_mib << 20, dtype=torch.uint8, device="cuda")

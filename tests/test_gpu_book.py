@@ -125,5 +125,8 @@ def test_benchmark_book_var_runs_end_to_end():
     assert np.all(rev.value >= 0)
     p = pnl(book, rev)
     r = var_es(p)
-    assert r["var"] > 0 and r["es"] >= r["var"] - 1e-12
+    # ES at 97.5 % is the mean loss beyond the 97.5 % quantile: at least that quantile's loss (it may sit below the
+    # 99 % VaR, the reference's two levels differ)
+    q975 = -np.quantile(p, 0.025, method="lower")
+    assert r["var"] > 0 and r["es"] >= q975 - 1e-12 and r["var"] >= q975 - 1e-12
     assert rev.name_steps > 0 and rev.kernel_ms > 0

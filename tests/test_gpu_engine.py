@@ -290,3 +290,30 @@ def test_strip_table_cache_never_serves_a_stale_table(lib):
     np.testing.assert_array_equal(results[0], results[3])
     np.testing.assert_array_equal(results[1], results[5])
     assert not np.array_equal(results[0], results[2])  # FP32 against FP64 of the same strip: not bit-equal
+
+
+def test_black_scholes_fp64_workload_equals_the_oracle_and_the_closed_forms(lib):
+    """The plain FP64 Black-Scholes workload: S0 100, r .03, q .01, sigma .2, T 1, 252 daily steps, 2^18 paths, seed
+    42; European call K 100, arithmetic Asian call K 100 with the geometric control variate, up-and-out call K 100
+    B 130 on every close. FP64 GPU = the oracle (1e-12); the European within 4 SE of Black-Scholes, the geometric
+    Asian within 4 SE of its closed form, the up-and-out within 4 SE of the continuous price shifted by the
+    Broadie-Glasserman-Kou correction."""
+    from optlab.core.blackscholes import black_price, geometric_asian, up_out_call_bgk
+
+    common = dict(S0=100.0, r=0.03, q=0.01, T=1.0, sigma=0.20, n_paths=1 << 18, seed=42)
+    F, D = 100.0 * math.exp(0.02), math.exp(-0.03)
+    euro = _request(0, 0, (1, 0, 0), (100, 0, 0), 252, **common)
+    asian = _request(0, 1, (1, 0, 0), (100, 0, 0), 252, **common)
+    barrier = _request(0, 3, (1, 0, 0), (100, 0, 0), 252, H=130.0, **common)
+    for rq in (euro, asian, barrier):
+        want, got = _oracle_price(rq), _engine_price(lib, rq, 0)
+        for k in range(4):
+            assert got.price[k] == pytest.approx(want.price[k], rel=1e-12, abs=1e-14), (rq.product, k)
+    e = _engine_price(lib, euro, 0)
+    assert abs(e.price[0] - black_price(F, 100.0, 1.0, 0.2, D)) < 4 * e.se[0]
+    a = _engine_price(lib, asian, 0)
+    geo = geometric_asian(100.0, 100.0, 1.0, 0.03, 0.01, 0.2, 252)
+    assert abs(a.price[1] - geo) < 4 * a.se[1]
+    assert abs(a.price[2] - a.price[0]) < 4 * a.se[0] and a.se[2] < 0.2 * a.se[0]  # the variate cuts the SE
+    b = _engine_price(lib, barrier, 0)
+    assert abs(b.price[0] - up_out_call_bgk(100.0, 100.0, 130.0, 1.0, 0.03, 0.01, 0.2, 252)) < 4 * b.se[0] + 0.01
