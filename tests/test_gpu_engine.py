@@ -294,15 +294,17 @@ def test_strip_table_cache_never_serves_a_stale_table(lib):
 
 def test_black_scholes_fp64_workload_equals_the_oracle_and_the_closed_forms(lib):
     """The plain FP64 Black-Scholes workload: S0 100, r .03, q .01, sigma .2, T 1, 252 daily steps, 2^18 paths, seed
-    42; European call K 100, arithmetic Asian call K 100 with the geometric control variate, up-and-out call K 100
-    B 130 on every close. FP64 GPU = the oracle (1e-12); the European within 4 SE of Black-Scholes, the geometric
+    42; European call K 100 (one exact step), arithmetic Asian call K 100 with the geometric control variate,
+    up-and-out call K 100 B 130 on every close. FP64 GPU = the oracle (1e-12); the European within 4 SE of Black-Scholes, the geometric
     Asian within 4 SE of its closed form, the up-and-out within 4 SE of the continuous price shifted by the
     Broadie-Glasserman-Kou correction."""
     from optlab.core.blackscholes import black_price, geometric_asian, up_out_call_bgk
 
     common = dict(S0=100.0, r=0.03, q=0.01, T=1.0, sigma=0.20, n_paths=1 << 18, seed=42)
     F, D = 100.0 * math.exp(0.02), math.exp(-0.03)
-    euro = _request(0, 0, (1, 0, 0), (100, 0, 0), 252, **common)
+    # the fixed European under Black-Scholes is one exact step of T / n_steps (schemes.hpp:190-198 with
+    # make_consts, :116-117): the whole year in one step; the Asian and the barrier read the 252 daily closes
+    euro = _request(0, 0, (1, 0, 0), (100, 0, 0), 1, **common)
     asian = _request(0, 1, (1, 0, 0), (100, 0, 0), 252, **common)
     barrier = _request(0, 3, (1, 0, 0), (100, 0, 0), 252, H=130.0, **common)
     for rq in (euro, asian, barrier):
