@@ -167,7 +167,7 @@ def autocall_note():
     return WorstOfAutocall(surfaces, [Market.flat(100.0, 0.03, 0.0)] * 3, corr)
 
 
-def bench_adjoint(n_paths: int, seed: int = 42, reps: int = 3) -> dict:
+def bench_adjoint(n_paths: int, seed: int = 42, reps: int = 3, bumps: bool = True) -> dict:
     """Adjoint Greeks (3 deltas + 240 node vegas) against bump-and-revalue of all 486 bumps on common random numbers,
     both in FP32 on the same engine; the speedup at matched standard error is t_bump / t_adjoint times the median over
     nodes of (SE_bump / SE_adjoint)^2 (the paths the bumps need for the adjoint's SE)."""
@@ -184,6 +184,10 @@ def bench_adjoint(n_paths: int, seed: int = 42, reps: int = 3) -> dict:
         fwd.append(g.seconds["forward_ms"])
         bwd.append(g.seconds["backward_ms"])
     t_adj = sorted(walls)[reps // 2]
+    if not bumps:
+        note.close()
+        return {"adjoint_ms": t_adj * 1e3, "adjoint_forward_kernel_ms": float(np.median(fwd)),
+                "adjoint_backward_ms": float(np.median(bwd)), "price": g.price, "se": g.se}
     t0 = time.perf_counter()
     b = note.bump(n_paths, seed)
     t_bump = time.perf_counter() - t0
@@ -299,6 +303,9 @@ def main(argv=None) -> int:
     ap.add_argument("--book-scale", type=int, default=16,
                     help="divide every trade's paths of the batched book by this (0: skip the book)")
     ap.add_argument("--slv-particles", type=int, default=1 << 20, help="SLV particles (0: skip the SLV block)")
+    ap.add_argument("--only", choices=("adjoint", "book", "slv"), default=None,
+                    help="run only this block and print its JSON (for profiling runs; the benchmark line needs all)")
+    ap.add_argument("--no-bumps", action="store_true", help="with --only adjoint: skip the bump-and-revalue comparator")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the timed region, one device-to-device copy of this many MiB: the known-size kernel "
                          "that scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -317,6 +324,12 @@ def main(argv=None) -> int:
         print("no HIP device", file=sys.stderr)
         return 1
     torch.cuda.init()
+    if args.only:
+        block = {"adjoint": lambda: bench_adjoint(args.adjoint_paths, bumps=not args.no_bumps),
+                 "book": lambda: bench_book(args.book_scale),
+                 "slv": lambda: bench_slv(args.slv_particles)}[args.only]()
+        print(json.dumps({args.only: block}), flush=True)
+        return 0
     ms32, kern32, p32, se32 = _time_gpu(lib, 1, args.steps, args.warmup)
     ms64, kern64, p64, se64 = _time_gpu(lib, 0, max(1, args.fp64_steps), 1)
     work = N_PATHS * N_STEPS
