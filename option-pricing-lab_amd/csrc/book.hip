@@ -6,14 +6,14 @@
 // the path engine's spots) and the worst-of autocall of autocall.hip with hard comparisons. Paths: the local-vol
 // step of packages/native/cuda/include/optlab_cuda/paths.hpp:274-361 on stream 0 of the trade's seed.
 //
-// MI355X design: the queue holds (trade, scenario, chunk of pairs) items ordered longest first; a grid of exactly
-// the resident workgroups takes items with one atomic add each until the queue is empty (every wave leaves
-// through the same test), so trades of 1 to 5 names, 63 to 1260 steps and 2^14 to 2^18 paths share the machine
-// without tail effects. An item is 64 x pairs_per_lane pairs, taken by one wave: one pair per lane (both legs in
-// registers), pairs_per_lane pairs in turn; each item's pair sums go to its own slot, and a reduction adds the slots
-// of a (trade, scenario) in chunk order, so results do not depend on which wave ran what. The counter is cleared by a kernel, not a memset, so the reset, the clock
-// stamp, the persistent kernel and the reduction form a sequence a HIP graph can hold; capturing it is switched
-// off (mode 2 is refused) until its first run, which went silent, is explained.
+// MI355X design: the queue holds (trade, scenario, chunk of pairs) items ordered longest first; a persistent grid
+// of exactly the resident workgroups walks it with a stride of the grid, so trades of 1 to 5 names, 63 to 1260
+// steps and 2^14 to 2^17 paths share the machine (longest first keeps the tail short). An item is blockDim x
+// pairs_per_lane pairs of one (trade, scenario), taken by one workgroup: one pair per lane (both legs in
+// registers), pairs_per_lane pairs in turn, the table rows of each step staged in LDS once for all the workgroup's
+// waves (double-buffered, one barrier a step) instead of gathered per lane from HBM. Each item's pair sums go to its
+// own slot and a reduction adds the slots of a (trade, scenario) in chunk order, so results do not depend on which
+// workgroup ran what. The clock stamp, the persistent kernel and the reduction are captured in one HIP graph.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -51,6 +51,15 @@ struct Item {
     long long slot;
 };
 
+// The fields of an item and its trade that steer control flow, read as scalars (readfirstlane): every loop around a
+// barrier then has an exit the compiler knows to be the same in all lanes.
+struct Shape {
+    int product, n, n_steps, obs_every, memory, scen;
+    int names[kNames];
+};
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 template <typename R>
 struct BookArgs {
     int n_book, n_steps, n_x, n_scen, n_items, pairs_per_lane;
@@ -63,31 +72,39 @@ struct BookArgs {
     const R* disc;    // n_steps + 1
     const TradeDev<R>* trades;
     const Item* items;
-    unsigned* counter;
     double* partial;              // slot x 2
     unsigned long long* finish;   // slot
     unsigned long long* start;
 };
 
-// Value of pair p of trade t under scenario s: the average of its two legs.
+// The rows of step `step` of every name of trade t under scenario s into LDS (n names x n_x), by the workgroup.
 template <typename R>
-__device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, int scen, long long p) {
+__device__ __forceinline__ void stage_rows(const BookArgs<R>& a, const Shape& sh, int step, R* dst) {
+    for (int k = 0; k < sh.n; ++k) {
+        const R* src = a.tables + ((static_cast<long long>(sh.scen) * a.n_book + sh.names[k]) * a.n_steps + step) * a.n_x;
+        for (int c = threadIdx.x; c < a.n_x; c += blockDim.x) dst[k * a.n_x + c] = src[c];
+    }
+}
+
+// Value of pair p of trade t under scenario s: the average of its two legs. Every lane of the workgroup calls it
+// together (lanes past the trade's pairs compute a pair they then drop): the step's table rows are staged in LDS,
+// double-buffered, with one barrier a step.
+template <typename R>
+__device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const Shape& sh, long long p, R* rows) {
     NormalStream<R> rng;
     rng.init(static_cast<std::uint64_t>(p), t.key0, t.key1, 0u);
-    const int n = t.n;
+    const int n = sh.n, scen = sh.scen;
     const int stride = a.n_steps + 1;
     R x[2][kNames];
     R lsh[kNames];
-    const R* tab[kNames];
     const R* lo[kNames];
     const R* inv[kNames];
     const R* lf[kNames];
 #pragma unroll
     for (int k = 0; k < kNames; ++k) {
         x[0][k] = x[1][k] = R(0);
-        const int nm = k < n ? t.names[k] : t.names[0];
+        const int nm = k < n ? sh.names[k] : sh.names[0];
         lsh[k] = a.lsh[scen * a.n_book + nm];
-        tab[k] = a.tables + (static_cast<long long>(scen) * a.n_book + nm) * a.n_steps * a.n_x;
         lo[k] = a.lo + static_cast<long long>(nm) * a.n_steps;
         inv[k] = a.inv + static_cast<long long>(nm) * a.n_steps;
         lf[k] = a.lf + static_cast<long long>(nm) * stride;
@@ -96,9 +113,10 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, int sce
     R acc[2] = {R(0), R(0)};        // Asian: sum of fixings; barrier: running max of ln S
     R ymin[2] = {R(0), R(0)};       // autocall: lowest ln performance over every close
     Note<R> note[2] = {{R(1), R(0), R(0)}, {R(1), R(0), R(0)}};
-    const Terms<R> terms{t.autocall, t.coupon_barrier, t.coupon, t.barrier, t.put_strike, R(0), t.memory};
-    if (t.product == 2) acc[0] = acc[1] = R(-1e30);
-    if (t.product == 3) {
+    const Terms<R> terms{t.autocall, t.coupon_barrier, t.coupon, t.barrier, t.put_strike, R(0), sh.memory};
+    const int product = sh.product, n_steps = sh.n_steps, obs_every = sh.obs_every;
+    if (product == 2) acc[0] = acc[1] = R(-1e30);
+    if (product == 3) {
         R y0 = lf[0][0] + lsh[0] - t.lfix[0];
 #pragma unroll
         for (int k = 1; k < kNames; ++k)
@@ -106,15 +124,21 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, int sce
         ymin[0] = ymin[1] = y0;
     }
     int obs_j = 0;
-    const int n_obs = t.product == 3 ? t.n_steps / t.obs_every : 0;
-    for (int step = 0; step < t.n_steps; ++step) {
+    const int n_obs = product == 3 ? n_steps / obs_every : 0;
+    const int stage_len = n * a.n_x;
+    __syncthreads();  // the previous pass has read its last rows
+    stage_rows(a, sh, 0, rows);
+    for (int step = 0; step < n_steps; ++step) {
+        const R* cur = rows + (step & 1) * stage_len;
+        __syncthreads();  // this step's rows are in; everyone is done with the last step's buffer
+        if (step + 1 < n_steps) stage_rows(a, sh, step + 1, rows + ((step + 1) & 1) * stage_len);
         R zs[kNames];
 #pragma unroll
         for (int k = 0; k < kNames; ++k)
             if (k < n) zs[k] = rng.next();
         R var[kNames];  // the table reads do not depend on the leg's sign until x differs: read per leg
         const int t1 = step + 1;
-        const bool fix = t1 % t.obs_every == 0;
+        const bool fix = t1 % obs_every == 0;
 #pragma unroll
         for (int leg = 0; leg < 2; ++leg) {
             const R sign = leg ? R(-1) : R(1);
@@ -126,17 +150,16 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, int sce
                 for (int b = 0; b < kNames; ++b)
                     if (b < n) acc_z += t.root[k * n + b] * zs[b];
                 const R z = sign * acc_z;
-                const Lerp<R> l = lerp_at(tab[k] + static_cast<long long>(step) * a.n_x, lo[k][step], inv[k][step],
-                                          a.n_x, x[leg][k]);
+                const Lerp<R> l = lerp_at(cur + k * a.n_x, lo[k][step], inv[k][step], a.n_x, x[leg][k]);
                 var[k] = l.var;
                 x[leg][k] += R(-0.5) * var[k] * a.dt + r_sqrt(var[k] * a.dt) * z;
             }
             const R y = lf[0][t1] + lsh[0] + x[leg][0];  // ln S of the first name
-            if (t.product == 1) {
+            if (product == 1) {
                 if (fix) acc[leg] += r_exp(y);
-            } else if (t.product == 2) {
+            } else if (product == 2) {
                 acc[leg] = r_max(acc[leg], y);
-            } else if (t.product == 3) {
+            } else if (product == 3) {
                 R yw = y - t.lfix[0];
 #pragma unroll
                 for (int k = 1; k < kNames; ++k)
@@ -145,20 +168,20 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, int sce
                 if (fix) observe(terms, obs_j == n_obs - 1, r_exp(yw), r_exp(ymin[leg]), a.disc[t1], note[leg]);
             }
         }
-        if (t.product == 3 && fix) ++obs_j;
+        if (product == 3 && fix) ++obs_j;
     }
-    const R D = a.disc[t.n_steps];
+    const R D = a.disc[n_steps];
     double v = 0.0;
 #pragma unroll
     for (int leg = 0; leg < 2; ++leg) {
-        const R ST = r_exp(lf[0][t.n_steps] + lsh[0] + x[leg][0]);
+        const R ST = r_exp(lf[0][n_steps] + lsh[0] + x[leg][0]);
         R pay;
-        if (t.product == 0) {
+        if (product == 0) {
             pay = D * r_max(ST - t.strike, R(0));
-        } else if (t.product == 1) {
-            const R avg = acc[leg] / static_cast<R>(t.n_steps / t.obs_every);
+        } else if (product == 1) {
+            const R avg = acc[leg] / static_cast<R>(n_steps / obs_every);
             pay = D * r_max(avg - t.strike, R(0));
-        } else if (t.product == 2) {
+        } else if (product == 2) {
             pay = acc[leg] < r_log(t.barrier) ? D * r_max(ST - t.strike, R(0)) : R(0);
         } else {
             pay = note[leg].value;
@@ -168,27 +191,31 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, int sce
     return 0.5 * v;
 }
 
-// Persistent and wave-granular: each wave takes the next item for itself (every lane adds lane == 0 to the counter,
-// so the wave's old value is lane 0's, read as a scalar), values the item's 64 x pairs_per_lane pairs, sums them
-// with shuffles and lane 0 writes the item's slot. No LDS and no barrier: the loop's only exit is the scalar test
-// on the item index, the same in every lane. (An earlier form took items per workgroup: thread 0's atomic, an LDS
-// broadcast and barriers. The compiler split that loop so that the barrier and the index read repeated without
-// thread 0's atomic, and the workgroup took one item forever.)
+// Persistent: a grid of the resident workgroups walks the queue (ordered longest first) with a stride of the grid,
+// item it on workgroup it % grid. An item is blockDim x pairs_per_lane pairs of one (trade, scenario): every wave of
+// the workgroup reads the same staged rows. Every loop bound is the same in all lanes (kernel arguments and the item
+// index), so each barrier is reached by the whole workgroup. Each item's pair sums go to its own slot.
 template <typename R>
 __global__ void __launch_bounds__(kMaxBlock) book_kernel(BookArgs<R> a) {
-    const int lane = threadIdx.x & 63;
-    const long long per_item = 64ll * a.pairs_per_lane;
-    for (;;) {
-        const unsigned got = atomicAdd(a.counter, lane == 0 ? 1u : 0u);
-        const int it = __builtin_amdgcn_readfirstlane(static_cast<int>(got));
-        if (it >= a.n_items) break;
+    extern __shared__ unsigned char lds_raw[];
+    R* rows = reinterpret_cast<R*>(lds_raw);  // 2 x kNames x n_x
+    __shared__ double part[kMaxBlock / 64][2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
+    for (long long it = blockIdx.x; it < a.n_items; it += gridDim.x) {
         const Item item = a.items[it];
-        const TradeDev<R>& t = a.trades[item.trade];
+        const TradeDev<R>& t = a.trades[uniform(item.trade)];
+        Shape sh;
+        sh.product = uniform(t.product), sh.n = uniform(t.n), sh.n_steps = uniform(t.n_steps);
+        sh.obs_every = uniform(t.obs_every), sh.memory = uniform(t.memory), sh.scen = uniform(item.scen);
+#pragma unroll
+        for (int k = 0; k < kNames; ++k) sh.names[k] = uniform(t.names[k]);
+        const int chunk = uniform(item.chunk);
         double s1 = 0.0, s2 = 0.0;
         for (int k = 0; k < a.pairs_per_lane; ++k) {
-            const long long p = item.chunk * per_item + 64ll * k + lane;
-            if (p < t.n_pairs) {
-                const double v = pair_value(a, t, item.scen, p);
+            const long long p = (static_cast<long long>(chunk) * a.pairs_per_lane + k) * blockDim.x + threadIdx.x;
+            const bool live = p < t.n_pairs;
+            const double v = pair_value(a, t, sh, live ? p : 0, rows);
+            if (live) {
                 s1 += v;
                 s2 += v * v;
             }
@@ -198,18 +225,19 @@ __global__ void __launch_bounds__(kMaxBlock) book_kernel(BookArgs<R> a) {
             s1 += shfl_xor_d(s1, m);
             s2 += shfl_xor_d(s2, m);
         }
-        if (lane == 0) {
-            a.partial[item.slot * 2] = s1;
-            a.partial[item.slot * 2 + 1] = s2;
-            a.finish[item.slot] = wall_clock64();
+        if (lane == 0) part[wave][0] = s1, part[wave][1] = s2;
+        __syncthreads();
+        if (threadIdx.x < 2) {
+            double v = 0.0;
+            for (int w = 0; w < n_waves; ++w) v += part[w][threadIdx.x];
+            a.partial[item.slot * 2 + threadIdx.x] = v;
+            if (threadIdx.x == 0) a.finish[item.slot] = wall_clock64();
         }
+        __syncthreads();  // part is read before the next item writes it
     }
 }
 
-__global__ void reset_kernel(unsigned* counter, unsigned long long* start) {
-    *counter = 0u;
-    *start = wall_clock64();
-}
+__global__ void stamp_kernel(unsigned long long* start) { *start = wall_clock64(); }
 
 // sums[(t * n_scen + s) * 2 + k] = the slots of (t, s) added in chunk order.
 template <typename R>
@@ -250,7 +278,7 @@ struct Prepared {
 template <typename R>
 Prepared<R> prepare(const optlab_book_request& rq, const std::vector<double>& spot, int block, int ppl) {
     Prepared<R> pr;
-    const long long per_item = 64ll * ppl;  // an item is one wave's: 64 lanes x ppl pairs
+    const long long per_item = static_cast<long long>(block) * ppl;  // an item is one workgroup's
     for (int i = 0; i < rq.n_trades; ++i) {
         const optlab_trade& c = rq.trades[i];
         TradeDev<R> t{};
@@ -301,13 +329,13 @@ struct Device {
 };
 
 template <typename R>
-int resident_blocks(int block) {
+int resident_blocks(int block, size_t lds) {
     int per_cu = 0;
-    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, book_kernel<R>, block, 0), "occupancy");
+    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, book_kernel<R>, block, lds), "occupancy");
     return std::max(per_cu, 1) * device_cu_count();
 }
 
-// One launch of the queue `items` with its counter reset and its reduction (captured in a HIP graph and the graph
+// One launch of the queue `items` with its clock stamp and its reduction (captured in a HIP graph and the graph
 // launched, when `graph`), synchronised.
 template <typename R>
 float launch_queue(BookArgs<R> args, const std::vector<Item>& items, Item* d_items, int block, int n_trades,
@@ -317,11 +345,12 @@ float launch_queue(BookArgs<R> args, const std::vector<Item>& items, Item* d_ite
               "copy queue");
     args.items = d_items;
     args.n_items = static_cast<int>(items.size());
-    const int grid = std::max(1, std::min<int>(resident_blocks<R>(block), static_cast<int>(items.size())));
+    const size_t lds = 2 * static_cast<size_t>(kNames) * args.n_x * sizeof(R);
+    const int grid = std::max(1, std::min<int>(resident_blocks<R>(block, lds), static_cast<int>(items.size())));
     const unsigned red = static_cast<unsigned>((static_cast<long long>(n_trades) * n_scen + 255) / 256);
     auto body = [&]() {
-        hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(1), 0, st, args.counter, args.start);
-        hipLaunchKernelGGL(book_kernel<R>, dim3(grid), dim3(block), 0, st, args);
+        hipLaunchKernelGGL(stamp_kernel, dim3(1), dim3(1), 0, st, args.start);
+        hipLaunchKernelGGL(book_kernel<R>, dim3(grid), dim3(block), lds, st, args);
         hipLaunchKernelGGL(book_reduce_kernel<R>, dim3(red), dim3(256), 0, st, d_trades, n_trades, n_scen,
                            args.partial, d_sums);
     };
@@ -430,7 +459,7 @@ void run_typed(optlab_book_request& rq) {
         auto up = [](size_t b) { return (b + 255) & ~size_t(255); };  // every piece starts 256-byte aligned
         const size_t bytes = up(items.size() * sizeof(Item)) + up(pr.trades.size() * sizeof(TradeDev<R>)) +
                              up(slots * 2 * sizeof(double)) + up(slots * sizeof(unsigned long long)) +
-                             up(static_cast<size_t>(rq.n_trades) * nsc * 2 * sizeof(double)) + up(sizeof(unsigned)) +
+                             up(static_cast<size_t>(rq.n_trades) * nsc * 2 * sizeof(double)) +
                              up(sizeof(unsigned long long));
         unsigned char* m = static_cast<unsigned char*>(scratch(kSlotBookQueue, bytes));
         auto carve = [&](size_t bytes) {
@@ -443,13 +472,12 @@ void run_typed(optlab_book_request& rq) {
         double* d_partial = reinterpret_cast<double*>(carve(slots * 2 * sizeof(double)));
         unsigned long long* d_finish = reinterpret_cast<unsigned long long*>(carve(slots * sizeof(unsigned long long)));
         double* d_sums = reinterpret_cast<double*>(carve(static_cast<size_t>(rq.n_trades) * nsc * 2 * sizeof(double)));
-        unsigned* d_counter = reinterpret_cast<unsigned*>(carve(sizeof(unsigned)));
         unsigned long long* d_start = reinterpret_cast<unsigned long long*>(carve(sizeof(unsigned long long)));
         hip_check(hipMemcpyAsync(d_trades, pr.trades.data(), pr.trades.size() * sizeof(TradeDev<R>),
                                  hipMemcpyHostToDevice, st),
                   "copy trades");
         BookArgs<R> ar = args;
-        ar.trades = d_trades, ar.pairs_per_lane = pp, ar.counter = d_counter, ar.partial = d_partial;
+        ar.trades = d_trades, ar.pairs_per_lane = pp, ar.partial = d_partial;
         ar.finish = d_finish, ar.start = d_start;
         *ms_out = launch_queue(ar, items, d_items, blk, rq.n_trades, n_scen_run, d_trades, d_sums, graph);
         if (keep) *keep = pr;
@@ -464,7 +492,7 @@ void run_typed(optlab_book_request& rq) {
             block = hit->second.first, ppl = hit->second.second;
         } else {
             // the sample: every trade under one scenario (the queue's shape without its length)
-            const int cand[][2] = {{256, 2}, {256, 4}, {512, 4}, {256, 8}, {128, 4}, {512, 8}};
+            const int cand[][2] = {{256, 1}, {256, 2}, {512, 1}, {128, 2}, {256, 4}, {512, 2}};
             float best = 1e30f;
             for (const auto& c : cand) {
                 float ms = 0.f;
