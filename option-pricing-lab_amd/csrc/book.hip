@@ -80,7 +80,9 @@ struct BookArgs {
 // The rows of step `step` of every name of trade t under scenario s into LDS (n names x n_x), by the workgroup.
 template <typename R>
 __device__ __forceinline__ void stage_rows(const BookArgs<R>& a, const Shape& sh, int step, R* dst) {
-    for (int k = 0; k < sh.n; ++k) {
+#pragma unroll
+    for (int k = 0; k < kNames; ++k) {  // unrolled: the names stay in registers (a runtime index would put them on
+        if (k >= sh.n) break;          // the stack)
         const R* src = a.tables + ((static_cast<long long>(sh.scen) * a.n_book + sh.names[k]) * a.n_steps + step) * a.n_x;
         for (int c = threadIdx.x; c < a.n_x; c += blockDim.x) dst[k * a.n_x + c] = src[c];
     }
