@@ -165,18 +165,32 @@ __global__ void __launch_bounds__(kBlock) bin_kernel(LevArgs s, Particles<R> P) 
     for (int k = threadIdx.x; k < 5 * nb; k += blockDim.x) sp[k] = sums[k];
 }
 
+// Totals of the workgroups' bins: a workgroup takes 64 slots, its 4 waves each add every fourth workgroup's row
+// (loads overlapped: integer sums give the same bits in any order), then wave 0 adds the 4 partials.
 __global__ void __launch_bounds__(kBlock) reduce_bins_kernel(LevArgs s) {
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ unsigned long long part[kBlock / 64][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = kBlock / 64;
+    const int slot = blockIdx.x * 64 + lane;
     const int nf = s.n_fine, ns = 5 * s.n_bins;
+    unsigned long long t = 0;
     if (slot < nf) {
-        unsigned long long t = 0;
-        for (int b = 0; b < s.n_bin_blocks; ++b) t += s.fine_part[static_cast<long long>(b) * nf + slot];
-        s.fine_tot[slot] = t;
+#pragma unroll 8
+        for (int b = wave; b < s.n_bin_blocks; b += n_waves) t += s.fine_part[static_cast<long long>(b) * nf + slot];
     } else if (slot < nf + ns) {
         const int k = slot - nf;
-        long long t = 0;
-        for (int b = 0; b < s.n_bin_blocks; ++b) t += s.sum_part[static_cast<long long>(b) * ns + k];
-        s.sum_tot[k] = t;
+#pragma unroll 8
+        for (int b = wave; b < s.n_bin_blocks; b += n_waves)
+            t += static_cast<unsigned long long>(s.sum_part[static_cast<long long>(b) * ns + k]);
+    }
+    part[wave][lane] = t;
+    __syncthreads();
+    if (wave == 0 && slot < nf + ns) {
+        unsigned long long v = 0;
+        for (int w = 0; w < n_waves; ++w) v += part[w][lane];
+        if (slot < nf)
+            s.fine_tot[slot] = v;
+        else
+            s.sum_tot[slot - nf] = static_cast<long long>(v);
     }
 }
 
@@ -470,7 +484,7 @@ void run_typed(optlab_leverage_request& rq, int bw_rule, double bw_coef) {
     const unsigned init_grid = static_cast<unsigned>(std::min<long long>((n + kBlock - 1) / kBlock, 4LL * cus));
     hipLaunchKernelGGL(init_kernel<R>, dim3(init_grid), dim3(kBlock), 0, st, P, n, static_cast<R>(rq.v0));
     hip_check(hipGetLastError(), "init kernel launch");
-    const unsigned red_grid = static_cast<unsigned>((nf + 5 * nb + kBlock - 1) / kBlock);
+    const unsigned red_grid = static_cast<unsigned>((nf + 5 * nb + 63) / 64);  // 64 slots a workgroup
     for (int i = 0; i < rq.n_steps; ++i) {
         if (i > 0) {
             hipLaunchKernelGGL(bin_kernel<R>, dim3(s.n_bin_blocks), dim3(kBlock), 0, st, s, P);
