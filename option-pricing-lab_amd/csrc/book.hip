@@ -261,15 +261,6 @@ __global__ void book_reduce_kernel(const TradeDev<R>* trades, int n_trades, int 
 
 // ---- host -------------------------------------------------------------------------------------------------------
 
-struct Stream {
-    hipStream_t s = nullptr;
-    Stream() { hip_check(hipStreamCreate(&s), "hipStreamCreate"); }  // default priority
-};
-
-hipStream_t book_stream() {
-    static Stream st;
-    return st.s;
-}
 
 template <typename R>
 struct Prepared {
@@ -342,7 +333,7 @@ int resident_blocks(int block, size_t lds) {
 template <typename R>
 float launch_queue(BookArgs<R> args, const std::vector<Item>& items, Item* d_items, int block, int n_trades,
                    int n_scen, const TradeDev<R>* d_trades, double* d_sums, bool graph) {
-    hipStream_t st = book_stream();
+    hipStream_t st = engine_stream();
     hip_check(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, st),
               "copy queue");
     args.items = d_items;
@@ -393,7 +384,7 @@ template <typename R>
 void run_typed(optlab_book_request& rq) {
     const auto wall0 = std::chrono::steady_clock::now();
     const int nb = rq.n_names, ns = rq.n_steps, nsc = rq.n_scen;
-    hipStream_t st = book_stream();
+    hipStream_t st = engine_stream();
     const int prec = std::is_same<R, double>::value ? 0 : 1;
     const TentGrid grid = make_tent_grid();
 

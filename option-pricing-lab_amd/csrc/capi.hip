@@ -90,6 +90,15 @@ void* scratch(int slot, size_t bytes) {
     return b.ptr;
 }
 
+hipStream_t engine_stream() {
+    static hipStream_t s = [] {
+        hipStream_t x = nullptr;
+        hip_check(hipStreamCreate(&x), "hipStreamCreate");
+        return x;
+    }();
+    return s;
+}
+
 int device_cu_count() {
     static int cached = 0;
     if (!cached) {

@@ -43,6 +43,10 @@ enum Slot {
 
 int device_cu_count();
 
+// A stream of the library's own (created once, default priority), for work captured in HIP graphs: the null stream
+// cannot be captured.
+hipStream_t engine_stream();
+
 // ---- path engine (paths.hip) ----------------------------------------------------------------------------------
 // Launch the path kernel for every pair of `rq` in precision prec (0 double, 1 float), output to d_out (device,
 // n_names x n_cols x 2 n_pairs doubles) on `stream`. Returns the kernel time in ms (HIP events).

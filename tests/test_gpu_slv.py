@@ -43,3 +43,12 @@ def test_the_leverage_reprices_the_surface():
     r = bench.slv_reprice(model, 1 << 18, mats, bench.SLV_STRIKES)
     assert np.all(np.abs(r["price_gap"]) < 4 * r["se"] + 0.03), np.abs(r["price_gap"]).max()
     assert np.nanmax(np.abs(r["vol_bp"][:, 3:18])) < 100  # K 79..121: within a vol point
+
+
+def test_calibration_graph_launches_repeat_bit_for_bit():
+    """The calibration's steps run as one captured HIP graph: two calls, with other engine work between them, give
+    the same leverage bit for bit (integer bin sums, fixed-order regression)."""
+    a = bench.slv_calibrate(1 << 15, 0.5)
+    bench.slv_calibrate(1 << 12, 0.25)
+    b = bench.slv_calibrate(1 << 15, 0.5)
+    np.testing.assert_array_equal(np.asarray(a.leverage.values), np.asarray(b.leverage.values))
