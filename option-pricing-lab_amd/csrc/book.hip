@@ -78,10 +78,10 @@ struct BookArgs {
 };
 
 // The rows of step `step` of every name of trade t under scenario s into LDS (n names x n_x), by the workgroup.
-template <typename R>
+template <typename R, int NM>
 __device__ __forceinline__ void stage_rows(const BookArgs<R>& a, const Shape& sh, int step, R* dst) {
 #pragma unroll
-    for (int k = 0; k < kNames; ++k) {  // unrolled: the names stay in registers (a runtime index would put them on
+    for (int k = 0; k < NM; ++k) {  // unrolled: the names stay in registers (a runtime index would put them on
         if (k >= sh.n) break;          // the stack)
         const R* src = a.tables + ((static_cast<long long>(sh.scen) * a.n_book + sh.names[k]) * a.n_steps + step) * a.n_x;
         for (int c = threadIdx.x; c < a.n_x; c += blockDim.x) dst[k * a.n_x + c] = src[c];
@@ -91,19 +91,19 @@ __device__ __forceinline__ void stage_rows(const BookArgs<R>& a, const Shape& sh
 // Value of pair p of trade t under scenario s: the average of its two legs. Every lane of the workgroup calls it
 // together (lanes past the trade's pairs compute a pair they then drop): the step's table rows are staged in LDS,
 // double-buffered, with one barrier a step.
-template <typename R>
+template <typename R, int NM>
 __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const Shape& sh, long long p, R* rows) {
     NormalStream<R> rng;
     rng.init(static_cast<std::uint64_t>(p), t.key0, t.key1, 0u);
     const int n = sh.n, scen = sh.scen;
     const int stride = a.n_steps + 1;
-    R x[2][kNames];
-    R lsh[kNames];
-    const R* lo[kNames];
-    const R* inv[kNames];
-    const R* lf[kNames];
+    R x[2][NM];
+    R lsh[NM];
+    const R* lo[NM];
+    const R* inv[NM];
+    const R* lf[NM];
 #pragma unroll
-    for (int k = 0; k < kNames; ++k) {
+    for (int k = 0; k < NM; ++k) {
         x[0][k] = x[1][k] = R(0);
         const int nm = k < n ? sh.names[k] : sh.names[0];
         lsh[k] = a.lsh[scen * a.n_book + nm];
@@ -121,7 +121,7 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
     if (product == 3) {
         R y0 = lf[0][0] + lsh[0] - t.lfix[0];
 #pragma unroll
-        for (int k = 1; k < kNames; ++k)
+        for (int k = 1; k < NM; ++k)
             if (k < n) y0 = r_min(y0, lf[k][0] + lsh[k] - t.lfix[k]);
         ymin[0] = ymin[1] = y0;
     }
@@ -129,27 +129,27 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
     const int n_obs = product == 3 ? n_steps / obs_every : 0;
     const int stage_len = n * a.n_x;
     __syncthreads();  // the previous pass has read its last rows
-    stage_rows(a, sh, 0, rows);
+    stage_rows<R, NM>(a, sh, 0, rows);
     for (int step = 0; step < n_steps; ++step) {
         const R* cur = rows + (step & 1) * stage_len;
         __syncthreads();  // this step's rows are in; everyone is done with the last step's buffer
-        if (step + 1 < n_steps) stage_rows(a, sh, step + 1, rows + ((step + 1) & 1) * stage_len);
-        R zs[kNames];
+        if (step + 1 < n_steps) stage_rows<R, NM>(a, sh, step + 1, rows + ((step + 1) & 1) * stage_len);
+        R zs[NM];
 #pragma unroll
-        for (int k = 0; k < kNames; ++k)
+        for (int k = 0; k < NM; ++k)
             if (k < n) zs[k] = rng.next();
-        R var[kNames];  // the table reads do not depend on the leg's sign until x differs: read per leg
+        R var[NM];  // the table reads do not depend on the leg's sign until x differs: read per leg
         const int t1 = step + 1;
         const bool fix = t1 % obs_every == 0;
 #pragma unroll
         for (int leg = 0; leg < 2; ++leg) {
             const R sign = leg ? R(-1) : R(1);
 #pragma unroll
-            for (int k = 0; k < kNames; ++k) {
+            for (int k = 0; k < NM; ++k) {
                 if (k >= n) break;
                 R acc_z = R(0);
 #pragma unroll
-                for (int b = 0; b < kNames; ++b)
+                for (int b = 0; b < NM; ++b)
                     if (b < n) acc_z += t.root[k * n + b] * zs[b];
                 const R z = sign * acc_z;
                 const Lerp<R> l = lerp_at(cur + k * a.n_x, lo[k][step], inv[k][step], a.n_x, x[leg][k]);
@@ -164,7 +164,7 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
             } else if (product == 3) {
                 R yw = y - t.lfix[0];
 #pragma unroll
-                for (int k = 1; k < kNames; ++k)
+                for (int k = 1; k < NM; ++k)
                     if (k < n) yw = r_min(yw, lf[k][t1] + lsh[k] + x[leg][k] - t.lfix[k]);
                 ymin[leg] = r_min(ymin[leg], yw);
                 if (fix) observe(terms, obs_j == n_obs - 1, r_exp(yw), r_exp(ymin[leg]), a.disc[t1], note[leg]);
@@ -193,14 +193,15 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
     return 0.5 * v;
 }
 
-// Persistent: a grid of the resident workgroups walks the queue (ordered longest first) with a stride of the grid,
+// Persistent, one launch per class of trades by name count (NM: 1, 3 or 5, the bound the registers are sized
+// for): a grid of the resident workgroups walks its class's queue (ordered longest first) with a stride of the grid,
 // item it on workgroup it % grid. An item is blockDim x pairs_per_lane pairs of one (trade, scenario): every wave of
 // the workgroup reads the same staged rows. Every loop bound is the same in all lanes (kernel arguments and the item
 // index), so each barrier is reached by the whole workgroup. Each item's pair sums go to its own slot.
-template <typename R>
+template <typename R, int NM>
 __global__ void __launch_bounds__(kMaxBlock) book_kernel(BookArgs<R> a) {
     extern __shared__ unsigned char lds_raw[];
-    R* rows = reinterpret_cast<R*>(lds_raw);  // 2 x kNames x n_x
+    R* rows = reinterpret_cast<R*>(lds_raw);  // 2 x NM x n_x
     __shared__ double part[kMaxBlock / 64][2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
     for (long long it = blockIdx.x; it < a.n_items; it += gridDim.x) {
@@ -216,7 +217,7 @@ __global__ void __launch_bounds__(kMaxBlock) book_kernel(BookArgs<R> a) {
         for (int k = 0; k < a.pairs_per_lane; ++k) {
             const long long p = (static_cast<long long>(chunk) * a.pairs_per_lane + k) * blockDim.x + threadIdx.x;
             const bool live = p < t.n_pairs;
-            const double v = pair_value(a, t, sh, live ? p : 0, rows);
+            const double v = pair_value<R, NM>(a, t, sh, live ? p : 0, rows);
             if (live) {
                 s1 += v;
                 s2 += v * v;
@@ -321,29 +322,64 @@ struct Device {
     double* sums = nullptr;
 };
 
-template <typename R>
+template <typename R, int NM>
 int resident_blocks(int block, size_t lds) {
     int per_cu = 0;
-    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, book_kernel<R>, block, lds), "occupancy");
+    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, book_kernel<R, NM>, block, lds), "occupancy");
     return std::max(per_cu, 1) * device_cu_count();
 }
 
-// One launch of the queue `items` with its clock stamp and its reduction (captured in a HIP graph and the graph
-// launched, when `graph`), synchronised.
+constexpr int kClasses = 3;
+constexpr int kClassNames[kClasses] = {1, 3, 5};  // the name bound of each class's kernel
+
 template <typename R>
-float launch_queue(BookArgs<R> args, const std::vector<Item>& items, Item* d_items, int block, int n_trades,
-                   int n_scen, const TradeDev<R>* d_trades, double* d_sums, bool graph) {
+int class_of(const TradeDev<R>& t) {
+    return t.n <= 1 ? 0 : t.n <= 3 ? 1 : 2;
+}
+
+// One launch of the queue `items` with its clock stamp and its reduction (captured in a HIP graph and the graph
+// launched, when `graph`), synchronised. The items are split by the name count of their trade into up to three
+// persistent launches, each of the kernel whose registers fit that many names, each over its own longest-first
+// queue.
+template <typename R>
+float launch_queue(BookArgs<R> args, const std::vector<Item>& items, const std::vector<TradeDev<R>>& trades,
+                   Item* d_items, int block, int n_trades, int n_scen, const TradeDev<R>* d_trades, double* d_sums,
+                   bool graph) {
     hipStream_t st = engine_stream();
-    hip_check(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, st),
+    std::vector<Item> ordered;
+    ordered.reserve(items.size());
+    int count[kClasses] = {0, 0, 0}, first[kClasses] = {0, 0, 0};
+    for (int c = 0; c < kClasses; ++c) {
+        first[c] = static_cast<int>(ordered.size());
+        for (const Item& it : items)
+            if (class_of(trades[it.trade]) == c) ordered.push_back(it), ++count[c];
+    }
+    hip_check(hipMemcpyAsync(d_items, ordered.data(), ordered.size() * sizeof(Item), hipMemcpyHostToDevice, st),
               "copy queue");
-    args.items = d_items;
-    args.n_items = static_cast<int>(items.size());
-    const size_t lds = 2 * static_cast<size_t>(kNames) * args.n_x * sizeof(R);
-    const int grid = std::max(1, std::min<int>(resident_blocks<R>(block, lds), static_cast<int>(items.size())));
+    int grid[kClasses];
+    size_t lds[kClasses];
+    for (int c = 0; c < kClasses; ++c) {
+        lds[c] = 2 * static_cast<size_t>(kClassNames[c]) * args.n_x * sizeof(R);
+        const int resident = c == 0   ? resident_blocks<R, 1>(block, lds[c])
+                             : c == 1 ? resident_blocks<R, 3>(block, lds[c])
+                                      : resident_blocks<R, 5>(block, lds[c]);
+        grid[c] = std::max(1, std::min(resident, count[c]));
+    }
     const unsigned red = static_cast<unsigned>((static_cast<long long>(n_trades) * n_scen + 255) / 256);
     auto body = [&]() {
         hipLaunchKernelGGL(stamp_kernel, dim3(1), dim3(1), 0, st, args.start);
-        hipLaunchKernelGGL(book_kernel<R>, dim3(grid), dim3(block), lds, st, args);
+        for (int c = 0; c < kClasses; ++c) {
+            if (!count[c]) continue;
+            BookArgs<R> ac = args;
+            ac.items = d_items + first[c];
+            ac.n_items = count[c];
+            if (c == 0)
+                hipLaunchKernelGGL((book_kernel<R, 1>), dim3(grid[c]), dim3(block), lds[c], st, ac);
+            else if (c == 1)
+                hipLaunchKernelGGL((book_kernel<R, 3>), dim3(grid[c]), dim3(block), lds[c], st, ac);
+            else
+                hipLaunchKernelGGL((book_kernel<R, 5>), dim3(grid[c]), dim3(block), lds[c], st, ac);
+        }
         hipLaunchKernelGGL(book_reduce_kernel<R>, dim3(red), dim3(256), 0, st, d_trades, n_trades, n_scen,
                            args.partial, d_sums);
     };
@@ -472,7 +508,7 @@ void run_typed(optlab_book_request& rq) {
         BookArgs<R> ar = args;
         ar.trades = d_trades, ar.pairs_per_lane = pp, ar.partial = d_partial;
         ar.finish = d_finish, ar.start = d_start;
-        *ms_out = launch_queue(ar, items, d_items, blk, rq.n_trades, n_scen_run, d_trades, d_sums, graph);
+        *ms_out = launch_queue(ar, items, pr.trades, d_items, blk, rq.n_trades, n_scen_run, d_trades, d_sums, graph);
         if (keep) *keep = pr;
         return std::make_tuple(d_sums, d_finish, d_start);
     };
