@@ -291,6 +291,39 @@ def bench_slv(n_particles: int = 1 << 20, T: float = 3.0, n_paths: int = 1 << 20
     }
 
 
+def bench_fixings(n_paths: int = 1 << 18, reps: int = 5) -> dict:
+    """The fixing-heavy pass of the path engine: Black-Scholes (S0 100, r .03, q .01, sigma .2), 252 daily steps,
+    every close kept, 2^18 paths, output left in HBM (optlab_paths_dev): 8 B written per path, close and name, so the
+    roofline is HBM write bandwidth."""
+    import numpy as np
+    import torch
+
+    from optlab import native
+    from optlab.core.market import Market
+    from optlab.core.models import BlackScholesModel
+
+    market = Market.flat(100.0, 0.03, 0.01)
+    times = [k / 252 for k in range(1, 253)]
+    p = native.plan(BlackScholesModel(0.2), market, times)
+    out = {}
+    for backend in ("fp32", "fp64"):
+        buf, _ = native.run_device(p, n_paths // 2, 42, backend=backend)  # warm-up, and the output buffer
+        ms = []
+        for _ in range(reps):
+            _, t = native.run_device(p, n_paths // 2, 42, backend=backend, out=buf)
+            ms.append(t["kernel_ms"])
+        torch.cuda.synchronize()
+        k = float(np.median(ms))
+        bytes_out = buf.numel() * 8
+        out[backend] = {"kernel_ms": k, "written_GB": bytes_out / 1e9, "write_GB_per_s": bytes_out / (k * 1e-3) / 1e9,
+                        "frac_of_8TBps": bytes_out / (k * 1e-3) / 8e12,
+                        "path_steps_per_s": n_paths * len(times) / (k * 1e-3)}
+        del buf
+    out["workload"] = (f"Black-Scholes daily closes: {n_paths} paths x 252 steps, every close kept (1 x 252 x "
+                       f"{n_paths} doubles in HBM)")
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -303,7 +336,7 @@ def main(argv=None) -> int:
     ap.add_argument("--book-scale", type=int, default=16,
                     help="divide every trade's paths of the batched book by this (0: skip the book)")
     ap.add_argument("--slv-particles", type=int, default=1 << 20, help="SLV particles (0: skip the SLV block)")
-    ap.add_argument("--only", choices=("adjoint", "book", "slv"), default=None,
+    ap.add_argument("--only", choices=("adjoint", "book", "slv", "fixings"), default=None,
                     help="run only this block and print its JSON (for profiling runs; the benchmark line needs all)")
     ap.add_argument("--no-bumps", action="store_true", help="with --only adjoint: skip the bump-and-revalue comparator")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -327,7 +360,8 @@ def main(argv=None) -> int:
     if args.only:
         block = {"adjoint": lambda: bench_adjoint(args.adjoint_paths, bumps=not args.no_bumps),
                  "book": lambda: bench_book(args.book_scale),
-                 "slv": lambda: bench_slv(args.slv_particles)}[args.only]()
+                 "slv": lambda: bench_slv(args.slv_particles),
+                 "fixings": lambda: bench_fixings()}[args.only]()
         print(json.dumps({args.only: block}), flush=True)
         return 0
     ms32, kern32, p32, se32 = _time_gpu(lib, 1, args.steps, args.warmup)
@@ -400,6 +434,8 @@ def main(argv=None) -> int:
     if args.slv_particles:
         line["slv"] = bench_slv(args.slv_particles)
         print("bench: slv block done", file=sys.stderr, flush=True)
+    line["fixings"] = bench_fixings()
+    print("bench: fixings block done", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
     if args.copy_mib:
         src = torch.ones(args.copy_mib << 20, dtype=torch.uint8, device="cuda")
