@@ -85,13 +85,13 @@ __global__ void __launch_bounds__(kFwdBlock) autocall_forward(AcArgs<R> s) {
     R x[N];
 #pragma unroll
     for (int a = 0; a < N; ++a) x[a] = R(0);
-    R ymin = s.lf[0];
+    R ymin = ldc(s.lf, 0);
     int at = 0;
 #pragma unroll
     for (int a = 1; a < N; ++a)
-        if (s.lf[a * stride] < ymin) ymin = s.lf[a * stride], at = a;
+        if (ldc(s.lf, a * stride) < ymin) ymin = ldc(s.lf, a * stride), at = a;
     Note<R> note{R(1), R(0), R(0)};
-    int j = 0, next = s.obs_step[0];
+    int j = 0, next = ldc(s.obs_step, 0);
 
     for (int step = 0; step < s.n_steps; ++step) {
         if (s.greeks && step % kSeg == 0 && active) {
@@ -106,15 +106,16 @@ __global__ void __launch_bounds__(kFwdBlock) autocall_forward(AcArgs<R> s) {
 #pragma unroll
         for (int a = 0; a < N; ++a) {
             const R* y = s.table[a] + static_cast<long long>(step) * s.n_x;
-            const Lerp<R> l = lerp_at(y, static_cast<R>(s.lo[a][step]), static_cast<R>(s.inv[a][step]), s.n_x, x[a]);
+            const Lerp<R> l =
+                lerp_at(y, static_cast<R>(ldc(s.lo[a], step)), static_cast<R>(ldc(s.inv[a], step)), s.n_x, x[a]);
             x[a] += R(-0.5) * l.var * s.dt + r_sqrt(l.var * s.dt) * z[a];
         }
         const int t1 = step + 1;
-        R yw = s.lf[t1] + x[0];
+        R yw = ldc(s.lf, t1) + x[0];
         int w = 0;
 #pragma unroll
         for (int a = 1; a < N; ++a) {
-            const R ya = s.lf[a * stride + t1] + x[a];
+            const R ya = ldc(s.lf, a * stride + t1) + x[a];
             if (ya < yw) yw = ya, w = a;
         }
         if (yw < ymin) ymin = yw, at = t1 * 8 + w;
@@ -125,9 +126,9 @@ __global__ void __launch_bounds__(kFwdBlock) autocall_forward(AcArgs<R> s) {
                 s.perf[static_cast<long long>(j) * s.n_legs + i] = p;
                 s.worst[static_cast<long long>(j) * s.n_legs + i] = static_cast<unsigned char>(w);
             }
-            observe(s.pr, last, p, r_exp(ymin), s.disc[j], note);
+            observe(s.pr, last, p, r_exp(ymin), ldc(s.disc, j), note);
             ++j;
-            next = j < s.n_obs ? s.obs_step[j] : -1;
+            next = j < s.n_obs ? ldc(s.obs_step, j) : -1;
         }
     }
 
@@ -149,7 +150,7 @@ __global__ void __launch_bounds__(kFwdBlock) autocall_forward(AcArgs<R> s) {
                 alive *= R(1) - ramp_ge(pq, t.autocall, t.smoothing).v;
             }
             const long long o = static_cast<long long>(jj) * s.n_legs + i;
-            const R p = s.perf[o], D = s.disc[jj];
+            const R p = s.perf[o], D = ldc(s.disc, jj);
             const Ramp<R> c = ramp_ge(p, t.coupon_barrier, t.smoothing);
             R pbar = R(0);
             if (jj == s.n_obs - 1) {
@@ -275,8 +276,8 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
                     xs[k][a] = x[a];
                     zz[k][a] = z[a];
                     const R* y = s.table[a] + static_cast<long long>(step) * s.n_x;
-                    const Lerp<R> l =
-                        lerp_at(y, static_cast<R>(s.lo[a][step]), static_cast<R>(s.inv[a][step]), s.n_x, x[a]);
+                    const Lerp<R> l = lerp_at(y, static_cast<R>(ldc(s.lo[a], step)), static_cast<R>(ldc(s.inv[a], step)),
+                                              s.n_x, x[a]);
                     x[a] += R(-0.5) * l.var * s.dt + r_sqrt(l.var * s.dt) * z[a];
                 }
             }
@@ -286,7 +287,7 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
             for (int k = kSeg - 1; k >= 0; --k) {
                 if (k >= len) continue;
                 const int step = s0 + k, t1 = step + 1;
-                const int j = s.obs_of[t1];
+                const int j = ldc(s.obs_of, t1);
                 if (j >= 0) {
                     const long long o = static_cast<long long>(j) * s.n_legs + leg;
                     const int w = s.worst[o];
@@ -303,8 +304,8 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
 #pragma unroll
                 for (int a = 0; a < N; ++a) {
                     const R* y = s.table[a] + static_cast<long long>(step) * s.n_x;
-                    const Lerp<R> l =
-                        lerp_at(y, static_cast<R>(s.lo[a][step]), static_cast<R>(s.inv[a][step]), s.n_x, xs[k][a]);
+                    const Lerp<R> l = lerp_at(y, static_cast<R>(ldc(s.lo[a], step)), static_cast<R>(ldc(s.inv[a], step)),
+                                              s.n_x, xs[k][a]);
                     const R sd = r_sqrt(l.var * s.dt);
                     const R vbar = xb[a] * s.dt * (R(-0.5) + R(0.5) * zz[k][a] / sd);
                     if (active) {

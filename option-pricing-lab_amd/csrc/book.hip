@@ -94,7 +94,7 @@ __device__ __forceinline__ void stage_rows(const BookArgs<R>& a, const Shape& sh
 template <typename R, int NM>
 __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const Shape& sh, long long p, R* rows) {
     NormalStream<R> rng;
-    rng.init(static_cast<std::uint64_t>(p), t.key0, t.key1, 0u);
+    rng.init(static_cast<std::uint64_t>(p), ldc(&t.key0, 0), ldc(&t.key1, 0), 0u);
     const int n = sh.n, scen = sh.scen;
     const int stride = a.n_steps + 1;
     R x[2][NM];
@@ -106,7 +106,7 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
     for (int k = 0; k < NM; ++k) {
         x[0][k] = x[1][k] = R(0);
         const int nm = k < n ? sh.names[k] : sh.names[0];
-        lsh[k] = a.lsh[scen * a.n_book + nm];
+        lsh[k] = ldc(a.lsh, scen * a.n_book + nm);
         lo[k] = a.lo + static_cast<long long>(nm) * a.n_steps;
         inv[k] = a.inv + static_cast<long long>(nm) * a.n_steps;
         lf[k] = a.lf + static_cast<long long>(nm) * stride;
@@ -115,14 +115,15 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
     R acc[2] = {R(0), R(0)};        // Asian: sum of fixings; barrier: running max of ln S
     R ymin[2] = {R(0), R(0)};       // autocall: lowest ln performance over every close
     Note<R> note[2] = {{R(1), R(0), R(0)}, {R(1), R(0), R(0)}};
-    const Terms<R> terms{t.autocall, t.coupon_barrier, t.coupon, t.barrier, t.put_strike, R(0), sh.memory};
+    const Terms<R> terms{ldc(&t.autocall, 0), ldc(&t.coupon_barrier, 0), ldc(&t.coupon, 0), ldc(&t.barrier, 0),
+                         ldc(&t.put_strike, 0), R(0), sh.memory};
     const int product = sh.product, n_steps = sh.n_steps, obs_every = sh.obs_every;
     if (product == 2) acc[0] = acc[1] = R(-1e30);
     if (product == 3) {
-        R y0 = lf[0][0] + lsh[0] - t.lfix[0];
+        R y0 = ldc(lf[0], 0) + lsh[0] - ldc(t.lfix, 0);
 #pragma unroll
         for (int k = 1; k < NM; ++k)
-            if (k < n) y0 = r_min(y0, lf[k][0] + lsh[k] - t.lfix[k]);
+            if (k < n) y0 = r_min(y0, ldc(lf[k], 0) + lsh[k] - ldc(t.lfix, k));
         ymin[0] = ymin[1] = y0;
     }
     int obs_j = 0;
@@ -150,41 +151,42 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
                 R acc_z = R(0);
 #pragma unroll
                 for (int b = 0; b < NM; ++b)
-                    if (b < n) acc_z += t.root[k * n + b] * zs[b];
+                    if (b < n) acc_z += ldc(t.root, k * n + b) * zs[b];
                 const R z = sign * acc_z;
-                const Lerp<R> l = lerp_at(cur + k * a.n_x, lo[k][step], inv[k][step], a.n_x, x[leg][k]);
+                const Lerp<R> l = lerp_at(cur + k * a.n_x, ldc(lo[k], step), ldc(inv[k], step), a.n_x, x[leg][k]);
                 var[k] = l.var;
                 x[leg][k] += R(-0.5) * var[k] * a.dt + r_sqrt(var[k] * a.dt) * z;
             }
-            const R y = lf[0][t1] + lsh[0] + x[leg][0];  // ln S of the first name
+            const R y = ldc(lf[0], t1) + lsh[0] + x[leg][0];  // ln S of the first name
             if (product == 1) {
                 if (fix) acc[leg] += r_exp(y);
             } else if (product == 2) {
                 acc[leg] = r_max(acc[leg], y);
             } else if (product == 3) {
-                R yw = y - t.lfix[0];
+                R yw = y - ldc(t.lfix, 0);
 #pragma unroll
                 for (int k = 1; k < NM; ++k)
-                    if (k < n) yw = r_min(yw, lf[k][t1] + lsh[k] + x[leg][k] - t.lfix[k]);
+                    if (k < n) yw = r_min(yw, ldc(lf[k], t1) + lsh[k] + x[leg][k] - ldc(t.lfix, k));
                 ymin[leg] = r_min(ymin[leg], yw);
-                if (fix) observe(terms, obs_j == n_obs - 1, r_exp(yw), r_exp(ymin[leg]), a.disc[t1], note[leg]);
+                if (fix) observe(terms, obs_j == n_obs - 1, r_exp(yw), r_exp(ymin[leg]), ldc(a.disc, t1), note[leg]);
             }
         }
         if (product == 3 && fix) ++obs_j;
     }
-    const R D = a.disc[n_steps];
+    const R D = ldc(a.disc, n_steps);
+    const R strike = ldc(&t.strike, 0);
     double v = 0.0;
 #pragma unroll
     for (int leg = 0; leg < 2; ++leg) {
-        const R ST = r_exp(lf[0][n_steps] + lsh[0] + x[leg][0]);
+        const R ST = r_exp(ldc(lf[0], n_steps) + lsh[0] + x[leg][0]);
         R pay;
         if (product == 0) {
-            pay = D * r_max(ST - t.strike, R(0));
+            pay = D * r_max(ST - strike, R(0));
         } else if (product == 1) {
             const R avg = acc[leg] / static_cast<R>(n_steps / obs_every);
-            pay = D * r_max(avg - t.strike, R(0));
+            pay = D * r_max(avg - strike, R(0));
         } else if (product == 2) {
-            pay = acc[leg] < r_log(t.barrier) ? D * r_max(ST - t.strike, R(0)) : R(0);
+            pay = acc[leg] < r_log(terms.ki_barrier) ? D * r_max(ST - strike, R(0)) : R(0);
         } else {
             pay = note[leg].value;
         }
@@ -205,18 +207,21 @@ __global__ void __launch_bounds__(kMaxBlock) book_kernel(BookArgs<R> a) {
     __shared__ double part[kMaxBlock / 64][2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
     for (long long it = blockIdx.x; it < a.n_items; it += gridDim.x) {
-        const Item item = a.items[it];
-        const TradeDev<R>& t = a.trades[uniform(item.trade)];
+        const Item* item = a.items + it;
+        const TradeDev<R>& t = a.trades[uniform(ldc(&item->trade, 0))];
         Shape sh;
-        sh.product = uniform(t.product), sh.n = uniform(t.n), sh.n_steps = uniform(t.n_steps);
-        sh.obs_every = uniform(t.obs_every), sh.memory = uniform(t.memory), sh.scen = uniform(item.scen);
+        sh.product = uniform(ldc(&t.product, 0)), sh.n = uniform(ldc(&t.n, 0)), sh.n_steps = uniform(ldc(&t.n_steps, 0));
+        sh.obs_every = uniform(ldc(&t.obs_every, 0)), sh.memory = uniform(ldc(&t.memory, 0));
+        sh.scen = uniform(ldc(&item->scen, 0));
 #pragma unroll
-        for (int k = 0; k < kNames; ++k) sh.names[k] = uniform(t.names[k]);
-        const int chunk = uniform(item.chunk);
+        for (int k = 0; k < kNames; ++k) sh.names[k] = uniform(ldc(t.names, k));
+        const int chunk = uniform(ldc(&item->chunk, 0));
+        const long long slot = ldc(&item->slot, 0);
+        const long long n_pairs = ldc(&t.n_pairs, 0);
         double s1 = 0.0, s2 = 0.0;
         for (int k = 0; k < a.pairs_per_lane; ++k) {
             const long long p = (static_cast<long long>(chunk) * a.pairs_per_lane + k) * blockDim.x + threadIdx.x;
-            const bool live = p < t.n_pairs;
+            const bool live = p < n_pairs;
             const double v = pair_value<R, NM>(a, t, sh, live ? p : 0, rows);
             if (live) {
                 s1 += v;
@@ -233,8 +238,8 @@ __global__ void __launch_bounds__(kMaxBlock) book_kernel(BookArgs<R> a) {
         if (threadIdx.x < 2) {
             double v = 0.0;
             for (int w = 0; w < n_waves; ++w) v += part[w][threadIdx.x];
-            a.partial[item.slot * 2 + threadIdx.x] = v;
-            if (threadIdx.x == 0) a.finish[item.slot] = wall_clock64();
+            a.partial[slot * 2 + threadIdx.x] = v;
+            if (threadIdx.x == 0) a.finish[slot] = wall_clock64();
         }
         __syncthreads();  // part is read before the next item writes it
     }

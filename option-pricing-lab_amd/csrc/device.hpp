@@ -32,6 +32,18 @@ __device__ __forceinline__ U4 philox10(U4 c, std::uint32_t k0, std::uint32_t k1)
     return c;
 }
 
+// ---- kernel-constant reads ------------------------------------------------------------------------------------
+// Element i of an array that no kernel writes while the reading kernel runs (request data, tables built by an
+// earlier launch), read through the constant address space. With a wave-uniform index the compiler issues a scalar
+// load whose wait counts scalar and LDS traffic only; the same read through a generic pointer is a vector load,
+// because the compiler cannot prove the kernel's own stores leave it alone, and waiting on a vector load also waits
+// for every store the lane has pending. With a divergent index it is an ordinary vector load.
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p, long long i) {
+    typedef const __attribute__((address_space(4))) T* ConstPtr;
+    return ((ConstPtr)p)[i];
+}
+
 // ---- math in the path precision ---------------------------------------------------------------------------------
 __device__ __forceinline__ float r_exp(float x) { return expf(x); }
 __device__ __forceinline__ double r_exp(double x) { return exp(x); }

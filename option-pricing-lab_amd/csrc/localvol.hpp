@@ -84,7 +84,7 @@ __device__ __forceinline__ void correlate(const R* root, R sign, const R (&zs)[N
     for (int a = 0; a < N; ++a) {
         R acc = R(0);
 #pragma unroll
-        for (int b = 0; b < N; ++b) acc += root[a * N + b] * zs[b];
+        for (int b = 0; b < N; ++b) acc += ldc(root, a * N + b) * zs[b];
         z[a] = sign * acc;
     }
 }

@@ -59,7 +59,7 @@ template <typename R>
 __device__ __forceinline__ void stage_rows(const PathArgs<R>& s, int step_index, R* buf) {
     for (int t = 0; t < s.n_tab; ++t) {
         const int a = s.tab_of[t];
-        const int row = s.step_row[a * s.n_steps + step_index];
+        const int row = ldc(s.step_row, a * s.n_steps + step_index);
         const R* src = s.table + static_cast<long long>(row) * s.n_x;
         R* dst = buf + t * s.n_x;
         for (int c = threadIdx.x; c < s.n_x; c += blockDim.x) dst[c] = src[c];
@@ -87,11 +87,11 @@ template <typename R, int N>
 __device__ __forceinline__ void record(const PathArgs<R>& s, long long p, bool active, int j, const R (&x)[2][N],
                                        Extremes<R, N>& e) {
     const long long row0 = p, row1 = s.n_pairs + p;
-    const int c = s.col_of[j];
+    const int c = ldc(s.col_of, j);
 #pragma unroll
     for (int a = 0; a < N; ++a) {
         if (a >= s.n_names) break;
-        const R f = s.fwd[a * s.n_obs + j];
+        const R f = ldc(s.fwd, a * s.n_obs + j);
         const R S0 = f * r_exp(x[0][a]), S1 = f * r_exp(x[1][a]);
         double* base = s.out + static_cast<long long>(a) * s.n_cols * s.rows;
         if (c >= 0 && active) {
@@ -135,14 +135,14 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
     int kind[N];
 #pragma unroll
     for (int a = 0; a < N; ++a) {
-        kind[a] = a < n ? s.kind[a] : 0;
+        kind[a] = a < n ? ldc(s.kind, a) : 0;
         const bool hv = kind[a] == 2 || kind[a] == 3;
         x[0][a] = x[1][a] = R(0);
-        v[0][a] = v[1][a] = (a < n && hv) ? s.hes[a * 5] : R(0);
+        v[0][a] = v[1][a] = (a < n && hv) ? ldc(s.hes, a * 5) : R(0);
     }
     Extremes<R, N> ext;
     int j = 0;
-    while (j < s.n_obs && s.obs_step[j] == 0) record<R, N>(s, p, active, j++, x, ext);
+    while (j < s.n_obs && ldc(s.obs_step, j) == 0) record<R, N>(s, p, active, j++, x, ext);
 
     NormalStream<R> rng, aux;
     rng.init(pair, s.key0, s.key1, 0u);
@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
             __syncthreads();
             if (i < s.n_steps) stage_rows(s, i, lds + (i & 1) * stage_len);
         }
-        const R dt = s.dt[i - 1], t0 = s.t0[i - 1];
+        const R dt = ldc(s.dt, i - 1), t0 = ldc(s.t0, i - 1);
         R zs[N], zr[N], fresh = R(0);
 #pragma unroll
         for (int a = 0; a < N; ++a)
@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
                     R acc = R(0);
 #pragma unroll
                     for (int b = 0; b < N; ++b)
-                        if (b < n) acc += s.low_root[a * n + b] * other[b];
+                        if (b < n) acc += ldc(s.low_root, a * n + b) * other[b];
                     z[a] = sw * common + sl * acc;
                 }
             } else {
@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
                     R acc = R(0);
 #pragma unroll
                     for (int b = 0; b < N; ++b)
-                        if (b < n) acc += s.root[a * n + b] * zs[b];
+                        if (b < n) acc += ldc(s.root, a * n + b) * zs[b];
                     z[a] = sign * acc;
                 }
             }
@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
                         R acc = R(0);
 #pragma unroll
                         for (int b = 0; b < N; ++b)
-                            if (b < n) acc += s.root[a * n + b] * zr[b];
+                            if (b < n) acc += ldc(s.root, a * n + b) * zr[b];
                         w[a] = sign * acc;
                     } else {
                         w[a] = sign * zr[a];
@@ -232,25 +232,25 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
                 R var;
                 const R vp = r_max(v[leg][a], R(0));
                 if (k == 0) {
-                    var = s.cvar[a];
+                    var = ldc(s.cvar, a);
                 } else if (k == 2) {
                     var = vp;
                 } else {
-                    const int row = s.step_row[a * s.n_steps + (i - 1)];
+                    const int row = ldc(s.step_row, a * s.n_steps + (i - 1));
                     const R* y = s.staged ? cur + s.tab_slot[a] * s.n_x : s.table + static_cast<long long>(row) * s.n_x;
-                    const R t = lerp_row(y, s.lo[row], s.inv_dx[row], s.n_x, x[leg][a]);
+                    const R t = lerp_row(y, ldc(s.lo, row), ldc(s.inv_dx, row), s.n_x, x[leg][a]);
                     var = k == 1 ? t : t * vp;
                 }
                 if (k == 2 || k == 3) {
-                    const R* h = s.hes + a * 5;
-                    const R rho = h[4];
+                    const R kappa = ldc(s.hes, a * 5 + 1), theta = ldc(s.hes, a * 5 + 2);
+                    const R xi = ldc(s.hes, a * 5 + 3), rho = ldc(s.hes, a * 5 + 4);
                     const R zv = rho * z[a] + r_sqrt(R(1) - rho * rho) * w[a];
-                    v[leg][a] += h[1] * (h[2] - vp) * dt + h[3] * r_sqrt(vp * dt) * zv;
+                    v[leg][a] += kappa * (theta - vp) * dt + xi * r_sqrt(vp * dt) * zv;
                 }
                 x[leg][a] += R(-0.5) * var * dt + r_sqrt(var * dt) * z[a];
             }
         }
-        while (j < s.n_obs && s.obs_step[j] == i) record<R, N>(s, p, active, j++, x, ext);
+        while (j < s.n_obs && ldc(s.obs_step, j) == i) record<R, N>(s, p, active, j++, x, ext);
     }
 }
 
