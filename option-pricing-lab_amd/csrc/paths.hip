@@ -82,77 +82,85 @@ struct Extremes {
     R lo[2][N][2], hi[2][N][2];
 };
 
-// paths.hpp:241-272.
+// paths.hpp:241-272, for the two legs of a lane's pair (output rows row[0], row[1]).
 template <typename R, int N>
-__device__ __forceinline__ void record(const PathArgs<R>& s, long long p, bool active, int j, const R (&x)[2][N],
-                                       Extremes<R, N>& e) {
-    const long long row0 = p, row1 = s.n_pairs + p;
+__device__ __forceinline__ void record(const PathArgs<R>& s, const long long (&row)[2], bool active, int j,
+                                       const R (&x)[2][N], Extremes<R, N>& e) {
     const int c = ldc(s.col_of, j);
 #pragma unroll
     for (int a = 0; a < N; ++a) {
         if (a >= s.n_names) break;
         const R f = ldc(s.fwd, a * s.n_obs + j);
-        const R S0 = f * r_exp(x[0][a]), S1 = f * r_exp(x[1][a]);
+        R S[2];
+#pragma unroll
+        for (int l = 0; l < 2; ++l) S[l] = f * r_exp(x[l][a]);
         double* base = s.out + static_cast<long long>(a) * s.n_cols * s.rows;
         if (c >= 0 && active) {
-            base[c * s.rows + row0] = static_cast<double>(S0);
-            base[c * s.rows + row1] = static_cast<double>(S1);
+#pragma unroll
+            for (int l = 0; l < 2; ++l) base[c * s.rows + row[l]] = static_cast<double>(S[l]);
         }
 #pragma unroll
         for (int w = 0; w < 2; ++w) {
             if (w >= s.n_win || j < s.win_lo[w] || j > s.win_hi[w]) continue;
-            if (j == s.win_lo[w]) {
-                e.lo[0][a][w] = e.hi[0][a][w] = S0;
-                e.lo[1][a][w] = e.hi[1][a][w] = S1;
-            } else {
-                e.lo[0][a][w] = r_min(e.lo[0][a][w], S0);
-                e.hi[0][a][w] = r_max(e.hi[0][a][w], S0);
-                e.lo[1][a][w] = r_min(e.lo[1][a][w], S1);
-                e.hi[1][a][w] = r_max(e.hi[1][a][w], S1);
+#pragma unroll
+            for (int l = 0; l < 2; ++l) {
+                if (j == s.win_lo[w]) {
+                    e.lo[l][a][w] = e.hi[l][a][w] = S[l];
+                } else {
+                    e.lo[l][a][w] = r_min(e.lo[l][a][w], S[l]);
+                    e.hi[l][a][w] = r_max(e.hi[l][a][w], S[l]);
+                }
             }
             if (j == s.win_hi[w] && active) {
                 const long long cl = s.n_keep + 2 * w, ch = cl + 1;
-                base[cl * s.rows + row0] = static_cast<double>(e.lo[0][a][w]);
-                base[ch * s.rows + row0] = static_cast<double>(e.hi[0][a][w]);
-                base[cl * s.rows + row1] = static_cast<double>(e.lo[1][a][w]);
-                base[ch * s.rows + row1] = static_cast<double>(e.hi[1][a][w]);
+#pragma unroll
+                for (int l = 0; l < 2; ++l) {
+                    base[cl * s.rows + row[l]] = static_cast<double>(e.lo[l][a][w]);
+                    base[ch * s.rows + row[l]] = static_cast<double>(e.hi[l][a][w]);
+                }
             }
         }
     }
 }
 
-// paths.hpp:274-361: both legs of pair p through every step.
-template <typename R, int N>
+// paths.hpp:274-361: lane p carries both legs of pair p (output rows p and n_pairs + p).
+// CV: every name has a constant variance and there is no local-correlation mixer (Black-Scholes names and baskets),
+// so the model dispatch, the variance factor and the table staging compile away: the same arithmetic on fewer
+// scalar instructions a step.
+template <typename R, int N, bool CV>
 __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
     extern __shared__ unsigned char lds_raw[];
     R* lds = reinterpret_cast<R*>(lds_raw);
     const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool active = p < s.n_pairs;
     const int n = s.n_names;
+    const long long row[2] = {p, s.n_pairs + p};
     const std::uint64_t pair = static_cast<std::uint64_t>(s.first_pair + (active ? p : 0));
 
-    R x[2][N], v[2][N];
+    R x[2][N], v[2][N], cvar[N];
     int kind[N];
 #pragma unroll
     for (int a = 0; a < N; ++a) {
         kind[a] = a < n ? ldc(s.kind, a) : 0;
+        cvar[a] = a < n ? ldc(s.cvar, a) : R(0);
         const bool hv = kind[a] == 2 || kind[a] == 3;
         x[0][a] = x[1][a] = R(0);
         v[0][a] = v[1][a] = (a < n && hv) ? ldc(s.hes, a * 5) : R(0);
     }
+    const bool staged = !CV && s.staged, has_var = !CV && s.has_var, mixer = !CV && s.mixer;
     Extremes<R, N> ext;
     int j = 0;
-    while (j < s.n_obs && ldc(s.obs_step, j) == 0) record<R, N>(s, p, active, j++, x, ext);
+    while (j < s.n_obs && ldc(s.obs_step, j) == 0) record<R, N>(s, row, active, j++, x, ext);
 
     NormalStream<R> rng, aux;
     rng.init(pair, s.key0, s.key1, 0u);
     aux.init(pair, s.key0, s.key1, kAuxTag);
     const int stage_len = s.n_tab * s.n_x;
-    if (s.staged && s.n_steps > 0) stage_rows(s, 0, lds);
+    if (staged && s.n_steps > 0) stage_rows(s, 0, lds);
 
     for (int i = 1; i <= s.n_steps; ++i) {
         const R* cur = lds + ((i - 1) & 1) * stage_len;
-        if (s.staged) {
+        if (staged) {
             __syncthreads();
             if (i < s.n_steps) stage_rows(s, i, lds + (i & 1) * stage_len);
         }
@@ -161,17 +169,17 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
 #pragma unroll
         for (int a = 0; a < N; ++a)
             if (a < n) zs[a] = rng.next();
-        if (s.has_var) {
+        if (has_var) {
 #pragma unroll
             for (int a = 0; a < N; ++a)
                 if (a < n) zr[a] = s.has_slv ? aux.next() : rng.next();
         }
-        if (s.mixer) fresh = rng.next();
+        if (mixer) fresh = rng.next();
 #pragma unroll
         for (int leg = 0; leg < 2; ++leg) {
             const R sign = leg ? R(-1) : R(1);
             R z[N], w[N];
-            if (s.mixer) {
+            if (mixer) {
                 R m = R(0);
 #pragma unroll
                 for (int a = 0; a < N; ++a)
@@ -210,7 +218,7 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
                     z[a] = sign * acc;
                 }
             }
-            if (s.has_var) {
+            if (has_var) {
 #pragma unroll
                 for (int a = 0; a < N; ++a) {
                     if (a >= n) break;
@@ -228,16 +236,16 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
 #pragma unroll
             for (int a = 0; a < N; ++a) {
                 if (a >= n) break;
-                const int k = kind[a];
+                const int k = CV ? 0 : kind[a];
                 R var;
                 const R vp = r_max(v[leg][a], R(0));
                 if (k == 0) {
-                    var = ldc(s.cvar, a);
+                    var = cvar[a];
                 } else if (k == 2) {
                     var = vp;
                 } else {
                     const int row = ldc(s.step_row, a * s.n_steps + (i - 1));
-                    const R* y = s.staged ? cur + s.tab_slot[a] * s.n_x : s.table + static_cast<long long>(row) * s.n_x;
+                    const R* y = staged ? cur + s.tab_slot[a] * s.n_x : s.table + static_cast<long long>(row) * s.n_x;
                     const R t = lerp_row(y, ldc(s.lo, row), ldc(s.inv_dx, row), s.n_x, x[leg][a]);
                     var = k == 1 ? t : t * vp;
                 }
@@ -250,7 +258,7 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
                 x[leg][a] += R(-0.5) * var * dt + r_sqrt(var * dt) * z[a];
             }
         }
-        while (j < s.n_obs && ldc(s.obs_step, j) == i) record<R, N>(s, p, active, j++, x, ext);
+        while (j < s.n_obs && ldc(s.obs_step, j) == i) record<R, N>(s, row, active, j++, x, ext);
     }
 }
 
@@ -273,10 +281,18 @@ struct Arena {
     size_t ints(const int* src, long long n) { return put(src, src ? static_cast<size_t>(n) * sizeof(int) : 0); }
 };
 
+template <typename R, int N, bool CV>
+void launch_cv(const PathArgs<R>& s, size_t lds, hipStream_t stream) {
+    const unsigned grid = static_cast<unsigned>((s.n_pairs + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL((paths_kernel<R, N, CV>), dim3(grid), dim3(kBlock), lds, stream, s);
+}
+
 template <typename R, int N>
 void launch_n(const PathArgs<R>& s, size_t lds, hipStream_t stream) {
-    const unsigned grid = static_cast<unsigned>((s.n_pairs + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL((paths_kernel<R, N>), dim3(grid), dim3(kBlock), lds, stream, s);
+    if (s.n_tab == 0 && !s.has_var && !s.mixer)
+        launch_cv<R, N, true>(s, lds, stream);
+    else
+        launch_cv<R, N, false>(s, lds, stream);
 }
 
 template <typename R>

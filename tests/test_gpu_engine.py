@@ -144,6 +144,27 @@ def test_batches_do_not_change_the_paths(lib, backend):
         np.testing.assert_array_equal(whole[k][5000:6234], a[k][1234:])
 
 
+@pytest.mark.parametrize("case", ["bs", "lv", "heston", "basket_slv_localcorr"])
+def test_a_large_grid_gives_the_paths_of_a_small_one(lib, case):
+    """The first 2000 pairs of a 2^18-pair run (1024 workgroups, the constant-variance kernel for Black-Scholes)
+    against a 2000-pair run, FP64 and FP32, windows included: the pair index alone fixes a path."""
+    market, models = _models()
+    p = native.plan(models[case], market, [0.25, 0.5], dt_max=1 / 40)
+    big, m = 1 << 18, 2000
+    for be in ("fp64", "fp32"):
+        whole, ext_w, _ = native.run(p, big, 21, backend=be, windows=[(0, 1)])
+        part, ext_p, _ = native.run(p, m, 21, backend=be, windows=[(0, 1)])
+        for k in range(len(whole)):
+            got = np.concatenate([whole[k][:m], whole[k][big:big + m]])
+            rel = np.abs(got / part[k] - 1)
+            if be == "fp64":
+                assert np.median(rel) < 1e-14 and np.mean(rel > 1e-12) < 1e-3, (case, k, rel.max())
+            else:
+                assert np.median(rel) < 1e-6 and np.mean(rel > 1e-4) < 0.01, (case, k, rel.max())
+            lo_w = np.concatenate([ext_w[k][0][0][:m], ext_w[k][0][0][big:big + m]])
+            assert np.median(np.abs(lo_w / ext_p[k][0][0] - 1)) < (1e-14 if be == "fp64" else 1e-6)
+
+
 def test_windows_on_the_gpu_equal_the_min_and_max_of_the_columns(lib):
     market, models = _models()
     times = np.linspace(0.02, 0.5, 25)
