@@ -174,8 +174,12 @@ def bench_adjoint(n_paths: int, seed: int = 42, reps: int = 3, bumps: bool = Tru
     import numpy as np
 
     note = autocall_note()
-    note.adjoint(n_paths, seed)  # warm-up: tables built, buffers sized
+    chosen = note.adjoint(n_paths, seed).seconds["checkpoint_every"]  # warm-up; the autotuner picks the interval
     note.price(n_paths, seed)
+    by_interval = {}
+    for k in (4, 8):  # the candidates for 3 names, each timed once more (forward + backward stage, device events)
+        t = note.adjoint(n_paths, seed, checkpoint_every=k).seconds
+        by_interval[str(k)] = t["forward_ms"] + t["backward_ms"]
     walls, fwd, bwd = [], [], []
     for _ in range(reps):
         t0 = time.perf_counter()
@@ -187,7 +191,8 @@ def bench_adjoint(n_paths: int, seed: int = 42, reps: int = 3, bumps: bool = Tru
     if not bumps:
         note.close()
         return {"adjoint_ms": t_adj * 1e3, "adjoint_forward_kernel_ms": float(np.median(fwd)),
-                "adjoint_backward_ms": float(np.median(bwd)), "price": g.price, "se": g.se}
+                "adjoint_backward_ms": float(np.median(bwd)), "checkpoint_every": chosen,
+                "device_ms_by_checkpoint_interval": by_interval, "price": g.price, "se": g.se}
     t0 = time.perf_counter()
     b = note.bump(n_paths, seed)
     t_bump = time.perf_counter() - t0
@@ -206,6 +211,8 @@ def bench_adjoint(n_paths: int, seed: int = 42, reps: int = 3, bumps: bool = Tru
         "adjoint_ms": t_adj * 1e3,
         "adjoint_forward_kernel_ms": float(np.median(fwd)),
         "adjoint_backward_ms": float(np.median(bwd)),
+        "checkpoint_every": chosen,
+        "device_ms_by_checkpoint_interval": by_interval,
         "bump_ms": t_bump * 1e3,
         "revaluations": b.seconds["revaluations"],
         "speedup_time": t_bump / t_adj,

@@ -219,6 +219,10 @@ typedef struct {
     double* vega;            /* host or NULL: n_names x OPTLAB_GRID_NODES, per vol point (optlab_adjoint) */
     double* vega_se;         /* host or NULL: the same shape */
     double forward_ms, backward_ms, total_ms;
+    int checkpoint_every;    /* optlab_adjoint: steps between checkpoints of the forward state, 4, 8 (up to 3
+                                names) or 16 (one name), where the segment's rows fit in LDS; 0: the autotuner
+                                times the candidates on the first adjoint of a shape and keeps the fastest */
+    int checkpoint_used;     /* out: the interval the adjoint ran with (0 for a price) */
 } optlab_autocall_request;
 
 /* ---- addition: a book of trades revalued under scenarios, in one persistent launch --------------------------- */

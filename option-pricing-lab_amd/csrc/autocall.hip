@@ -10,10 +10,10 @@
 // MI355X design:
 //   forward   one leg per lane (lanes 2p, 2p + 1 are the two legs of pair p, so a wave reduces pair averages with
 //             one shuffle), path state in VGPRs, the step's table rows read through the L1/L2 (every lane of the
-//             launch reads the same rows at the same step); with Greeks on, the state every kSeg steps goes to a
+//             launch reads the same rows at the same step); with Greeks on, the state every SEG steps goes to a
 //             checkpoint array (structure of arrays, coalesced) and the payoff's adjoint seeds are stored per leg.
-//   backward  persistent workgroups; segments of kSeg steps from the last to the first; within a segment each
-//             workgroup recomputes its legs' kSeg steps from the checkpoint, walks them backwards and adds the
+//   backward  persistent workgroups; segments of SEG steps from the last to the first; within a segment each
+//             workgroup recomputes its legs' SEG steps from the checkpoint, walks them backwards and adds the
 //             adjoint of every table entry it read into an LDS copy of the segment's rows as 64-bit fixed point
 //             (integer adds: the result does not depend on the order, so the Greeks are bit-reproducible); the
 //             rows are then written once to the workgroup's private table, and a reduction sums the workgroups
@@ -22,6 +22,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "device.hpp"
@@ -40,7 +41,9 @@ template <typename R>
 constexpr int bwd_block() {
     return sizeof(R) == 4 ? 1024 : 256;
 }
-constexpr int kSeg = 4;  // steps between checkpoints: with n normals a step, kSeg n normals fill whole Philox blocks
+// SEG, the steps between checkpoints, is a multiple of 4, so a segment's SEG n normals fill whole Philox blocks and
+// the backward pass can start a segment's stream at its block. Candidates: 4; 8 up to 3 names; 16 for one name
+// (the segment's LDS rows, SEG x n x n_x x 8 B, must fit in a workgroup's 160 KiB).
 constexpr int kBatches = 16;
 constexpr int kMaxNames = 5;
 constexpr int kMaxObs = 64;
@@ -71,7 +74,7 @@ struct AcArgs {
     double* partials;     // forward workgroups x 16
 };
 
-template <typename R, int N>
+template <typename R, int N, int SEG>
 __global__ void __launch_bounds__(kFwdBlock) autocall_forward(AcArgs<R> s) {
     __shared__ double wave_part[kFwdBlock / 64][16];
     const long long i = static_cast<long long>(blockIdx.x) * kFwdBlock + threadIdx.x;
@@ -94,8 +97,8 @@ __global__ void __launch_bounds__(kFwdBlock) autocall_forward(AcArgs<R> s) {
     int j = 0, next = ldc(s.obs_step, 0);
 
     for (int step = 0; step < s.n_steps; ++step) {
-        if (s.greeks && step % kSeg == 0 && active) {
-            const long long c = step / kSeg;
+        if (s.greeks && step % SEG == 0 && active) {
+            const long long c = step / SEG;
 #pragma unroll
             for (int a = 0; a < N; ++a) s.ck[(c * N + a) * s.n_legs + i] = x[a];
         }
@@ -234,19 +237,19 @@ __device__ __forceinline__ long long to_fixed(R v) {
     return static_cast<long long>(rint(v * static_cast<R>(kFixed)));
 }
 
-template <typename R, int N>
+template <typename R, int N, int SEG>
 __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s, long long n_chunks, R* xbar,
                                                                long long* partial) {
     constexpr int kBwdBlock = bwd_block<R>();
     extern __shared__ unsigned char lds_raw[];
-    unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds_raw);  // kSeg x N x n_x
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds_raw);  // SEG x N x n_x
     const long long G = gridDim.x, blk = blockIdx.x;
     const long long c0 = blk * n_chunks / G, c1 = (blk + 1) * n_chunks / G;
-    const int n_seg = (s.n_steps + kSeg - 1) / kSeg;
+    const int n_seg = (s.n_steps + SEG - 1) / SEG;
     const int row_len = N * s.n_x;
     for (int seg = n_seg - 1; seg >= 0; --seg) {
-        const int s0 = seg * kSeg, len = min(kSeg, s.n_steps - s0);
-        for (int e = threadIdx.x; e < kSeg * row_len; e += kBwdBlock) acc[e] = 0ull;
+        const int s0 = seg * SEG, len = min(SEG, s.n_steps - s0);
+        for (int e = threadIdx.x; e < SEG * row_len; e += kBwdBlock) acc[e] = 0ull;
         __syncthreads();
         for (long long chunk = c0; chunk < c1; ++chunk) {
             const long long i = chunk * kBwdBlock + threadIdx.x;
@@ -261,10 +264,10 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
             }
             NormalStream<R> rng;
             rng.init(static_cast<std::uint64_t>(s.first_pair + (leg >> 1)), s.key0, s.key1, 0u);
-            rng.block = static_cast<std::uint32_t>(seg * (kSeg * N / 4));
-            R xs[kSeg][N], zz[kSeg][N];
+            rng.block = static_cast<std::uint32_t>(seg * (SEG * N / 4));
+            R xs[SEG][N], zz[SEG][N];
 #pragma unroll
-            for (int k = 0; k < kSeg; ++k) {
+            for (int k = 0; k < SEG; ++k) {
                 if (k >= len) break;
                 const int step = s0 + k;
                 R zs[N], z[N];
@@ -284,7 +287,7 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
             const int lat = s.lat[leg];
             const R lseed = s.lseed[leg];
 #pragma unroll
-            for (int k = kSeg - 1; k >= 0; --k) {
+            for (int k = SEG - 1; k >= 0; --k) {
                 if (k >= len) continue;
                 const int step = s0 + k, t1 = step + 1;
                 const int j = ldc(s.obs_of, t1);
@@ -394,8 +397,9 @@ void mean_se(double sum, double sum2, double n, double& mean, double& se) {  // 
     se = std::sqrt(var / n);
 }
 
-template <typename R, int N>
+template <typename R, int N, int SEG>
 void run_typed(optlab_autocall_request& rq, bool adjoint) {
+    static_assert(SEG % 4 == 0, "a segment's normals must fill whole Philox blocks");
     const auto wall0 = std::chrono::steady_clock::now();
     const int n = rq.n_names, ns = rq.n_steps;
     const long long n_legs = 2 * rq.n_pairs;
@@ -480,7 +484,7 @@ void run_typed(optlab_autocall_request& rq, bool adjoint) {
     s.greeks = adjoint ? 1 : 0;
 
     const long long fwd_blocks = (n_legs + kFwdBlock - 1) / kFwdBlock;
-    const int n_seg = (ns + kSeg - 1) / kSeg;
+    const int n_seg = (ns + SEG - 1) / SEG;
     const size_t align = 256;
     auto up = [&](size_t b) { return (b + align - 1) & ~(align - 1); };
     const size_t part_bytes = up(static_cast<size_t>(fwd_blocks) * 16 * sizeof(double));
@@ -505,7 +509,7 @@ void run_typed(optlab_autocall_request& rq, bool adjoint) {
 
     Events ev;
     hip_check(hipEventRecord(ev.e[0], stream), "event");
-    hipLaunchKernelGGL((autocall_forward<R, N>), dim3(static_cast<unsigned>(fwd_blocks)), dim3(kFwdBlock), 0, stream,
+    hipLaunchKernelGGL((autocall_forward<R, N, SEG>), dim3(static_cast<unsigned>(fwd_blocks)), dim3(kFwdBlock), 0, stream,
                        s);
     hip_check(hipGetLastError(), "autocall forward launch");
     hipLaunchKernelGGL(batch_sums_kernel, dim3(1), dim3(kBatches * 16), 0, stream, partials, fwd_blocks, batches);
@@ -515,17 +519,17 @@ void run_typed(optlab_autocall_request& rq, bool adjoint) {
     std::vector<double> node_out;
     long long bwd_blocks = 0, n_chunks = 0;
     if (adjoint) {
-        const size_t lds = static_cast<size_t>(kSeg) * N * n_x * sizeof(unsigned long long);
+        const size_t lds = static_cast<size_t>(SEG) * N * n_x * sizeof(unsigned long long);
         int max_lds = 0;
         int dev_id = 0;
         hip_check(hipGetDevice(&dev_id), "hipGetDevice");
         hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_id), "attribute");
         if (lds > static_cast<size_t>(max_lds)) invalid("n_x too large for the adjoint's LDS rows");
-        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(autocall_backward<R, N>),
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(autocall_backward<R, N, SEG>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
                   "LDS attribute");
         int per_cu = 0;
-        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, autocall_backward<R, N>, kBwdBlock, lds),
+        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, autocall_backward<R, N, SEG>, kBwdBlock, lds),
                   "occupancy");
         n_chunks = (n_legs + kBwdBlock - 1) / kBwdBlock;
         bwd_blocks = std::min<long long>(n_chunks, static_cast<long long>(std::max(per_cu, 1)) * device_cu_count());
@@ -537,7 +541,7 @@ void run_typed(optlab_autocall_request& rq, bool adjoint) {
             scratch(kSlotBatches, static_cast<size_t>(kBatches) * entries * sizeof(long long) +
                                       static_cast<size_t>(n) * kBatches * n_nodes * sizeof(double)));
         double* vega_dev = reinterpret_cast<double*>(batch + kBatches * entries);
-        hipLaunchKernelGGL((autocall_backward<R, N>), dim3(static_cast<unsigned>(bwd_blocks)), dim3(kBwdBlock), lds,
+        hipLaunchKernelGGL((autocall_backward<R, N, SEG>), dim3(static_cast<unsigned>(bwd_blocks)), dim3(kBwdBlock), lds,
                            stream, s, n_chunks, xbar, partial);
         hip_check(hipGetLastError(), "autocall backward launch");
         const unsigned rb = static_cast<unsigned>(std::min<long long>((entries + 255) / 256, 8192));
@@ -606,17 +610,84 @@ void run_typed(optlab_autocall_request& rq, bool adjoint) {
     }
     rq.forward_ms = ev.ms(0, 1);
     rq.backward_ms = adjoint ? ev.ms(1, 2) : 0.0;
+    rq.checkpoint_used = adjoint ? SEG : 0;
     rq.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+}
+
+// The checkpoint intervals compiled for N names (a segment's rows, SEG x N x 801 x 8 B at the risk grid's n_x,
+// within 160 KiB of LDS).
+template <int N>
+constexpr bool seg_compiled(int seg) {
+    return seg == 4 || (seg == 8 && N <= 3) || (seg == 16 && N == 1);
+}
+
+template <typename R, int N>
+void run_seg(optlab_autocall_request& rq, bool adjoint, int seg) {
+    if (seg == 8 && N <= 3) return run_typed<R, N, (N <= 3 ? 8 : 4)>(rq, adjoint);
+    if (seg == 16 && N == 1) return run_typed<R, N, (N == 1 ? 16 : 4)>(rq, adjoint);
+    return run_typed<R, N, 4>(rq, adjoint);
+}
+
+// Autotuned checkpoint interval per shape (precision, names, steps, legs, n_x): more steps a segment means fewer
+// checkpoints to write and read and fewer segment passes, but more state held in registers while a segment is walked
+// back and more LDS a workgroup. The first adjoint of a shape with checkpoint_every = 0 runs every candidate twice
+// (the same request: the outputs are overwritten by the call that follows) and keeps the fastest.
+struct SegKey {
+    int prec, n, n_steps, n_x;
+    long long n_legs;
+    bool operator<(const SegKey& o) const {
+        if (prec != o.prec) return prec < o.prec;
+        if (n != o.n) return n < o.n;
+        if (n_steps != o.n_steps) return n_steps < o.n_steps;
+        if (n_x != o.n_x) return n_x < o.n_x;
+        return n_legs < o.n_legs;
+    }
+};
+std::map<SegKey, int> g_seg;
+
+template <typename R, int N>
+int pick_seg(optlab_autocall_request& rq) {
+    const int n_x = static_cast<int>(surface_rows(rq.surface[0]).n_x);
+    const SegKey key{sizeof(R) == 4 ? 1 : 0, N, rq.n_steps, n_x, 2 * rq.n_pairs};
+    const auto it = g_seg.find(key);
+    if (it != g_seg.end()) return it->second;
+    int dev_id = 0, max_lds = 0;
+    hip_check(hipGetDevice(&dev_id), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_id), "attribute");
+    int best = 4;
+    double best_ms = 0.0;
+    for (const int seg : {4, 8, 16}) {
+        if (!seg_compiled<N>(seg) || static_cast<size_t>(seg) * N * n_x * 8 > static_cast<size_t>(max_lds)) continue;
+        double ms = 0.0;
+        for (int rep = 0; rep < 2; ++rep) {
+            optlab_autocall_request t = rq;
+            run_seg<R, N>(t, true, seg);
+            const double m = t.forward_ms + t.backward_ms;
+            ms = rep == 0 ? m : std::min(ms, m);
+        }
+        if (seg == 4 || ms < best_ms) best = seg, best_ms = ms;
+    }
+    g_seg[key] = best;
+    return best;
+}
+
+template <typename R, int N>
+void run_names(optlab_autocall_request& rq, bool adjoint) {
+    if (!adjoint) return run_typed<R, N, 4>(rq, false);  // a price takes no checkpoints
+    int seg = rq.checkpoint_every;
+    if (seg == 0) seg = pick_seg<R, N>(rq);
+    if (!seg_compiled<N>(seg)) invalid("checkpoint_every must be 0, 4, 8 (up to 3 names) or 16 (one name)");
+    run_seg<R, N>(rq, true, seg);
 }
 
 template <typename R>
 void run_prec(optlab_autocall_request& rq, bool adjoint) {
     switch (rq.n_names) {
-        case 1: return run_typed<R, 1>(rq, adjoint);
-        case 2: return run_typed<R, 2>(rq, adjoint);
-        case 3: return run_typed<R, 3>(rq, adjoint);
-        case 4: return run_typed<R, 4>(rq, adjoint);
-        default: return run_typed<R, 5>(rq, adjoint);
+        case 1: return run_names<R, 1>(rq, adjoint);
+        case 2: return run_names<R, 2>(rq, adjoint);
+        case 3: return run_names<R, 3>(rq, adjoint);
+        case 4: return run_names<R, 4>(rq, adjoint);
+        default: return run_names<R, 5>(rq, adjoint);
     }
 }
 
@@ -643,6 +714,9 @@ void check_autocall_request(const optlab_autocall_request& c, bool adjoint) {
     if (c.n_pairs < 1 || c.first_pair < 0) invalid("n_pairs must be >= 1 and first_pair >= 0");
     if (c.n_pairs > (1ll << 40)) invalid("n_pairs too large");
     if (adjoint && c.nodes) invalid("adjoint Greeks are taken at the unbumped surfaces: nodes must be NULL");
+    const int k = c.checkpoint_every;
+    if (adjoint && !(k == 0 || k == 4 || (k == 8 && c.n_names <= 3) || (k == 16 && c.n_names == 1)))
+        invalid("checkpoint_every must be 0, 4, 8 (up to 3 names) or 16 (one name)");
     int n_x = -1;
     for (int a = 0; a < c.n_names; ++a) {
         const SurfaceRows& r = surface_rows(c.surface[a]);

@@ -235,6 +235,8 @@ class AutocallRequest(C.Structure):
         ("forward_ms", _D),
         ("backward_ms", _D),
         ("total_ms", _D),
+        ("checkpoint_every", _I),
+        ("checkpoint_used", _I),
     ]
 
 

@@ -143,17 +143,22 @@ class WorstOfAutocall:
         self._call(self._lib.optlab_autocall, rq)
         return rq.price, rq.se, np.array(rq.batch[:]), {"forward_ms": rq.forward_ms, "total_ms": rq.total_ms}
 
-    def adjoint(self, n_paths: int, seed: int, backend: str = "gpu", first_pair: int = 0) -> Greeks:
-        """Price, deltas and every node vega from one forward and one adjoint pass."""
+    def adjoint(self, n_paths: int, seed: int, backend: str = "gpu", first_pair: int = 0,
+                checkpoint_every: int = 0) -> Greeks:
+        """Price, deltas and every node vega from one forward and one adjoint pass. `checkpoint_every`: steps
+        between checkpoints of the forward state (4, 8 up to 3 names, 16 for one name); 0 lets the engine's
+        autotuner choose per shape (timings["checkpoint_every"] is the interval used)."""
         rq = self.request(n_paths, seed, first_pair)
         rq._prec = precision_code(backend)
+        rq.checkpoint_every = int(checkpoint_every)
         shape = (self.n_names, _abi.GRID_STRIKES, _abi.GRID_TENORS)
         vega, vega_se = np.empty(shape), np.empty(shape)
         rq.vega, rq.vega_se = _abi.ptr(vega), _abi.ptr(vega_se)
         self._call(self._lib.optlab_adjoint, rq)
         n = self.n_names
         return Greeks(rq.price, rq.se, np.array(rq.delta[:n]), np.array(rq.delta_se[:n]), vega, vega_se, n_paths,
-                      {"forward_ms": rq.forward_ms, "backward_ms": rq.backward_ms, "total_ms": rq.total_ms})
+                      {"forward_ms": rq.forward_ms, "backward_ms": rq.backward_ms, "total_ms": rq.total_ms,
+                       "checkpoint_every": rq.checkpoint_used})
 
     def bump(self, n_paths: int, seed: int, backend: str = "gpu", h_spot: float = 0.01, h_vol: float = 0.01,
              nodes=None) -> Greeks:

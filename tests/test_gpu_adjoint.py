@@ -173,6 +173,34 @@ def test_adjoint_greeks_are_bit_reproducible():
     np.testing.assert_array_equal(g1.vega_se, g2.vega_se)
 
 
+def test_checkpoint_intervals_give_the_same_greeks_and_the_autotuner_picks_one():
+    """Every 4 or every 8 steps (3 names): the same price and deltas (the forward pass is the same) and the same
+    node vegas up to the 2^-32 fixed point of the table adjoints; with checkpoint_every = 0 the autotuner runs one
+    of the candidates, and its Greeks are that candidate's."""
+    nt = note()
+    g4 = nt.adjoint(1 << 14, 31, backend="fp64", checkpoint_every=4)
+    g8 = nt.adjoint(1 << 14, 31, backend="fp64", checkpoint_every=8)
+    assert g4.seconds["checkpoint_every"] == 4 and g8.seconds["checkpoint_every"] == 8
+    assert g8.price == pytest.approx(g4.price, rel=1e-12)
+    np.testing.assert_allclose(g8.delta, g4.delta, rtol=1e-12)
+    np.testing.assert_allclose(g8.vega, g4.vega, rtol=1e-9, atol=1e-12)
+    by = {k: nt.adjoint(1 << 14, 31, backend="fp32", checkpoint_every=k) for k in (4, 8)}
+    g = nt.adjoint(1 << 14, 31, backend="fp32")
+    k = g.seconds["checkpoint_every"]
+    assert k in by
+    assert g.price == by[k].price
+    np.testing.assert_array_equal(g.vega, by[k].vega)
+    # FP32: the same paths, so the two intervals differ by far less than the vegas' standard errors
+    assert np.all(np.abs(by[8].vega - by[4].vega) <= 0.05 * by[4].vega_se + 1e-9)
+    rq = nt.request(1024, 1)
+    rq._prec = 1
+    rq.checkpoint_every = 16  # one name only
+    from optlab.core.engines import EngineError
+
+    with pytest.raises(EngineError, match="checkpoint_every"):
+        nt._call(nt._lib.optlab_adjoint, rq)
+
+
 def test_autograd_function_carries_the_adjoint():
     import torch
 
