@@ -49,7 +49,11 @@ __device__ __forceinline__ float r_exp(float x) { return expf(x); }
 __device__ __forceinline__ double r_exp(double x) { return exp(x); }
 __device__ __forceinline__ float r_log(float x) { return logf(x); }
 __device__ __forceinline__ double r_log(double x) { return log(x); }
-__device__ __forceinline__ float r_sqrt(float x) { return sqrtf(x); }
+// FP32 square root: the hardware v_sqrt_f32 (1 ulp; denormals flushed), as -prec-sqrt=false gave the reference's
+// GPU build. Named explicitly: sqrtf carries its accuracy as metadata, which the optimizer drops where it merges
+// the calls of several branches, and without it the compiler expands the correctly rounded sequence (an rsq and
+// Newton steps, about 15 instructions).
+__device__ __forceinline__ float r_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double r_sqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float r_erfc(float x) { return erfcf(x); }
 __device__ __forceinline__ double r_erfc(double x) { return erfc(x); }
@@ -66,7 +70,7 @@ __device__ __forceinline__ R unit01(std::uint32_t i) {
 
 __device__ __forceinline__ void gauss2(std::uint32_t a, std::uint32_t b, float& z0, float& z1) {
     const float u = unit01<float>(a), v = unit01<float>(b);
-    const float radius = sqrtf(-2.0f * logf(u));
+    const float radius = r_sqrt(-2.0f * logf(u));
     float s, c;
     sincospif(2.0f * v, &s, &c);
     z0 = radius * c;
