@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "device.hpp"
@@ -171,6 +172,83 @@ __device__ __forceinline__ void qe_step<float>(const Consts<float>& c, float& y,
     v = vn;
 }
 
+// Both legs of a pair, one FP32 QE-M step (legs 0 and 1 take normals (zv, zx) and (-zv, -zx)). On the strip's
+// Heston workload about one leg-step in six takes the exponential branch (psi > psi_c), but nearly every wave has
+// lanes on both sides, so the per-leg step ran both branches for each leg. Here the quadratic branch runs per leg,
+// the exponential branch once, on the leg of each lane that needs it (leg 0 when both do), and a second time only
+// for lanes whose two legs both need it (about one wave in a hundred has such a lane). Each leg gets the arithmetic
+// of qe_step<float>; a pair-step issues one exponential branch less.
+struct QeBranch {
+    float vn, lin, w, arg;
+    bool finite;
+};
+
+__device__ __forceinline__ QeBranch qe_quadratic(const Consts<float>& c, float m, float s2, float m2, float zv) {
+    QeBranch o;
+    const float inv = 2.0f * m2 / s2;
+    const float b2 = inv - 1.0f + sqrtf(inv * fmaxf(inv - 1.0f, 0.0f));
+    const float a = m / (1.0f + b2), b = sqrtf(b2);
+    o.vn = a * (b + zv) * (b + zv);
+    o.arg = 1.0f - 2.0f * c.A * a;
+    o.finite = o.arg > 0.0f;
+    o.lin = -c.A * b2 * a / o.arg;
+    o.w = 0.5f;
+    return o;
+}
+
+__device__ __forceinline__ QeBranch qe_exponential(const Consts<float>& c, float m, float s2, float m2, float zv) {
+    QeBranch o;
+    const float den = 1.0f / (s2 + m2);
+    const float p = (s2 - m2) * den, q = 2.0f * m2 * den, beta = 2.0f * m * den;
+    const float one_minus_u = 0.5f * erfc_fast(zv * 0.70710678f);
+    o.vn = one_minus_u >= q ? 0.0f : logf(q / one_minus_u) * (0.5f * (s2 + m2) / m);
+    o.arg = p + beta * q / (beta - c.A);
+    o.finite = c.A < beta;
+    o.lin = 0.0f;
+    o.w = -1.0f;
+    return o;
+}
+
+__device__ __forceinline__ void qe_tail(const Consts<float>& c, float& y, float& v, const QeBranch& o, float zx) {
+    const float K0 = o.finite ? o.lin + o.w * logf(o.arg) - (c.K1 + 0.5f * c.K3) * v : c.K0p;
+    y += c.mu_dt + K0 + c.K1 * v + c.K2 * o.vn + sqrtf(fmaxf(c.K3 * v + c.K4 * o.vn, 0.0f)) * zx;
+    v = o.vn;
+}
+
+__device__ __forceinline__ void qe_pair(const Consts<float>& c, float (&y)[2], float (&v)[2], float zv, float zx) {
+    float m[2], s2[2], m2[2];
+    bool ex[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        m[a] = c.theta + (v[a] - c.theta) * c.E;
+        s2[a] = v[a] * c.c1 + c.c2;
+        m2[a] = m[a] * m[a];
+        ex[a] = !(s2[a] <= c.psi_c * m2[a]);
+    }
+    QeBranch o[2] = {};
+    if (!ex[0]) o[0] = qe_quadratic(c, m[0], s2[0], m2[0], zv);
+    if (!ex[1]) o[1] = qe_quadratic(c, m[1], s2[1], m2[1], -zv);
+    if (ex[0] || ex[1]) {
+        const bool first = ex[0];  // the leg this pass serves: 0 when leg 0 needs it, else 1
+        const QeBranch r = qe_exponential(c, first ? m[0] : m[1], first ? s2[0] : s2[1], first ? m2[0] : m2[1],
+                                          first ? zv : -zv);
+        if (first)
+            o[0] = r;
+        else
+            o[1] = r;
+    }
+    if (ex[0] && ex[1]) o[1] = qe_exponential(c, m[1], s2[1], m2[1], -zv);
+    qe_tail(c, y[0], v[0], o[0], zx);
+    qe_tail(c, y[1], v[1], o[1], -zx);
+}
+
+// The pair step from code templated on the precision: the FP32 one above; FP64 keeps the reference's per-leg
+// order of operations (qe_step<double>), so this overload is never called.
+__device__ __forceinline__ void qe_pair_any(const Consts<float>& c, float (&y)[2], float (&v)[2], float zv, float zx) {
+    qe_pair(c, y, v, zv, zx);
+}
+__device__ __forceinline__ void qe_pair_any(const Consts<double>&, double (&)[2], double (&)[2], double, double) {}
+
 // schemes.hpp:254-311: European / Asian / down-and-out (+ up-and-out) pair averages; out[4] = out[0] out[1].
 template <typename R, int MODEL, int PRODUCT>
 __device__ __forceinline__ void fixed_unit(const Consts<R>& c, NormalStream<R>& src, R out[5]) {
@@ -195,13 +273,15 @@ __device__ __forceinline__ void fixed_unit(const Consts<R>& c, NormalStream<R>& 
             } else {
                 z = src.next();
             }
+            if (MODEL == 1 && std::is_same<R, float>::value) qe_pair_any(c, y, v, zv, zx);
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
                 const R sg = a == 0 ? R(1) : R(-1);
-                if (MODEL == 1)
-                    qe_step(c, y[a], v[a], sg * zv, sg * zx);
-                else
+                if (MODEL == 1) {
+                    if (!std::is_same<R, float>::value) qe_step(c, y[a], v[a], sg * zv, sg * zx);
+                } else {
                     y[a] += c.bs_drift + c.bs_vol * sg * z;
+                }
                 if (PRODUCT == 1) {
                     sum_s[a] += r_exp(y[a]);
                     sum_y[a] += y[a];
@@ -306,7 +386,11 @@ __device__ __forceinline__ R strip_value(const StripSlot<R>& t, R S0, R S1) {
 // One path step of both legs from normals z (z[0], z[1] for Heston: Zv then Zx; z[0] for Black-Scholes).
 template <typename R, int MODEL>
 __device__ __forceinline__ void strip_step(const Consts<R>& c, R& y0, R& y1, R& v0, R& v1, R za, R zb) {
-    if (MODEL == 1) {
+    if (MODEL == 1 && std::is_same<R, float>::value) {
+        R y[2] = {y0, y1}, v[2] = {v0, v1};
+        qe_pair_any(c, y, v, za, zb);
+        y0 = y[0], y1 = y[1], v0 = v[0], v1 = v[1];
+    } else if (MODEL == 1) {
         qe_step(c, y0, v0, za, zb);
         qe_step(c, y1, v1, -za, -zb);
     } else {
