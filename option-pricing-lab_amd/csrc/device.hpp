@@ -56,6 +56,15 @@ __device__ __forceinline__ double r_log(double x) { return log(x); }
 __device__ __forceinline__ float r_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double r_sqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float r_erfc(float x) { return erfcf(x); }
+
+// FP32 reciprocal, natural logarithm and exponential by the hardware instructions (v_rcp_f32, v_log_f32 = log2,
+// v_exp_f32 = exp2; about 1 ulp each, 2 with the scaling by ln 2), the accuracy the reference's FP32 GPU build asked
+// for (-prec-div=false). The library forms add range handling that these call sites never need: denominators far
+// below 2^96 in magnitude, logarithms of positive finite numbers, exponents within the normal range (smaller results
+// flush to zero, as FTZ does anyway). logf expands to 7 instructions and a division to 5; these take 2 and 2.
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float hw_log(float x) { return __builtin_amdgcn_logf(x) * 0.693147181f; }
+__device__ __forceinline__ float hw_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
 __device__ __forceinline__ double r_erfc(double x) { return erfc(x); }
 template <typename R>
 __device__ __forceinline__ R r_max(R a, R b) { return a > b ? a : b; }
@@ -70,7 +79,7 @@ __device__ __forceinline__ R unit01(std::uint32_t i) {
 
 __device__ __forceinline__ void gauss2(std::uint32_t a, std::uint32_t b, float& z0, float& z1) {
     const float u = unit01<float>(a), v = unit01<float>(b);
-    const float radius = r_sqrt(-2.0f * logf(u));
+    const float radius = r_sqrt(-2.0f * hw_log(u));
     float s, c;
     sincospif(2.0f * v, &s, &c);
     z0 = radius * c;

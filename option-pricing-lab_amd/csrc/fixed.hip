@@ -120,64 +120,28 @@ __device__ __forceinline__ void qe_step(const Consts<R>& c, R& y, R& v, R zv, R 
     v = vn;
 }
 
-// The FP32 step: the same scheme with the divisions shared between the quantities that need them. With
+// erfc by the Chebyshev fit of Numerical Recipes in C (2nd ed., section 6.2, "erfcc"): fractional error below
+// 1.2e-7 on the whole line, one reciprocal and one exponential.
+__device__ __forceinline__ float erfc_fast(float x) {
+    const float z = fabsf(x), t = hw_rcp(1.0f + 0.5f * z);
+    const float poly = -1.26551223f + t * (1.00002368f + t * (0.37409196f + t * (0.09678418f + t * (-0.18628806f +
+                       t * (0.27886807f + t * (-1.13520398f + t * (1.48851587f + t * (-0.82215223f + t * 0.17087277f))))))));
+    const float r = t * hw_exp(-z * z + poly);
+    return x >= 0.0f ? r : 2.0f - r;
+}
+
+// The FP32 step: the scheme of qe_step with the divisions shared between the quantities that need them. With
 // s2 = v c1 + c2 and m2 = m^2 (psi = s2 / m2): the branch tests s2 <= psi_c m2; the quadratic branch takes
 // 2 / psi = 2 m2 / s2 and sqrt(inv) sqrt(inv - 1) as one root; A < 1 / (2 a) is arg = 1 - 2 A a > 0; the
 // exponential branch takes p, 1 - p and beta from the one reciprocal of s2 + m2. Four divisions and three roots
 // where the literal transcription has seven and four; the FP64 kernels keep the reference's order of operations.
-#ifndef OPTLAB_NR_ERFC
-#define OPTLAB_NR_ERFC 1
-#endif
-#if OPTLAB_NR_ERFC
-// erfc by the Chebyshev fit of Numerical Recipes in C (2nd ed., section 6.2, "erfcc"): fractional error below
-// 1.2e-7 on the whole line, one reciprocal and one exponential.
-__device__ __forceinline__ float erfc_fast(float x) {
-    const float z = fabsf(x), t = 1.0f / (1.0f + 0.5f * z);
-    const float poly = -1.26551223f + t * (1.00002368f + t * (0.37409196f + t * (0.09678418f + t * (-0.18628806f +
-                       t * (0.27886807f + t * (-1.13520398f + t * (1.48851587f + t * (-0.82215223f + t * 0.17087277f))))))));
-    const float r = t * expf(-z * z + poly);
-    return x >= 0.0f ? r : 2.0f - r;
-}
-#else
-__device__ __forceinline__ float erfc_fast(float x) { return erfcf(x); }
-#endif
-
-template <>
-__device__ __forceinline__ void qe_step<float>(const Consts<float>& c, float& y, float& v, float zv, float zx) {
-    const float m = c.theta + (v - c.theta) * c.E;
-    const float s2 = v * c.c1 + c.c2, m2 = m * m;
-    float vn, lin, w, arg;
-    bool finite;
-    if (s2 <= c.psi_c * m2) {
-        const float inv = 2.0f * m2 / s2;
-        const float b2 = inv - 1.0f + sqrtf(inv * fmaxf(inv - 1.0f, 0.0f));
-        const float a = m / (1.0f + b2), b = sqrtf(b2);
-        vn = a * (b + zv) * (b + zv);
-        arg = 1.0f - 2.0f * c.A * a;
-        finite = arg > 0.0f;
-        lin = -c.A * b2 * a / arg;
-        w = 0.5f;
-    } else {
-        const float den = 1.0f / (s2 + m2);
-        const float p = (s2 - m2) * den, q = 2.0f * m2 * den, beta = 2.0f * m * den;  // p, 1 - p, (1 - p) / m
-        const float one_minus_u = 0.5f * erfc_fast(zv * 0.70710678f);
-        vn = one_minus_u >= q ? 0.0f : logf(q / one_minus_u) * (0.5f * (s2 + m2) / m);
-        arg = p + beta * q / (beta - c.A);
-        finite = c.A < beta;
-        lin = 0.0f;
-        w = -1.0f;
-    }
-    const float K0 = finite ? lin + w * logf(arg) - (c.K1 + 0.5f * c.K3) * v : c.K0p;
-    y += c.mu_dt + K0 + c.K1 * v + c.K2 * vn + sqrtf(fmaxf(c.K3 * v + c.K4 * vn, 0.0f)) * zx;
-    v = vn;
-}
-
-// Both legs of a pair, one FP32 QE-M step (legs 0 and 1 take normals (zv, zx) and (-zv, -zx)). On the strip's
+//
+// Both legs of a pair take one step together (legs 0 and 1 take normals (zv, zx) and (-zv, -zx)). On the strip's
 // Heston workload about one leg-step in six takes the exponential branch (psi > psi_c), but nearly every wave has
-// lanes on both sides, so the per-leg step ran both branches for each leg. Here the quadratic branch runs per leg,
+// lanes on both sides, so a per-leg step ran both branches for each leg. Here the quadratic branch runs per leg,
 // the exponential branch once, on the leg of each lane that needs it (leg 0 when both do), and a second time only
-// for lanes whose two legs both need it (about one wave in a hundred has such a lane). Each leg gets the arithmetic
-// of qe_step<float>; a pair-step issues one exponential branch less.
+// for lanes whose two legs both need it (about one wave in a hundred has such a lane): each leg gets the same
+// arithmetic, and a pair-step issues one exponential branch less.
 struct QeBranch {
     float vn, lin, w, arg;
     bool finite;
@@ -185,24 +149,24 @@ struct QeBranch {
 
 __device__ __forceinline__ QeBranch qe_quadratic(const Consts<float>& c, float m, float s2, float m2, float zv) {
     QeBranch o;
-    const float inv = 2.0f * m2 / s2;
-    const float b2 = inv - 1.0f + sqrtf(inv * fmaxf(inv - 1.0f, 0.0f));
-    const float a = m / (1.0f + b2), b = sqrtf(b2);
+    const float inv = 2.0f * m2 * hw_rcp(s2);
+    const float b2 = inv - 1.0f + r_sqrt(inv * fmaxf(inv - 1.0f, 0.0f));
+    const float a = m * hw_rcp(1.0f + b2), b = r_sqrt(b2);
     o.vn = a * (b + zv) * (b + zv);
     o.arg = 1.0f - 2.0f * c.A * a;
     o.finite = o.arg > 0.0f;
-    o.lin = -c.A * b2 * a / o.arg;
+    o.lin = -c.A * b2 * a * hw_rcp(o.arg);
     o.w = 0.5f;
     return o;
 }
 
 __device__ __forceinline__ QeBranch qe_exponential(const Consts<float>& c, float m, float s2, float m2, float zv) {
     QeBranch o;
-    const float den = 1.0f / (s2 + m2);
+    const float den = hw_rcp(s2 + m2);
     const float p = (s2 - m2) * den, q = 2.0f * m2 * den, beta = 2.0f * m * den;
     const float one_minus_u = 0.5f * erfc_fast(zv * 0.70710678f);
-    o.vn = one_minus_u >= q ? 0.0f : logf(q / one_minus_u) * (0.5f * (s2 + m2) / m);
-    o.arg = p + beta * q / (beta - c.A);
+    o.vn = one_minus_u >= q ? 0.0f : hw_log(q * hw_rcp(one_minus_u)) * (0.5f * (s2 + m2) * hw_rcp(m));
+    o.arg = p + beta * q * hw_rcp(beta - c.A);
     o.finite = c.A < beta;
     o.lin = 0.0f;
     o.w = -1.0f;
@@ -210,8 +174,8 @@ __device__ __forceinline__ QeBranch qe_exponential(const Consts<float>& c, float
 }
 
 __device__ __forceinline__ void qe_tail(const Consts<float>& c, float& y, float& v, const QeBranch& o, float zx) {
-    const float K0 = o.finite ? o.lin + o.w * logf(o.arg) - (c.K1 + 0.5f * c.K3) * v : c.K0p;
-    y += c.mu_dt + K0 + c.K1 * v + c.K2 * o.vn + sqrtf(fmaxf(c.K3 * v + c.K4 * o.vn, 0.0f)) * zx;
+    const float K0 = o.finite ? o.lin + o.w * hw_log(o.arg) - (c.K1 + 0.5f * c.K3) * v : c.K0p;
+    y += c.mu_dt + K0 + c.K1 * v + c.K2 * o.vn + r_sqrt(fmaxf(c.K3 * v + c.K4 * o.vn, 0.0f)) * zx;
     v = o.vn;
 }
 
