@@ -60,13 +60,19 @@ struct Shape {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// What a step reads of each name besides its table row, in one scalar load: (lo, 1 / dx of the row's grid, ln F at
+// the end of the step, 0), at name * n_steps + step.
+template <typename R>
+struct StepRec {
+    typedef R type __attribute__((ext_vector_type(4)));
+};
+
 template <typename R>
 struct BookArgs {
     int n_book, n_steps, n_x, n_scen, n_items, pairs_per_lane;
     R dt;
     const R* tables;  // ((scen * n_book + name) * n_steps + step) * n_x
-    const R* lo;      // name * n_steps + step
-    const R* inv;
+    const typename StepRec<R>::type* steps;  // name * n_steps + step
     const R* lf;      // name * (n_steps + 1) + step: ln F
     const R* lsh;     // scen * n_book + name: ln(1 + shock)
     const R* disc;    // n_steps + 1
@@ -77,12 +83,12 @@ struct BookArgs {
     unsigned long long* start;
 };
 
-// The rows of step `step` of every name of trade t under scenario s into LDS (n names x n_x), by the workgroup.
+// The rows of step `step` of the NM names of trade t under scenario s into LDS (NM x n_x), by the workgroup.
 template <typename R, int NM>
 __device__ __forceinline__ void stage_rows(const BookArgs<R>& a, const Shape& sh, int step, R* dst) {
 #pragma unroll
     for (int k = 0; k < NM; ++k) {  // unrolled: the names stay in registers (a runtime index would put them on
-        if (k >= sh.n) break;          // the stack)
+                                    // the stack)
         const R* src = a.tables + ((static_cast<long long>(sh.scen) * a.n_book + sh.names[k]) * a.n_steps + step) * a.n_x;
         for (int c = threadIdx.x; c < a.n_x; c += blockDim.x) dst[k * a.n_x + c] = src[c];
     }
@@ -95,21 +101,20 @@ template <typename R, int NM>
 __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const Shape& sh, long long p, R* rows) {
     NormalStream<R> rng;
     rng.init(static_cast<std::uint64_t>(p), ldc(&t.key0, 0), ldc(&t.key1, 0), 0u);
-    const int n = sh.n, scen = sh.scen;
+    constexpr int n = NM;  // the kernel of a trade's class is compiled for its name count
+    const int scen = sh.scen;
     const int stride = a.n_steps + 1;
     R x[2][NM];
     R lsh[NM];
-    const R* lo[NM];
-    const R* inv[NM];
-    const R* lf[NM];
+    int rec[NM];  // the name's first step record; its ln F row starts at name * (n_steps + 1)
+    int lfr[NM];
 #pragma unroll
     for (int k = 0; k < NM; ++k) {
         x[0][k] = x[1][k] = R(0);
         const int nm = k < n ? sh.names[k] : sh.names[0];
         lsh[k] = ldc(a.lsh, scen * a.n_book + nm);
-        lo[k] = a.lo + static_cast<long long>(nm) * a.n_steps;
-        inv[k] = a.inv + static_cast<long long>(nm) * a.n_steps;
-        lf[k] = a.lf + static_cast<long long>(nm) * stride;
+        rec[k] = nm * a.n_steps;
+        lfr[k] = nm * stride;
     }
     // product state, per leg
     R acc[2] = {R(0), R(0)};        // Asian: sum of fixings; barrier: running max of ln S
@@ -120,10 +125,10 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
     const int product = sh.product, n_steps = sh.n_steps, obs_every = sh.obs_every;
     if (product == 2) acc[0] = acc[1] = R(-1e30);
     if (product == 3) {
-        R y0 = ldc(lf[0], 0) + lsh[0] - ldc(t.lfix, 0);
+        R y0 = ldc(a.lf, lfr[0]) + lsh[0] - ldc(t.lfix, 0);
 #pragma unroll
         for (int k = 1; k < NM; ++k)
-            if (k < n) y0 = r_min(y0, ldc(lf[k], 0) + lsh[k] - ldc(t.lfix, k));
+            if (k < n) y0 = r_min(y0, ldc(a.lf, lfr[k]) + lsh[k] - ldc(t.lfix, k));
         ymin[0] = ymin[1] = y0;
     }
     int obs_j = 0;
@@ -139,7 +144,21 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
 #pragma unroll
         for (int k = 0; k < NM; ++k)
             if (k < n) zs[k] = rng.next();
+        R zc[NM];  // correlated normals of the + leg (the - leg takes their negatives)
+#pragma unroll
+        for (int k = 0; k < NM; ++k) {
+            if (k >= n) break;
+            R acc_z = R(0);
+#pragma unroll
+            for (int b = 0; b < NM; ++b)
+                if (b < n) acc_z += ldc(t.root, k * n + b) * zs[b];
+            zc[k] = acc_z;
+        }
         R var[NM];  // the table reads do not depend on the leg's sign until x differs: read per leg
+        typename StepRec<R>::type g[NM];
+#pragma unroll
+        for (int k = 0; k < NM; ++k)
+            if (k < n) g[k] = ldc(a.steps, rec[k] + step);
         const int t1 = step + 1;
         const bool fix = t1 % obs_every == 0;
 #pragma unroll
@@ -148,16 +167,12 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
 #pragma unroll
             for (int k = 0; k < NM; ++k) {
                 if (k >= n) break;
-                R acc_z = R(0);
-#pragma unroll
-                for (int b = 0; b < NM; ++b)
-                    if (b < n) acc_z += ldc(t.root, k * n + b) * zs[b];
-                const R z = sign * acc_z;
-                const Lerp<R> l = lerp_at(cur + k * a.n_x, ldc(lo[k], step), ldc(inv[k], step), a.n_x, x[leg][k]);
+                const R z = sign * zc[k];
+                const Lerp<R> l = lerp_at(cur + k * a.n_x, g[k].x, g[k].y, a.n_x, x[leg][k]);
                 var[k] = l.var;
                 x[leg][k] += R(-0.5) * var[k] * a.dt + r_sqrt(var[k] * a.dt) * z;
             }
-            const R y = ldc(lf[0], t1) + lsh[0] + x[leg][0];  // ln S of the first name
+            const R y = g[0].z + lsh[0] + x[leg][0];  // ln S of the first name
             if (product == 1) {
                 if (fix) acc[leg] += r_exp(y);
             } else if (product == 2) {
@@ -166,7 +181,7 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
                 R yw = y - ldc(t.lfix, 0);
 #pragma unroll
                 for (int k = 1; k < NM; ++k)
-                    if (k < n) yw = r_min(yw, ldc(lf[k], t1) + lsh[k] + x[leg][k] - ldc(t.lfix, k));
+                    if (k < n) yw = r_min(yw, g[k].z + lsh[k] + x[leg][k] - ldc(t.lfix, k));
                 ymin[leg] = r_min(ymin[leg], yw);
                 if (fix) observe(terms, obs_j == n_obs - 1, r_exp(yw), r_exp(ymin[leg]), ldc(a.disc, t1), note[leg]);
             }
@@ -178,7 +193,7 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
     double v = 0.0;
 #pragma unroll
     for (int leg = 0; leg < 2; ++leg) {
-        const R ST = r_exp(ldc(lf[0], n_steps) + lsh[0] + x[leg][0]);
+        const R ST = r_exp(ldc(a.lf, lfr[0] + n_steps) + lsh[0] + x[leg][0]);
         R pay;
         if (product == 0) {
             pay = D * r_max(ST - strike, R(0));
@@ -334,18 +349,39 @@ int resident_blocks(int block, size_t lds) {
     return std::max(per_cu, 1) * device_cu_count();
 }
 
-constexpr int kClasses = 3;
-constexpr int kClassNames[kClasses] = {1, 3, 5};  // the name bound of each class's kernel
+constexpr int kClasses = kNames;  // class c: the trades on c + 1 names
 
 template <typename R>
 int class_of(const TradeDev<R>& t) {
-    return t.n <= 1 ? 0 : t.n <= 3 ? 1 : 2;
+    return t.n - 1;
+}
+
+template <typename R>
+int class_resident(int c, int block, size_t lds) {
+    switch (c) {
+        case 0: return resident_blocks<R, 1>(block, lds);
+        case 1: return resident_blocks<R, 2>(block, lds);
+        case 2: return resident_blocks<R, 3>(block, lds);
+        case 3: return resident_blocks<R, 4>(block, lds);
+        default: return resident_blocks<R, 5>(block, lds);
+    }
+}
+
+template <typename R>
+void launch_class(int c, const BookArgs<R>& ac, int grid, int block, size_t lds, hipStream_t st) {
+    switch (c) {
+        case 0: hipLaunchKernelGGL((book_kernel<R, 1>), dim3(grid), dim3(block), lds, st, ac); break;
+        case 1: hipLaunchKernelGGL((book_kernel<R, 2>), dim3(grid), dim3(block), lds, st, ac); break;
+        case 2: hipLaunchKernelGGL((book_kernel<R, 3>), dim3(grid), dim3(block), lds, st, ac); break;
+        case 3: hipLaunchKernelGGL((book_kernel<R, 4>), dim3(grid), dim3(block), lds, st, ac); break;
+        default: hipLaunchKernelGGL((book_kernel<R, 5>), dim3(grid), dim3(block), lds, st, ac); break;
+    }
 }
 
 // One launch of the queue `items` with its clock stamp and its reduction (captured in a HIP graph and the graph
-// launched, when `graph`), synchronised. The items are split by the name count of their trade into up to three
-// persistent launches, each of the kernel whose registers fit that many names, each over its own longest-first
-// queue.
+// launched, when `graph`), synchronised. The items are split by the name count of their trade into up to five
+// persistent launches, each of the kernel compiled for exactly that many names (no per-name conditions, the
+// correlation root at fixed offsets), each over its own longest-first queue.
 template <typename R>
 float launch_queue(BookArgs<R> args, const std::vector<Item>& items, const std::vector<TradeDev<R>>& trades,
                    Item* d_items, int block, int n_trades, int n_scen, const TradeDev<R>* d_trades, double* d_sums,
@@ -353,7 +389,7 @@ float launch_queue(BookArgs<R> args, const std::vector<Item>& items, const std::
     hipStream_t st = engine_stream();
     std::vector<Item> ordered;
     ordered.reserve(items.size());
-    int count[kClasses] = {0, 0, 0}, first[kClasses] = {0, 0, 0};
+    int count[kClasses] = {}, first[kClasses] = {};
     for (int c = 0; c < kClasses; ++c) {
         first[c] = static_cast<int>(ordered.size());
         for (const Item& it : items)
@@ -364,11 +400,8 @@ float launch_queue(BookArgs<R> args, const std::vector<Item>& items, const std::
     int grid[kClasses];
     size_t lds[kClasses];
     for (int c = 0; c < kClasses; ++c) {
-        lds[c] = 2 * static_cast<size_t>(kClassNames[c]) * args.n_x * sizeof(R);
-        const int resident = c == 0   ? resident_blocks<R, 1>(block, lds[c])
-                             : c == 1 ? resident_blocks<R, 3>(block, lds[c])
-                                      : resident_blocks<R, 5>(block, lds[c]);
-        grid[c] = std::max(1, std::min(resident, count[c]));
+        lds[c] = 2 * static_cast<size_t>(c + 1) * args.n_x * sizeof(R);
+        grid[c] = std::max(1, std::min(class_resident<R>(c, block, lds[c]), count[c]));
     }
     const unsigned red = static_cast<unsigned>((static_cast<long long>(n_trades) * n_scen + 255) / 256);
     auto body = [&]() {
@@ -378,12 +411,7 @@ float launch_queue(BookArgs<R> args, const std::vector<Item>& items, const std::
             BookArgs<R> ac = args;
             ac.items = d_items + first[c];
             ac.n_items = count[c];
-            if (c == 0)
-                hipLaunchKernelGGL((book_kernel<R, 1>), dim3(grid[c]), dim3(block), lds[c], st, ac);
-            else if (c == 1)
-                hipLaunchKernelGGL((book_kernel<R, 3>), dim3(grid[c]), dim3(block), lds[c], st, ac);
-            else
-                hipLaunchKernelGGL((book_kernel<R, 5>), dim3(grid[c]), dim3(block), lds[c], st, ac);
+            launch_class<R>(c, ac, grid[c], block, lds[c], st);
         }
         hipLaunchKernelGGL(book_reduce_kernel<R>, dim3(red), dim3(256), 0, st, d_trades, n_trades, n_scen,
                            args.partial, d_sums);
@@ -434,13 +462,14 @@ void run_typed(optlab_book_request& rq) {
     for (int a = 0; a < nb; ++a) rows[a] = &surface_rows(rq.surface[a]);
     const int n_x = rows[0]->n_x;
     const size_t table = static_cast<size_t>(ns) * n_x;
-    const size_t small = static_cast<size_t>(nb) * ns * 2 + static_cast<size_t>(nb) * (ns + 1) +
+    // step records (4 values each, 64-byte aligned after the tables), then ln F, ln(1 + shock), discount factors
+    const size_t tables_n = (static_cast<size_t>(nsc) * nb * table + 63) & ~size_t(63);
+    const size_t small = static_cast<size_t>(nb) * ns * 4 + static_cast<size_t>(nb) * (ns + 1) +
                          static_cast<size_t>(nsc) * nb + (ns + 1);
-    R* mem = static_cast<R*>(scratch(kSlotBook, (static_cast<size_t>(nsc) * nb * table + small) * sizeof(R)));
+    R* mem = static_cast<R*>(scratch(kSlotBook, (tables_n + small) * sizeof(R)));
     R* tables = mem;
-    R* lo = tables + static_cast<size_t>(nsc) * nb * table;
-    R* inv = lo + static_cast<size_t>(nb) * ns;
-    R* lf = inv + static_cast<size_t>(nb) * ns;
+    R* steps = tables + tables_n;
+    R* lf = steps + static_cast<size_t>(nb) * ns * 4;
     R* lsh = lf + static_cast<size_t>(nb) * (ns + 1);
     R* disc = lsh + static_cast<size_t>(nsc) * nb;
     hipEvent_t t0, t1;
@@ -466,18 +495,25 @@ void run_typed(optlab_book_request& rq) {
         }
         std::vector<R> host(small);
         size_t k = 0;
-        for (double v : hlo) host[k++] = static_cast<R>(v);
-        for (double v : hinv) host[k++] = static_cast<R>(v);
+        for (int a = 0; a < nb; ++a)
+            for (int i = 0; i < ns; ++i) {
+                const size_t r = static_cast<size_t>(a) * ns + i;
+                host[k++] = static_cast<R>(hlo[r]);
+                host[k++] = static_cast<R>(hinv[r]);
+                host[k++] = static_cast<R>(std::log(rq.fwd[static_cast<size_t>(a) * (ns + 1) + i + 1]));
+                host[k++] = R(0);
+            }
         for (size_t i = 0; i < static_cast<size_t>(nb) * (ns + 1); ++i) host[k++] = static_cast<R>(std::log(rq.fwd[i]));
         for (size_t i = 0; i < static_cast<size_t>(nsc) * nb; ++i) host[k++] = static_cast<R>(std::log1p(rq.spot_shock[i]));
         for (int i = 0; i <= ns; ++i) host[k++] = static_cast<R>(rq.disc[i]);
-        hip_check(hipMemcpy(lo, host.data(), small * sizeof(R), hipMemcpyHostToDevice), "copy book");  // `host` ends here
+        hip_check(hipMemcpy(steps, host.data(), small * sizeof(R), hipMemcpyHostToDevice), "copy book");  // `host` ends here
     }
 
     BookArgs<R> args{};
     args.n_book = nb, args.n_steps = ns, args.n_x = n_x, args.n_scen = nsc;
     args.dt = static_cast<R>(rq.dt);
-    args.tables = tables, args.lo = lo, args.inv = inv, args.lf = lf, args.lsh = lsh, args.disc = disc;
+    args.tables = tables, args.lf = lf, args.lsh = lsh, args.disc = disc;
+    args.steps = reinterpret_cast<const typename StepRec<R>::type*>(steps);
 
     // workgroup size and pairs per lane: the request's, or the fastest of a few on a sample of the book
     long long total_pairs = 0;

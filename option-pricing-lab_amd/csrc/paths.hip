@@ -175,6 +175,31 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
                 if (a < n) zr[a] = s.has_slv ? aux.next() : rng.next();
         }
         if (mixer) fresh = rng.next();
+        // root . zs (and root . zr for Heston/SLV names with the spot normals' correlation) do not depend on the
+        // leg: the - leg takes their negatives
+        R zc[N], wc[N];
+        if (!mixer) {
+#pragma unroll
+            for (int a = 0; a < N; ++a) {
+                if (a >= n) break;
+                R acc = R(0);
+#pragma unroll
+                for (int b = 0; b < N; ++b)
+                    if (b < n) acc += ldc(s.root, a * n + b) * zs[b];
+                zc[a] = acc;
+            }
+        }
+        if (has_var && s.same) {
+#pragma unroll
+            for (int a = 0; a < N; ++a) {
+                if (a >= n) break;
+                R acc = R(0);
+#pragma unroll
+                for (int b = 0; b < N; ++b)
+                    if (b < n) acc += ldc(s.root, a * n + b) * zr[b];
+                wc[a] = acc;
+            }
+        }
 #pragma unroll
         for (int leg = 0; leg < 2; ++leg) {
             const R sign = leg ? R(-1) : R(1);
@@ -209,25 +234,15 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
                 }
             } else {
 #pragma unroll
-                for (int a = 0; a < N; ++a) {
-                    if (a >= n) break;
-                    R acc = R(0);
-#pragma unroll
-                    for (int b = 0; b < N; ++b)
-                        if (b < n) acc += ldc(s.root, a * n + b) * zs[b];
-                    z[a] = sign * acc;
-                }
+                for (int a = 0; a < N; ++a)
+                    if (a < n) z[a] = sign * zc[a];
             }
             if (has_var) {
 #pragma unroll
                 for (int a = 0; a < N; ++a) {
                     if (a >= n) break;
                     if (s.same) {
-                        R acc = R(0);
-#pragma unroll
-                        for (int b = 0; b < N; ++b)
-                            if (b < n) acc += ldc(s.root, a * n + b) * zr[b];
-                        w[a] = sign * acc;
+                        w[a] = sign * wc[a];
                     } else {
                         w[a] = sign * zr[a];
                     }
