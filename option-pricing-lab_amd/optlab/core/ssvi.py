@@ -58,8 +58,11 @@ class SSVI:
         a, b, _, _ = self._wings(th)
         return th, 0.5 * (a + b), (a - b) / (a + b)
 
+    # w and derivatives evaluate the maturity's pieces (theta, the wings and their slopes, PCHIP) on T as given and
+    # broadcast against k in the arithmetic: with T of shape (rows, 1) and k of (rows, n) that is one PCHIP
+    # evaluation a row instead of one an element.
     def w(self, k, T):
-        k, T = np.broadcast_arrays(np.asarray(k, dtype=float), np.asarray(T, dtype=float))
+        k, T = np.asarray(k, dtype=float), np.asarray(T, dtype=float)
         th = self.theta(T)
         a, b, _, _ = self._wings(th)
         psi, chi = 0.5 * (a + b), 0.5 * (a - b)
@@ -69,8 +72,8 @@ class SSVI:
         return np.sqrt(self.w(k, T) / np.asarray(T, dtype=float))
 
     def derivatives(self, k, T):
-        """w, w_k, w_kk, w_T."""
-        k, T = np.broadcast_arrays(np.asarray(k, dtype=float), np.asarray(T, dtype=float))
+        """w, w_k, w_kk, w_T (each of the broadcast shape of k and T)."""
+        k, T = np.asarray(k, dtype=float), np.asarray(T, dtype=float)
         th = self.theta(T)
         a, b, da, db = self._wings(th)
         psi, chi = 0.5 * (a + b), 0.5 * (a - b)
@@ -83,10 +86,28 @@ class SSVI:
         dw_dpsi = 0.5 * psi * k**2 / R
         dw_dchi = 0.5 * k * (1 + th / R)
         wT = (dw_dtheta + dw_dpsi * dpsi + dw_dchi * dchi) * self.dtheta(T)
-        return w, wk, wkk, wT
+        return np.broadcast_arrays(w, wk, wkk, wT)
 
     def local_var(self, k, T, floor=1e-4, cap=4.0):
         T = np.maximum(np.asarray(T, dtype=float), 1.0 / 365)
         w, wk, wkk, wT = self.derivatives(k, T)
         g = (1 - k * wk / (2 * w)) ** 2 - wk**2 / 4 * (1 / w + 0.25) + wkk / 2
         return np.clip(wT / np.maximum(g, 1e-6), floor, cap)
+
+    def row_grid(self, t0, n_x, sd_range=(-8.0, 6.0)):
+        """(T, lo, hi, k) of the engine's local-variance rows at step starts t0: T = max(t0, 1 day), the row spans
+        sd_range ATM standard deviations sqrt(w(0, T)), k[i] the n_x points of np.linspace(lo[i], hi[i], n_x) (the
+        same arithmetic, element by element, in one array: the same numbers)."""
+        T = np.maximum(np.asarray(t0, dtype=float), 1.0 / 365)
+        sd = np.sqrt(self.w(0.0, T))
+        lo, hi = sd_range[0] * sd, sd_range[1] * sd
+        k = np.arange(n_x, dtype=float)[None, :] * ((hi - lo) / (n_x - 1))[:, None] + lo[:, None]
+        k[:, -1] = hi
+        return T, lo, hi, k
+
+    def local_var_rows(self, t0, n_x, sd_range=(-8.0, 6.0)):
+        """(lo, inv_dx, rows) of the engine's Dupire local-variance rows (native/__init__.py:129-144) in one
+        vectorised pass over every row: the grid of the per-row loop exactly, the rows equal to it up to the last
+        bit of a few elements (on the 3-year daily grid: 3 rows of 1095, 1e-16)."""
+        T, lo, hi, k = self.row_grid(t0, n_x, sd_range)
+        return lo, (n_x - 1) / (hi - lo), self.local_var(k, T[:, None])
