@@ -265,7 +265,14 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
             NormalStream<R> rng;
             rng.init(static_cast<std::uint64_t>(s.first_pair + (leg >> 1)), s.key0, s.key1, 0u);
             rng.block = static_cast<std::uint32_t>(seg * (SEG * N / 4));
-            R xs[SEG][N], zz[SEG][N];
+            // What the walk back needs of each (step, name). Up to 12 (step, name) pairs a segment it is kept from
+            // the recompute: the lerp's grid coordinate (its entry and weight), its slope and dx'/dvar =
+            // dt (-1/2 + z / (2 sqrt(var dt))), which spares the walk back its table reads. Beyond that the three
+            // values would not fit the 128 VGPRs of a 1024-lane workgroup: the state x and the normal z are kept
+            // instead and the walk back reads the table again.
+            constexpr bool kKeep = SEG * N <= 12;
+            constexpr int KS = kKeep ? SEG : 1, XS = kKeep ? 1 : SEG;
+            R lu[KS][N], ls[KS][N], lw[KS][N], xs[XS][N], zz[XS][N];
 #pragma unroll
             for (int k = 0; k < SEG; ++k) {
                 if (k >= len) break;
@@ -276,12 +283,19 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
                 correlate<R, N>(s.root, sign, zs, z);
 #pragma unroll
                 for (int a = 0; a < N; ++a) {
-                    xs[k][a] = x[a];
-                    zz[k][a] = z[a];
                     const R* y = s.table[a] + static_cast<long long>(step) * s.n_x;
                     const Lerp<R> l = lerp_at(y, static_cast<R>(ldc(s.lo[a], step)), static_cast<R>(ldc(s.inv[a], step)),
                                               s.n_x, x[a]);
-                    x[a] += R(-0.5) * l.var * s.dt + r_sqrt(l.var * s.dt) * z[a];
+                    const R sd = r_sqrt(l.var * s.dt);
+                    if constexpr (kKeep) {
+                        lu[k][a] = l.u;
+                        ls[k][a] = l.slope;
+                        lw[k][a] = s.dt * (R(-0.5) + R(0.5) * z[a] / sd);
+                    } else {
+                        xs[k][a] = x[a];
+                        zz[k][a] = z[a];
+                    }
+                    x[a] += R(-0.5) * l.var * s.dt + sd * z[a];
                 }
             }
             const int lat = s.lat[leg];
@@ -289,7 +303,7 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
 #pragma unroll
             for (int k = SEG - 1; k >= 0; --k) {
                 if (k >= len) continue;
-                const int step = s0 + k, t1 = step + 1;
+                const int t1 = s0 + k + 1;
                 const int j = ldc(s.obs_of, t1);
                 if (j >= 0) {
                     const long long o = static_cast<long long>(j) * s.n_legs + leg;
@@ -306,18 +320,27 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
                 }
 #pragma unroll
                 for (int a = 0; a < N; ++a) {
-                    const R* y = s.table[a] + static_cast<long long>(step) * s.n_x;
-                    const Lerp<R> l = lerp_at(y, static_cast<R>(ldc(s.lo[a], step)), static_cast<R>(ldc(s.inv[a], step)),
-                                              s.n_x, xs[k][a]);
-                    const R sd = r_sqrt(l.var * s.dt);
-                    const R vbar = xb[a] * s.dt * (R(-0.5) + R(0.5) * zz[k][a] / sd);
+                    R u, slope, wgt;
+                    if constexpr (kKeep) {
+                        u = lu[k][a], slope = ls[k][a], wgt = lw[k][a];
+                    } else {
+                        const int step = s0 + k;
+                        const R* y = s.table[a] + static_cast<long long>(step) * s.n_x;
+                        const Lerp<R> l = lerp_at(y, static_cast<R>(ldc(s.lo[a], step)),
+                                                  static_cast<R>(ldc(s.inv[a], step)), s.n_x, xs[k][a]);
+                        u = l.u, slope = l.slope;
+                        wgt = s.dt * (R(-0.5) + R(0.5) * zz[k][a] / r_sqrt(l.var * s.dt));
+                    }
+                    const R vbar = xb[a] * wgt;
+                    const int i0 = static_cast<int>(u);
+                    const R f = u - static_cast<R>(i0);
                     if (active) {
-                        unsigned long long* r = acc + (k * N + a) * s.n_x + l.i0;
-                        const long long q0 = to_fixed(vbar * (R(1) - l.f)), q1 = to_fixed(vbar * l.f);
+                        unsigned long long* r = acc + (k * N + a) * s.n_x + i0;
+                        const long long q0 = to_fixed(vbar * (R(1) - f)), q1 = to_fixed(vbar * f);
                         if (q0) atomicAdd(r, static_cast<unsigned long long>(q0));
                         if (q1) atomicAdd(r + 1, static_cast<unsigned long long>(q1));
                     }
-                    xb[a] += vbar * l.slope;
+                    xb[a] += vbar * slope;
                 }
             }
             if (seg > 0 && active) {
@@ -630,8 +653,9 @@ void run_seg(optlab_autocall_request& rq, bool adjoint, int seg) {
 
 // Autotuned checkpoint interval per shape (precision, names, steps, legs, n_x): more steps a segment means fewer
 // checkpoints to write and read and fewer segment passes, but more state held in registers while a segment is walked
-// back and more LDS a workgroup. The first adjoint of a shape with checkpoint_every = 0 runs every candidate twice
-// (the same request: the outputs are overwritten by the call that follows) and keeps the fastest.
+// back and more LDS a workgroup. The first adjoint of a shape with checkpoint_every = 0 runs once untimed (buffers
+// sized, tables built and cached), then every candidate three times, and keeps the fastest by its best device time
+// (the same request: the outputs are overwritten by the call that follows).
 struct SegKey {
     int prec, n, n_steps, n_x;
     long long n_legs;
@@ -656,10 +680,14 @@ int pick_seg(optlab_autocall_request& rq) {
     hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_id), "attribute");
     int best = 4;
     double best_ms = 0.0;
+    {
+        optlab_autocall_request t = rq;
+        run_seg<R, N>(t, true, 4);
+    }
     for (const int seg : {4, 8, 16}) {
         if (!seg_compiled<N>(seg) || static_cast<size_t>(seg) * N * n_x * 8 > static_cast<size_t>(max_lds)) continue;
         double ms = 0.0;
-        for (int rep = 0; rep < 2; ++rep) {
+        for (int rep = 0; rep < 3; ++rep) {
             optlab_autocall_request t = rq;
             run_seg<R, N>(t, true, seg);
             const double m = t.forward_ms + t.backward_ms;

@@ -59,11 +59,12 @@ __device__ __forceinline__ void observe(const Terms<R>& t, bool last, R p, R low
     }
 }
 
-// The lerp of paths.hpp:196-205 with its slope in x (zero where u is clamped) and the weight of entry i0 + 1.
+// The lerp of paths.hpp:196-205 with its slope in x (zero where u is clamped) and the weight of entry i0 + 1; u is
+// the clamped grid coordinate (i0 = int(u), f = u - i0).
 template <typename R>
 struct Lerp {
     int i0;
-    R f, var, slope;
+    R f, var, slope, u;
 };
 
 template <typename R>
@@ -75,7 +76,7 @@ __device__ __forceinline__ Lerp<R> lerp_at(const R* y, R lo, R inv, int n_x, R x
     const R f = u - static_cast<R>(i);
     const R y0 = y[i], y1 = y[i + 1];
     const bool inside = u0 > R(0) && u0 < top;
-    return Lerp<R>{i, f, y0 * (R(1) - f) + y1 * f, inside ? (y1 - y0) * inv : R(0)};
+    return Lerp<R>{i, f, y0 * (R(1) - f) + y1 * f, inside ? (y1 - y0) * inv : R(0), u};
 }
 
 template <typename R, int N>
