@@ -215,8 +215,16 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
 // item it on workgroup it % grid. An item is blockDim x pairs_per_lane pairs of one (trade, scenario): every wave of
 // the workgroup reads the same staged rows. Every loop bound is the same in all lanes (kernel arguments and the item
 // index), so each barrier is reached by the whole workgroup. Each item's pair sums go to its own slot.
+// Waves a SIMD asked of the compiler for the kernel of NM names: 6 for FP32 (80 VGPRs, three 512-lane workgroups a
+// CU), which the 5-name kernel (88 VGPRs, two workgroups) then meets; 1 leaves the choice to the compiler.
 template <typename R, int NM>
-__global__ void __launch_bounds__(kMaxBlock) book_kernel(BookArgs<R> a) {
+constexpr int book_waves() {
+    return sizeof(R) == 4 ? 6 : 1;
+}
+
+template <typename R, int NM>
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(book_waves<R, NM>(), 8)))
+book_kernel(BookArgs<R> a) {
     extern __shared__ unsigned char lds_raw[];
     R* rows = reinterpret_cast<R*>(lds_raw);  // 2 x NM x n_x
     __shared__ double part[kMaxBlock / 64][2];
