@@ -13,7 +13,8 @@
 // registers), pairs_per_lane pairs in turn, the table rows of each step staged in LDS once for all the workgroup's
 // waves (double-buffered, one barrier a step) instead of gathered per lane from HBM. Each item's pair sums go to its
 // own slot and a reduction adds the slots of a (trade, scenario) in chunk order, so results do not depend on which
-// workgroup ran what. The clock stamp, the persistent kernel and the reduction are captured in one HIP graph.
+// workgroup ran what. Trades are split by name count, each class run by a persistent kernel compiled for exactly
+// that many names; the clock stamp, the class kernels and the reduction are captured in one HIP graph.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -210,11 +211,6 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
     return 0.5 * v;
 }
 
-// Persistent, one launch per class of trades by name count (NM: 1, 3 or 5, the bound the registers are sized
-// for): a grid of the resident workgroups walks its class's queue (ordered longest first) with a stride of the grid,
-// item it on workgroup it % grid. An item is blockDim x pairs_per_lane pairs of one (trade, scenario): every wave of
-// the workgroup reads the same staged rows. Every loop bound is the same in all lanes (kernel arguments and the item
-// index), so each barrier is reached by the whole workgroup. Each item's pair sums go to its own slot.
 // Waves a SIMD asked of the compiler for the kernel of NM names: 6 for FP32 (80 VGPRs, three 512-lane workgroups a
 // CU), which the 5-name kernel (88 VGPRs, two workgroups) then meets; 1 leaves the choice to the compiler.
 template <typename R, int NM>
@@ -222,6 +218,11 @@ constexpr int book_waves() {
     return sizeof(R) == 4 ? 6 : 1;
 }
 
+// Persistent, one launch per class of trades by name count (NM: the trades' exact name count): a grid of the
+// resident workgroups walks its class's queue (ordered longest first) with a stride of the grid, item it on
+// workgroup it % grid. An item is blockDim x pairs_per_lane pairs of one (trade, scenario): every wave of the
+// workgroup reads the same staged rows. Every loop bound is the same in all lanes (kernel arguments and the item
+// index), so each barrier is reached by the whole workgroup. Each item's pair sums go to its own slot.
 template <typename R, int NM>
 __global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(book_waves<R, NM>(), 8)))
 book_kernel(BookArgs<R> a) {
