@@ -38,6 +38,9 @@ struct Consts {  // schemes.hpp:159-207, computed in double on the host and cast
     int leg[3], n_steps;
     R inv_n;
     R v0, theta, mu_dt, E, c1, c2, K1, K2, K3, K4, A, K0p, psi_c;
+    // theta (1 - E), 2 A and K1 + K3 / 2 for the FP32 step: the scalar unit has no float arithmetic, so a float
+    // expression of kernel arguments formed in the step would cost a vector instruction every step
+    R th1E, twoA, k1h;
 };
 
 struct HostConsts {
@@ -52,6 +55,7 @@ struct HostConsts {
         c.n_steps = n_steps, c.inv_n = R(inv_n), c.v0 = R(v0), c.theta = R(theta), c.mu_dt = R(mu_dt), c.E = R(E);
         c.c1 = R(c1), c.c2 = R(c2), c.K1 = R(K1), c.K2 = R(K2), c.K3 = R(K3), c.K4 = R(K4), c.A = R(A);
         c.K0p = R(K0p), c.psi_c = R(psi_c);
+        c.th1E = R(theta * (1 - E)), c.twoA = R(2 * A), c.k1h = R(K1 + 0.5 * K3);
         return c;
     }
 };
@@ -153,7 +157,7 @@ __device__ __forceinline__ QeBranch qe_quadratic(const Consts<float>& c, float m
     const float b2 = inv - 1.0f + r_sqrt(inv * fmaxf(inv - 1.0f, 0.0f));
     const float a = m * hw_rcp(1.0f + b2), b = r_sqrt(b2);
     o.vn = a * (b + zv) * (b + zv);
-    o.arg = 1.0f - 2.0f * c.A * a;
+    o.arg = 1.0f - c.twoA * a;
     o.finite = o.arg > 0.0f;
     o.lin = -c.A * b2 * a * hw_rcp(o.arg);
     o.w = 0.5f;
@@ -174,7 +178,7 @@ __device__ __forceinline__ QeBranch qe_exponential(const Consts<float>& c, float
 }
 
 __device__ __forceinline__ void qe_tail(const Consts<float>& c, float& y, float& v, const QeBranch& o, float zx) {
-    const float K0 = o.finite ? o.lin + o.w * hw_log(o.arg) - (c.K1 + 0.5f * c.K3) * v : c.K0p;
+    const float K0 = o.finite ? o.lin + o.w * hw_log(o.arg) - c.k1h * v : c.K0p;
     y += c.mu_dt + K0 + c.K1 * v + c.K2 * o.vn + r_sqrt(fmaxf(c.K3 * v + c.K4 * o.vn, 0.0f)) * zx;
     v = o.vn;
 }
@@ -184,7 +188,7 @@ __device__ __forceinline__ void qe_pair(const Consts<float>& c, float (&y)[2], f
     bool ex[2];
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
-        m[a] = c.theta + (v[a] - c.theta) * c.E;
+        m[a] = v[a] * c.E + c.th1E;  // theta + (v - theta) E
         s2[a] = v[a] * c.c1 + c.c2;
         m2[a] = m[a] * m[a];
         ex[a] = !(s2[a] <= c.psi_c * m2[a]);
