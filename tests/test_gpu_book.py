@@ -45,6 +45,26 @@ SCEN = Scenarios(np.array([[0.0, 0.0, 0.0], [0.03, -0.02, 0.01], [-0.05, 0.0, 0.
                  np.array([0.0, 0.015, -0.02]), np.array([0.0, -0.005, 0.01]))
 
 
+def wide_book(n_steps=126, n_paths=512):
+    """Five names, and autocalls on 2, 4 and 5 of them: the kernels of those name counts (the small book has trades
+    on 1 and 3 names only)."""
+    params = ((0.03, -0.5), (0.04, -0.6), (0.05, -0.7), (0.035, -0.55), (0.045, -0.65))
+    surfaces = [SSVI.power_law(r, 1.2, 0.5, PILLARS, s * PILLARS) for s, r in params]
+    markets = [Market.flat(100.0, 0.03, q) for q in (0.0, 0.01, 0.02, 0.005, 0.015)]
+    corr = np.full((5, 5), 0.5) + 0.5 * np.eye(5)
+    trades = [
+        Trade("autocall", (3, 1), 126, 42, 1.0, 0.6, 0.02, 0.7, 1.0, True, n_paths=n_paths, seed=21),
+        Trade("autocall", (0, 2, 4, 1), 126, 63, 1.0, 0.65, 0.03, 0.75, 1.0, False, n_paths=n_paths, seed=22),
+        Trade("autocall", (4, 3, 2, 1, 0), 84, 21, 1.0, 0.6, 0.025, 0.7, 1.02, True, n_paths=n_paths, seed=23),
+        Trade("european", (4,), 105, 1, 1.05, n_paths=n_paths, seed=24),
+    ]
+    return Book(surfaces, markets, corr, trades, n_steps=n_steps)
+
+
+WIDE_SCEN = Scenarios(np.array([[0.0] * 5, [0.02, -0.01, 0.03, -0.02, 0.01]]), np.array([0.0, -0.01]),
+                      np.array([0.0, 0.005]))
+
+
 def test_fp64_book_is_the_oracle_on_the_same_paths():
     _fp64_book_against_the_oracle("batched", 128, 2)
 
@@ -53,10 +73,14 @@ def test_fp64_book_in_the_per_trade_loop_is_the_oracle_on_the_same_paths():
     _fp64_book_against_the_oracle("loop", 256, 1)
 
 
-def _fp64_book_against_the_oracle(mode, block, ppl):
-    book = small_book()
+def test_fp64_book_on_2_4_and_5_names_is_the_oracle_on_the_same_paths():
+    _fp64_book_against_the_oracle("batched", 256, 1, wide_book(), WIDE_SCEN)
+
+
+def _fp64_book_against_the_oracle(mode, block, ppl, book=None, scen=SCEN):
+    book = book or small_book()
     print(f"book fp64 {mode} block {block} ppl {ppl}: engine call", flush=True)
-    rev = revalue(book, SCEN, backend="fp64", mode=mode, block=block, pairs_per_lane=ppl)
+    rev = revalue(book, scen, backend="fp64", mode=mode, block=block, pairs_per_lane=ppl)
     print("engine call returned; oracle", flush=True)
     t = np.arange(book.n_steps + 1) * book.dt
     rows = [so.base_rows(s, t[:-1]) for s in book.surfaces]
@@ -67,8 +91,8 @@ def _fp64_book_against_the_oracle(mode, block, ppl):
         td = dict(product=tr.product, names=tr.names, root=root, n_steps=tr.n_steps, obs_every=tr.obs_every,
                   strike=tr.strike, barrier=tr.barrier, coupon=tr.coupon, coupon_barrier=tr.coupon_barrier,
                   autocall=tr.autocall, memory=tr.memory, n_pairs=tr.n_paths // 2, seed=tr.seed)
-        for s in range(len(SCEN.vol_shift)):
-            sc = {"spot": SCEN.spot_shock[s], "vol": SCEN.vol_shift[s], "skew": SCEN.skew[s]}
+        for s in range(len(scen.vol_shift)):
+            sc = {"spot": scen.spot_shock[s], "vol": scen.vol_shift[s], "skew": scen.skew[s]}
             want, want_se = bo.trade_value(oracle(), _abi.PathsRequest, td, ob, sc)
             assert rev.value[i, s] == pytest.approx(want, rel=1e-12, abs=1e-13), (tr.product, s)
             assert rev.se[i, s] == pytest.approx(want_se, rel=1e-8, abs=1e-13), (tr.product, s)
