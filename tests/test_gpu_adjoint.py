@@ -201,6 +201,22 @@ def test_checkpoint_intervals_give_the_same_greeks_and_the_autotuner_picks_one()
         nt._call(nt._lib.optlab_adjoint, rq)
 
 
+def test_one_name_note_at_every_checkpoint_interval():
+    """One name: intervals 4, 8 and 16 give the same price and delta and the same vegas up to the fixed point."""
+    from optlab.native.adjoint import WorstOfAutocall
+
+    nt = WorstOfAutocall(surfaces()[:1], [Market.flat(100.0, 0.03, 0.0)], corr(1))
+    g = {k: nt.adjoint(1 << 13, 5, backend="fp64", checkpoint_every=k) for k in (4, 8, 16)}
+    for k in (8, 16):
+        assert g[k].seconds["checkpoint_every"] == k
+        assert g[k].price == pytest.approx(g[4].price, rel=1e-12)
+        np.testing.assert_allclose(g[k].delta, g[4].delta, rtol=1e-12)
+        np.testing.assert_allclose(g[k].vega, g[4].vega, rtol=1e-9, atol=1e-12)
+    auto = nt.adjoint(1 << 13, 5, backend="fp32")
+    assert auto.seconds["checkpoint_every"] in (4, 8, 16)
+    nt.close()
+
+
 def test_autograd_function_carries_the_adjoint():
     import torch
 
