@@ -277,6 +277,31 @@ typedef struct {
     int block_used, pairs_per_lane_used;
 } optlab_book_request;
 
+/* ---- addition: an American option by Longstaff-Schwartz on the engine's paths ------------------------------- */
+/* packages/core/src/optlab/core/products.py:105-169 (american_lsm): one name; at each exercise date, walking back
+   from the last, the discounted continuation value of the in-the-money paths is regressed on the basis
+   (1, s, s^2, s^3) in s = S / K, plus (v, v^2, s v) of the variance factor when basis_var; a path exercises where
+   its exercise value beats the fitted continuation. The rule is estimated on the pairs of `paths` (pass 1) and
+   applied to the next n_pairs pairs of the same seed (pass 2: first_pair + n_pairs on), so the price is a lower
+   bound up to Monte Carlo error. The reference solves each regression with NumPy's lstsq; here the normal
+   equations are formed and solved on the device in FP64 (sums in a fixed order, a Cholesky factorisation). A date
+   with min_itm or fewer paths in the money, or whose normal matrix is not positive definite, exercises nothing.
+   The price is the pair-average mean of K cash D(t_tau) (products.py:30-35), or the immediate exercise value when
+   that is higher (then se = 0). */
+typedef struct {
+    optlab_paths_request paths; /* one name, kept columns = the exercise dates (n_keep = n_obs = n_exercise,
+                                   no windows); out unused */
+    double strike;
+    int call;                   /* 1 call, 0 put */
+    const double* disc;         /* host, n_obs discount factors D(t_j) of the exercise dates */
+    double spot;                /* today's spot (the immediate exercise value) */
+    int basis_var;              /* 1: add v, v^2, s v (a Heston or SLV name) */
+    int min_itm;                /* regress only with more than this many paths in the money (50) */
+    double price, se;
+    double* coefs;              /* host or NULL: n_obs x 7 regression coefficients, NaN at dates that exercise nothing */
+    double paths_ms, regression_ms, apply_ms, total_ms;
+} optlab_lsm_request;
+
 /* ---- entry points ------------------------------------------------------------------------------------------- */
 
 /* requests.hpp:97-101 */
@@ -330,6 +355,10 @@ OPTLAB_API int optlab_adjoint(optlab_autocall_request* rq, int backend);
 
 /* Addition: the book above (backend 0 double paths, 1 float paths; sums in double either way). */
 OPTLAB_API int optlab_book(optlab_book_request* rq, int backend);
+
+/* Addition: the American option above (backend 0 double paths, 1 float paths; the regressions in double either way).
+   Replaces core/products.py:112-169 (NumPy paths, lstsq per date). */
+OPTLAB_API int optlab_lsm(optlab_lsm_request* rq, int backend);
 
 /* Addition, for tests of the integer stream: the Philox4x32-10 words of counters (c0 + i, c1, c2, c3), i < n,
    key (k0, k1), computed by the device function the kernels use; out_dev holds 4 n uint32 (device memory). */

@@ -335,6 +335,16 @@ int optlab_book(optlab_book_request* rq, int backend) {
     });
 }
 
+int optlab_lsm(optlab_lsm_request* rq, int backend) {
+    return guarded([&] {
+        if (!rq) invalid("null request");
+        const int prec = precision_of(backend);
+        need_device();
+        check_lsm_request(*rq);
+        run_lsm(*rq, prec);
+    });
+}
+
 int optlab_philox_dev(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1, long long n,
                       unsigned* out_dev, void* stream) {
     return guarded([&] {

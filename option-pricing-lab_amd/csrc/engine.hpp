@@ -38,6 +38,7 @@ enum Slot {
     kSlotBatches,      // adjoint: per-batch table adjoints
     kSlotBook,         // book: the tables of every (scenario, name) and the book's small arrays
     kSlotBookQueue,    // book: the work queue, trades, per-item sums and clocks
+    kSlotLsm,          // Longstaff-Schwartz: spots and variances at the dates, values, partial sums, coefficients
     kSlotCount
 };
 
@@ -49,8 +50,10 @@ hipStream_t engine_stream();
 
 // ---- path engine (paths.hip) ----------------------------------------------------------------------------------
 // Launch the path kernel for every pair of `rq` in precision prec (0 double, 1 float), output to d_out (device,
-// n_names x n_cols x 2 n_pairs doubles) on `stream`. Returns the kernel time in ms (HIP events).
-double launch_paths(const optlab_paths_request& rq, int prec, double* d_out, hipStream_t stream);
+// n_names x n_cols x 2 n_pairs doubles) on `stream`; d_var, when not null, receives the variance factor of name 0 at
+// the kept columns (n_keep x 2 n_pairs doubles). Returns the kernel time in ms (HIP events).
+double launch_paths(const optlab_paths_request& rq, int prec, double* d_out, hipStream_t stream,
+                    double* d_var = nullptr);
 void check_paths_request(const optlab_paths_request& rq);
 
 // ---- fixed products and strips (fixed.hip) ---------------------------------------------------------------------
@@ -72,6 +75,10 @@ void check_autocall_request(const optlab_autocall_request& rq, bool adjoint);
 void run_autocall(optlab_autocall_request& rq, int prec, bool adjoint);
 void check_book_request(const optlab_book_request& rq);
 void run_book(optlab_book_request& rq, int prec);
+
+// ---- American options by Longstaff-Schwartz (lsm.hip) ----------------------------------------------------------
+void check_lsm_request(const optlab_lsm_request& rq);
+void run_lsm(optlab_lsm_request& rq, int prec);
 
 // ---- test entry points (paths.hip) -----------------------------------------------------------------------------
 void launch_philox(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1, long long n,

@@ -1,0 +1,370 @@
+// American options by Longstaff-Schwartz on the engine's paths.
+// Algorithm: packages/core/src/optlab/core/products.py:105-169 (american_lsm, _basis) and :30-35 (_result): the
+// exercise rule is estimated on one set of paths by a backward sweep of least-squares regressions of the discounted
+// continuation value of the in-the-money paths on (1, s, s^2, s^3) [+ (v, v^2, s v)], s = S / K, and applied to an
+// independent set.
+//
+// MI355X design: the path engine writes the spots (and the variance factor) at the exercise dates into HBM,
+// structure of arrays (a date's paths contiguous). Each date of the backward sweep is three launches on one stream,
+// with no host round trip: a grid-stride pass forms the normal equations of the in-the-money paths (X'X, X'y and
+// the count: 15 or 36 FP64 sums) as per-workgroup rows, one workgroup adds the rows in a fixed order and solves the
+// system by an FP64 Cholesky factorisation, and a second pass moves the exercised paths' values. The rule is then
+// applied to the second set of paths in one pass that reduces pair averages. Every sum has a fixed order, so a
+// result does not depend on anything but the request.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device.hpp"
+#include "engine.hpp"
+
+namespace optlab_mi {
+
+namespace {
+
+constexpr int kLsmBlock = 256;
+constexpr int kMaxBasis = 7;
+constexpr int kCoefStride = 7;  // coefficients of a date in the request's coefs array
+
+template <int NB>
+constexpr int n_sums() {
+    return NB * (NB + 1) / 2 + NB + 1;  // X'X (lower triangle), X'y, the count
+}
+
+struct LsmArgs {
+    const double* S;  // n_obs x rows: the spots at the dates
+    const double* V;  // n_obs x rows: the variance factor (basis with v), or null
+    double* value;    // rows: the discounted value of each path (pass 1)
+    long long rows;   // 2 n_pairs
+    int n_obs, min_itm;
+    double inv_strike, phi;  // phi: +1 call, -1 put
+    const double* disc;      // n_obs
+    double* coefs;           // n_obs x kCoefStride
+    int* valid;              // n_obs
+    double* partial;         // workgroups x n_sums
+};
+
+// products.py:105-109, with s^3 formed as s s s.
+template <int NB>
+__device__ __forceinline__ void basis(double s, double v, double (&X)[NB]) {
+    X[0] = 1.0;
+    X[1] = s;
+    X[2] = s * s;
+    X[3] = s * s * s;
+    if constexpr (NB == 7) {
+        X[4] = v;
+        X[5] = v * v;
+        X[6] = s * v;
+    }
+}
+
+// Sum over a workgroup of K values per lane; thread k < K of the workgroup returns total k (others: 0).
+template <int K>
+__device__ __forceinline__ double workgroup_sums(double (&acc)[K], double* lds /* [waves][K] */) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double v = acc[k];
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) v += shfl_xor_d(v, m);
+        acc[k] = v;
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) lds[wave * K + k] = acc[k];
+    }
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x < K)
+        for (int w = 0; w < n_waves; ++w) t += lds[w * K + threadIdx.x];
+    return t;
+}
+
+// The normal equations of date j over the in-the-money paths: one row of n_sums per workgroup.
+template <int NB>
+__global__ void __launch_bounds__(kLsmBlock) lsm_sums_kernel(LsmArgs a, int j) {
+    constexpr int K = n_sums<NB>();
+    __shared__ double lds[(kLsmBlock / 64) * K];
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = 0.0;
+    const double* Sj = a.S + static_cast<long long>(j) * a.rows;
+    const double* Vj = a.V ? a.V + static_cast<long long>(j) * a.rows : nullptr;
+    const double dj = ldc(a.disc, j);
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.rows; i += stride) {
+        const double s = Sj[i] * a.inv_strike;
+        const double ex = fmax(a.phi * (s - 1.0), 0.0);
+        if (ex > 0.0) {
+            double X[NB];
+            basis<NB>(s, Vj ? Vj[i] : 0.0, X);
+            const double y = a.value[i] / dj;
+            int m = 0;
+#pragma unroll
+            for (int r = 0; r < NB; ++r)
+#pragma unroll
+                for (int c = 0; c <= r; ++c) acc[m++] += X[r] * X[c];
+#pragma unroll
+            for (int r = 0; r < NB; ++r) acc[m++] += X[r] * y;
+            acc[m] += 1.0;
+        }
+    }
+    const double t = workgroup_sums<K>(acc, lds);
+    if (threadIdx.x < K) a.partial[static_cast<long long>(blockIdx.x) * K + threadIdx.x] = t;
+}
+
+// The rows added in order, then (one lane) the FP64 Cholesky solve of X'X beta = X'y.
+template <int NB>
+__global__ void __launch_bounds__(64) lsm_solve_kernel(LsmArgs a, int j, int n_rows) {
+    constexpr int K = n_sums<NB>();
+    __shared__ double tot[K];
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        double t = 0.0;
+        for (int r = 0; r < n_rows; ++r) t += a.partial[static_cast<long long>(r) * K + k];
+        tot[k] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double L[NB][NB], b[NB];
+    int m = 0;
+    for (int r = 0; r < NB; ++r)
+        for (int c = 0; c <= r; ++c) L[r][c] = tot[m++];
+    for (int r = 0; r < NB; ++r) b[r] = tot[m++];
+    bool ok = tot[m] > static_cast<double>(a.min_itm);
+    for (int c = 0; c < NB && ok; ++c) {  // A = L L' in place (lower triangle)
+        double d = L[c][c];
+        for (int k = 0; k < c; ++k) d -= L[c][k] * L[c][k];
+        if (!(d > 0.0)) {
+            ok = false;
+            break;
+        }
+        L[c][c] = sqrt(d);
+        for (int r = c + 1; r < NB; ++r) {
+            double e = L[r][c];
+            for (int k = 0; k < c; ++k) e -= L[r][k] * L[c][k];
+            L[r][c] = e / L[c][c];
+        }
+    }
+    double beta[NB];
+    if (ok) {
+        double y[NB];
+        for (int r = 0; r < NB; ++r) {
+            double e = b[r];
+            for (int k = 0; k < r; ++k) e -= L[r][k] * y[k];
+            y[r] = e / L[r][r];
+        }
+        for (int r = NB - 1; r >= 0; --r) {
+            double e = y[r];
+            for (int k = r + 1; k < NB; ++k) e -= L[k][r] * beta[k];
+            beta[r] = e / L[r][r];
+        }
+    }
+    for (int k = 0; k < kCoefStride; ++k) a.coefs[j * kCoefStride + k] = ok && k < NB ? beta[k] : (ok ? 0.0 : NAN);
+    a.valid[j] = ok ? 1 : 0;
+}
+
+// products.py:155-158: the in-the-money paths whose exercise value beats the fitted continuation exercise at j.
+template <int NB>
+__global__ void __launch_bounds__(kLsmBlock) lsm_update_kernel(LsmArgs a, int j) {
+    if (!ldc(a.valid, j)) return;
+    double beta[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) beta[k] = ldc(a.coefs, j * kCoefStride + k);
+    const double* Sj = a.S + static_cast<long long>(j) * a.rows;
+    const double* Vj = a.V ? a.V + static_cast<long long>(j) * a.rows : nullptr;
+    const double dj = ldc(a.disc, j);
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.rows; i += stride) {
+        const double s = Sj[i] * a.inv_strike;
+        const double ex = fmax(a.phi * (s - 1.0), 0.0);
+        if (ex > 0.0) {
+            double X[NB];
+            basis<NB>(s, Vj ? Vj[i] : 0.0, X);
+            double cont = 0.0;
+#pragma unroll
+            for (int k = 0; k < NB; ++k) cont += X[k] * beta[k];
+            if (ex > cont) a.value[i] = ex * dj;
+        }
+    }
+}
+
+// products.py:137-150 on the second set of paths: each leg exercises at the first date whose rule says so (the
+// reference's backward loop keeps the earliest), else at maturity; the pair average of K cash D(t_tau) and its
+// square, summed per workgroup.
+template <int NB>
+__global__ void __launch_bounds__(kLsmBlock) lsm_apply_kernel(LsmArgs a, double strike, double* sums) {
+    __shared__ double lds[(kLsmBlock / 64) * 2];
+    const long long n_pairs = a.rows / 2;
+    double acc[2] = {0.0, 0.0};
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; p < n_pairs; p += stride) {
+        double pay = 0.0;
+        for (int leg = 0; leg < 2; ++leg) {
+            const long long i = leg ? n_pairs + p : p;
+            const int last = a.n_obs - 1;
+            double cash = fmax(a.phi * (a.S[static_cast<long long>(last) * a.rows + i] * a.inv_strike - 1.0), 0.0);
+            int tau = last;
+            for (int j = 0; j < last; ++j) {
+                if (!ldc(a.valid, j)) continue;
+                const double s = a.S[static_cast<long long>(j) * a.rows + i] * a.inv_strike;
+                const double ex = fmax(a.phi * (s - 1.0), 0.0);
+                if (!(ex > 0.0)) continue;
+                double X[NB];
+                basis<NB>(s, a.V ? a.V[static_cast<long long>(j) * a.rows + i] : 0.0, X);
+                double cont = 0.0;
+#pragma unroll
+                for (int k = 0; k < NB; ++k) cont += X[k] * ldc(a.coefs, j * kCoefStride + k);
+                if (ex > cont) {
+                    cash = ex;
+                    tau = j;
+                    break;
+                }
+            }
+            pay += strike * cash * ldc(a.disc, tau);
+        }
+        const double d = 0.5 * pay;
+        acc[0] += d;
+        acc[1] += d * d;
+    }
+    const double t = workgroup_sums<2>(acc, lds);
+    if (threadIdx.x < 2) sums[blockIdx.x * 2 + threadIdx.x] = t;
+}
+
+// The value of every path of pass 1 at the last date, discounted (products.py:147).
+__global__ void lsm_init_kernel(LsmArgs a) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= a.rows) return;
+    const int last = a.n_obs - 1;
+    const double s = a.S[static_cast<long long>(last) * a.rows + i] * a.inv_strike;
+    a.value[i] = fmax(a.phi * (s - 1.0), 0.0) * ldc(a.disc, last);
+}
+
+struct Events {
+    hipEvent_t e[4];
+    Events() {
+        for (auto& x : e) hip_check(hipEventCreate(&x), "event");
+    }
+    ~Events() {
+        for (auto& x : e) (void)hipEventDestroy(x);
+    }
+    double ms(int a, int b) const {
+        float t = 0.f;
+        hip_check(hipEventElapsedTime(&t, e[a], e[b]), "elapsed");
+        return t;
+    }
+};
+
+template <int NB>
+void sweep(optlab_lsm_request& rq, int prec) {
+    const auto wall0 = std::chrono::steady_clock::now();
+    hipStream_t st = engine_stream();
+    const int n_obs = rq.paths.n_obs;
+    const long long n_pairs = rq.paths.n_pairs, rows = 2 * n_pairs;
+    constexpr int K = n_sums<NB>();
+    const int blocks = static_cast<int>(std::max<long long>(
+        1, std::min<long long>((rows + kLsmBlock - 1) / kLsmBlock, 4LL * device_cu_count())));
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t s_bytes = up(static_cast<size_t>(n_obs) * rows * sizeof(double));
+    const size_t v_bytes = NB == 7 ? s_bytes : 0;
+    const size_t val_bytes = up(static_cast<size_t>(rows) * sizeof(double));
+    const size_t part_bytes = up(static_cast<size_t>(blocks) * K * sizeof(double));
+    const size_t coef_bytes = up(static_cast<size_t>(n_obs) * kCoefStride * sizeof(double));
+    const size_t valid_bytes = up(static_cast<size_t>(n_obs) * sizeof(int));
+    const size_t disc_bytes = up(static_cast<size_t>(n_obs) * sizeof(double));
+    const size_t sums_bytes = up(static_cast<size_t>(blocks) * 2 * sizeof(double));
+    unsigned char* m = static_cast<unsigned char*>(scratch(
+        kSlotLsm, s_bytes + v_bytes + val_bytes + part_bytes + coef_bytes + valid_bytes + disc_bytes + sums_bytes));
+    LsmArgs a{};
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        unsigned char* p = m + off;
+        off += bytes;
+        return p;
+    };
+    double* S = reinterpret_cast<double*>(carve(s_bytes));
+    double* V = NB == 7 ? reinterpret_cast<double*>(carve(v_bytes)) : nullptr;
+    a.S = S, a.V = V;
+    a.value = reinterpret_cast<double*>(carve(val_bytes));
+    a.partial = reinterpret_cast<double*>(carve(part_bytes));
+    a.coefs = reinterpret_cast<double*>(carve(coef_bytes));
+    a.valid = reinterpret_cast<int*>(carve(valid_bytes));
+    double* disc = reinterpret_cast<double*>(carve(disc_bytes));
+    double* sums = reinterpret_cast<double*>(carve(sums_bytes));
+    a.disc = disc;
+    a.rows = rows, a.n_obs = n_obs, a.min_itm = rq.min_itm;
+    a.inv_strike = 1.0 / rq.strike, a.phi = rq.call ? 1.0 : -1.0;
+    hip_check(hipMemcpyAsync(disc, rq.disc, n_obs * sizeof(double), hipMemcpyHostToDevice, st), "copy disc");
+
+    Events ev;
+    // pass 1: paths, then the backward sweep of regressions
+    hip_check(hipEventRecord(ev.e[0], st), "event");
+    double paths_ms = launch_paths(rq.paths, prec, S, st, V);
+    hip_check(hipEventRecord(ev.e[1], st), "event");
+    hipLaunchKernelGGL(lsm_init_kernel, dim3(static_cast<unsigned>((rows + 255) / 256)), dim3(256), 0, st, a);
+    hip_check(hipGetLastError(), "lsm init launch");
+    hip_check(hipMemsetAsync(a.valid, 0, n_obs * sizeof(int), st), "clear valid");
+    for (int j = n_obs - 2; j >= 0; --j) {
+        hipLaunchKernelGGL((lsm_sums_kernel<NB>), dim3(blocks), dim3(kLsmBlock), 0, st, a, j);
+        hipLaunchKernelGGL((lsm_solve_kernel<NB>), dim3(1), dim3(64), 0, st, a, j, blocks);
+        hipLaunchKernelGGL((lsm_update_kernel<NB>), dim3(blocks), dim3(kLsmBlock), 0, st, a, j);
+    }
+    hip_check(hipGetLastError(), "lsm sweep launch");
+    hip_check(hipEventRecord(ev.e[2], st), "event");
+    // pass 2: the next n_pairs pairs of the same seed, the rule applied
+    optlab_paths_request p2 = rq.paths;
+    p2.first_pair = rq.paths.first_pair + n_pairs;
+    paths_ms += launch_paths(p2, prec, S, st, V);
+    hipLaunchKernelGGL((lsm_apply_kernel<NB>), dim3(blocks), dim3(kLsmBlock), 0, st, a, rq.strike, sums);
+    hip_check(hipGetLastError(), "lsm apply launch");
+    hip_check(hipEventRecord(ev.e[3], st), "event");
+    std::vector<double> h(static_cast<size_t>(blocks) * 2);
+    hip_check(hipMemcpyAsync(h.data(), sums, h.size() * sizeof(double), hipMemcpyDeviceToHost, st), "copy sums");
+    std::vector<double> coefs(static_cast<size_t>(n_obs) * kCoefStride);
+    hip_check(hipMemcpyAsync(coefs.data(), a.coefs, coefs.size() * sizeof(double), hipMemcpyDeviceToHost, st),
+              "copy coefficients");
+    hip_check(hipStreamSynchronize(st), "synchronize");
+
+    double s1 = 0.0, s2 = 0.0;
+    for (int b = 0; b < blocks; ++b) s1 += h[2 * b], s2 += h[2 * b + 1];
+    const double n = static_cast<double>(n_pairs);
+    rq.price = s1 / n;
+    rq.se = n > 1 ? std::sqrt(std::max((s2 - s1 * s1 / n) / (n - 1.0), 0.0) / n) : 0.0;
+    const double immediate = rq.strike * std::max((rq.call ? 1.0 : -1.0) * (rq.spot / rq.strike - 1.0), 0.0);
+    if (immediate > rq.price) rq.price = immediate, rq.se = 0.0;  // products.py:165-168
+    for (int k = 0; k < kCoefStride; ++k) coefs[static_cast<size_t>(n_obs - 1) * kCoefStride + k] = NAN;  // maturity
+    if (rq.coefs) std::memcpy(rq.coefs, coefs.data(), coefs.size() * sizeof(double));
+    rq.paths_ms = paths_ms;
+    rq.regression_ms = ev.ms(1, 2);
+    rq.apply_ms = ev.ms(2, 3);
+    rq.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+}
+
+}  // namespace
+
+void check_lsm_request(const optlab_lsm_request& rq) {
+    const optlab_paths_request& p = rq.paths;
+    check_paths_request(p);
+    if (p.n_names != 1) invalid("an American option is on one name");
+    if (p.n_obs < 2) invalid("n_obs (exercise dates) must be >= 2");
+    if (p.n_windows != 0 || p.n_keep != p.n_obs || p.n_cols != p.n_obs) invalid("keep every exercise date, no windows");
+    for (int j = 0; j < p.n_obs; ++j)
+        if (p.col_of[j] != j) invalid("keep every exercise date, no windows");
+    if (p.n_pairs < 1) invalid("n_pairs must be >= 1");
+    if (!(rq.strike > 0)) invalid("strike must be > 0");
+    if (!rq.disc) invalid("null array");
+    for (int j = 0; j < p.n_obs; ++j)
+        if (!(rq.disc[j] > 0)) invalid("discount factors must be > 0");
+    if (rq.min_itm < 0) invalid("min_itm must be >= 0");
+    if (rq.basis_var && !(p.kind[0] == 2 || p.kind[0] == 3)) invalid("basis_var needs a Heston or SLV name");
+}
+
+void run_lsm(optlab_lsm_request& rq, int prec) {
+    if (rq.basis_var)
+        sweep<kMaxBasis>(rq, prec);
+    else
+        sweep<4>(rq, prec);
+}
+
+}  // namespace optlab_mi

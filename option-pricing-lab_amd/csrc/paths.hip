@@ -48,6 +48,7 @@ struct PathArgs {
     int n_win;
     int win_lo[2], win_hi[2];
     double* out;
+    double* out_var;     // or null: the variance factor of name 0 at the kept columns, as out's first name
     int n_tab;           // names reading a table
     int tab_of[8];       // those names, in order
     int tab_slot[8];     // slot of name a among them (-1: none)
@@ -85,8 +86,12 @@ struct Extremes {
 // paths.hpp:241-272, for the two legs of a lane's pair (output rows row[0], row[1]).
 template <typename R, int N>
 __device__ __forceinline__ void record(const PathArgs<R>& s, const long long (&row)[2], bool active, int j,
-                                       const R (&x)[2][N], Extremes<R, N>& e) {
+                                       const R (&x)[2][N], const R (&v)[2][N], Extremes<R, N>& e) {
     const int c = ldc(s.col_of, j);
+    if (s.out_var && c >= 0 && active) {
+#pragma unroll
+        for (int l = 0; l < 2; ++l) s.out_var[c * s.rows + row[l]] = static_cast<double>(v[l][0]);
+    }
 #pragma unroll
     for (int a = 0; a < N; ++a) {
         if (a >= s.n_names) break;
@@ -150,7 +155,7 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
     const bool staged = !CV && s.staged, has_var = !CV && s.has_var, mixer = !CV && s.mixer;
     Extremes<R, N> ext;
     int j = 0;
-    while (j < s.n_obs && ldc(s.obs_step, j) == 0) record<R, N>(s, row, active, j++, x, ext);
+    while (j < s.n_obs && ldc(s.obs_step, j) == 0) record<R, N>(s, row, active, j++, x, v, ext);
 
     NormalStream<R> rng, aux;
     rng.init(pair, s.key0, s.key1, 0u);
@@ -273,7 +278,7 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
                 x[leg][a] += R(-0.5) * var * dt + r_sqrt(var * dt) * z[a];
             }
         }
-        while (j < s.n_obs && ldc(s.obs_step, j) == i) record<R, N>(s, row, active, j++, x, ext);
+        while (j < s.n_obs && ldc(s.obs_step, j) == i) record<R, N>(s, row, active, j++, x, v, ext);
     }
 }
 
@@ -311,7 +316,7 @@ void launch_n(const PathArgs<R>& s, size_t lds, hipStream_t stream) {
 }
 
 template <typename R>
-double launch_typed(const optlab_paths_request& rq, double* d_out, hipStream_t stream) {
+double launch_typed(const optlab_paths_request& rq, double* d_out, double* d_var, hipStream_t stream) {
     const int n = rq.n_names;
     const long long nsn = static_cast<long long>(n) * rq.n_steps;
     Arena ar;
@@ -355,6 +360,7 @@ double launch_typed(const optlab_paths_request& rq, double* d_out, hipStream_t s
         s.win_hi[w] = w < rq.n_windows ? rq.win_hi[w] : -1;
     }
     s.out = d_out;
+    s.out_var = d_var;
     s.n_tab = 0;
     s.has_var = s.has_slv = 0;
     for (int a = 0; a < 8; ++a) s.tab_slot[a] = -1, s.tab_of[a] = 0;
@@ -406,8 +412,8 @@ __global__ void normals_kernel(long long first_pair, long long n_pairs, int n_no
 
 }  // namespace
 
-double launch_paths(const optlab_paths_request& rq, int prec, double* d_out, hipStream_t stream) {
-    return prec == 0 ? launch_typed<double>(rq, d_out, stream) : launch_typed<float>(rq, d_out, stream);
+double launch_paths(const optlab_paths_request& rq, int prec, double* d_out, hipStream_t stream, double* d_var) {
+    return prec == 0 ? launch_typed<double>(rq, d_out, d_var, stream) : launch_typed<float>(rq, d_out, d_var, stream);
 }
 
 void launch_philox(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1, long long n,

@@ -414,6 +414,51 @@ def simulate(model, market, times, n_paths: int, seed=None, dt_max=1 / 252, dt_m
     return spots[0]
 
 
+@dataclass
+class LSMResult:
+    """american_lsm's result: MCResult's (price, stderr, n_paths) (products.py:16-24), the regression coefficients
+    of each exercise date (NaN where the date exercises nothing) and the device timings."""
+
+    price: float
+    stderr: float
+    n_paths: int
+    coefs: np.ndarray
+    timing: dict
+
+    def __iter__(self):
+        return iter((self.price, self.stderr))
+
+
+def american_lsm(model, market, K, T, cp="put", n_exercise=50, n_paths=100_000, rng=None, dt_max=1 / 252,
+                 backend: str = "auto", min_itm: int = 50) -> LSMResult:
+    """`core.products.american_lsm` (products.py:112-169) on the engine: Longstaff-Schwartz with the basis
+    (1, s, s^2, s^3) in s = S / K, plus (v, v^2, s v) when the model has a variance factor, regressing on the
+    in-the-money paths only; the rule is estimated on the first n_paths / 2 pairs of the seed and applied to the next
+    n_paths / 2 (an independent set: a lower bound up to Monte Carlo error). The regressions run on the device with
+    FP64 normal equations (the reference: NumPy lstsq); the paths are the engine's (Philox), in the backend's
+    precision."""
+    lib = _need_lib()
+    prec = precision_code(backend)
+    times = np.linspace(T / n_exercise, T, n_exercise)
+    p = plan(model, market, times, dt_max)
+    if len(p.names) != 1:
+        raise ValueError("american_lsm prices an option on one name")
+    pr, keepalive, _ = paths_request(p, n_paths // 2, seed64(rng))
+    rq = _abi.LsmRequest()
+    rq.paths = pr
+    disc = np.ascontiguousarray(market.D(times), dtype=float)
+    coefs = np.empty((n_exercise, 7))
+    rq.strike, rq.call = float(K), int(str(cp).lower().startswith("c"))
+    rq.disc, rq.spot = _abi.ptr(disc), float(market.spot)
+    rq.basis_var, rq.min_itm = int(bool(model.has_variance)), int(min_itm)
+    rq.coefs = _abi.ptr(coefs)
+    _check(lib, lib.optlab_lsm(C.byref(rq), prec))
+    del keepalive
+    return LSMResult(rq.price, rq.se, n_paths, coefs,
+                     {"paths_ms": rq.paths_ms, "regression_ms": rq.regression_ms, "apply_ms": rq.apply_ms,
+                      "total_ms": rq.total_ms, "precision": "fp64" if prec == 0 else "fp32"})
+
+
 def leverage_request(params: HestonParams, surface, T_max: float, mix: float, n_particles: int, dt_max, dt_min,
                      n_x, bandwidth_scale, rng, l_bounds, regression, positive_part, n_bins=400, n_fine=8192,
                      atm_vol=None, bandwidth=None):

@@ -293,6 +293,27 @@ class BookRequest(C.Structure):
     ]
 
 
+class LsmRequest(C.Structure):
+    """optlab_lsm_request (addition: an American option by Longstaff-Schwartz on the engine's paths)."""
+
+    _fields_ = [
+        ("paths", PathsRequest),
+        ("strike", _D),
+        ("call", _I),
+        ("disc", _PD),
+        ("spot", _D),
+        ("basis_var", _I),
+        ("min_itm", _I),
+        ("price", _D),
+        ("se", _D),
+        ("coefs", _PD),
+        ("paths_ms", _D),
+        ("regression_ms", _D),
+        ("apply_ms", _D),
+        ("total_ms", _D),
+    ]
+
+
 def ptr(a, kind=_PD):
     """A ctypes pointer to a NumPy array's data (None for None or an empty array)."""
     if a is None or a.size == 0:
@@ -328,6 +349,7 @@ _SIGNATURES = {
     "optlab_autocall": (_I, [C.POINTER(AutocallRequest), _I]),
     "optlab_adjoint": (_I, [C.POINTER(AutocallRequest), _I]),
     "optlab_book": (_I, [C.POINTER(BookRequest), _I]),
+    "optlab_lsm": (_I, [C.POINTER(LsmRequest), _I]),
     "optlab_philox_dev": (_I, [C.c_uint] * 6 + [_LL, C.c_void_p, C.c_void_p]),
     "optlab_normals_dev": (_I, [_LL, _LL, _I, C.c_uint, _ULL, _I, C.c_void_p, C.c_void_p]),
 }

@@ -54,6 +54,7 @@ def test_ctypes_layouts_match_the_c_header(tmp_path):
         "optlab_autocall_request": _abi.AutocallRequest,
         "optlab_trade": _abi.Trade,
         "optlab_book_request": _abi.BookRequest,
+        "optlab_lsm_request": _abi.LsmRequest,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for cname, py in structs.items():
@@ -133,6 +134,7 @@ def test_without_a_device_the_engine_refuses_loudly():
     assert lib.optlab_adjoint(C.byref(_abi.AutocallRequest()), 0) == _abi.EDEVICE
     assert lib.optlab_adjoint(C.byref(_abi.AutocallRequest()), 2) == _abi.EINVAL  # backend
     assert lib.optlab_book(C.byref(_abi.BookRequest()), 1) == _abi.EDEVICE
+    assert lib.optlab_lsm(C.byref(_abi.LsmRequest()), 1) == _abi.EDEVICE
 
 
 def test_the_engine_is_found_by_name():
