@@ -331,6 +331,60 @@ def bench_fixings(n_paths: int = 1 << 18, reps: int = 5) -> dict:
     return out
 
 
+def bench_lsm(n_paths: int = 1 << 20, n_ex: int = 50, reps: int = 5, cpu: bool = True) -> dict:
+    """American puts by Longstaff-Schwartz on the device: the first row of Longstaff and Schwartz (2001), table 1
+    (S0 36, K 40, r .06, sigma .2, T 1, 50 exercise dates; Black-Scholes paths of 300 steps, 6 per date), n_paths
+    paths for the rule and n_paths more for the estimate, in FP32 and FP64; and a Heston put (S0 100, K 110,
+    the basis with the variance factor). Medians of reps calls after a warm-up: the whole call, and the library's
+    HIP events around the paths, the backward sweep of regressions and the pass that applies the rule. With cpu,
+    the oracle's NumPy regression (normal equations, the reference's algorithm) on the same FP64 paths, on a sample
+    of 2^17 paths, as the CPU baseline of the sweep."""
+    import numpy as np
+
+    from optlab import native
+    from optlab.core.market import Market
+    from optlab.core.models import BlackScholesModel, HestonModel, HestonParams
+
+    cases = {"bs_put_ls_table1": (BlackScholesModel(0.2), Market.flat(36.0, 0.06, 0.0), 40.0),
+             "heston_put_var_basis": (HestonModel(HestonParams(0.04, 1.5, 0.04, 0.6, -0.7)),
+                                      Market.flat(100.0, 0.03, 0.0), 110.0)}
+    out = {}
+    for name, (model, market, strike) in cases.items():
+        for backend in ("fp32", "fp64"):
+            native.american_lsm(model, market, strike, 1.0, "put", n_ex, n_paths, rng=1, backend=backend)
+            runs = [native.american_lsm(model, market, strike, 1.0, "put", n_ex, n_paths, rng=1, backend=backend)
+                    for _ in range(reps)]
+            med = {k: float(np.median([r.timing[k] for r in runs]))
+                   for k in ("total_ms", "paths_ms", "regression_ms", "apply_ms")}
+            med.update(price=runs[0].price, stderr=runs[0].stderr,
+                       path_dates_per_s=2 * n_paths * n_ex / (med["total_ms"] * 1e-3))
+            out[f"{name}_{backend}"] = med
+    out["workload"] = (f"American put, {n_ex} exercise dates, {n_paths} paths for the rule + {n_paths} for the "
+                       "estimate, FP64 normal equations on the device")
+    if cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import lsm_oracle
+
+        model, market, strike = cases["bs_put_ls_table1"]
+        times = np.linspace(1.0 / n_ex, 1.0, n_ex)
+        p = native.plan(model, market, times)
+        n_cpu = 1 << 17
+        S1 = native.run(p, n_cpu // 2, native.seed64(1), backend="fp64")[0][0]
+        disc = market.D(times)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            lsm_oracle.lsm_rule(S1, None, disc, strike)
+            ts.append(time.perf_counter() - t0)
+        ms = float(np.median(ts)) * 1e3
+        gpu = out["bs_put_ls_table1_fp64"]["regression_ms"]
+        out["cpu_baseline"] = {"value": n_cpu * (n_ex - 1) / (ms * 1e-3), "unit": "path-dates/s of the sweep",
+                               "cores": 1, "kind": "port",
+                               "sample": f"oracle/lsm_oracle.py lsm_rule (NumPy) on {n_cpu} FP64 paths x {n_ex} dates"}
+        out["sweep_gpu_path_dates_per_s"] = n_paths * (n_ex - 1) / (gpu * 1e-3)
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -343,7 +397,7 @@ def main(argv=None) -> int:
     ap.add_argument("--book-scale", type=int, default=16,
                     help="divide every trade's paths of the batched book by this (0: skip the book)")
     ap.add_argument("--slv-particles", type=int, default=1 << 20, help="SLV particles (0: skip the SLV block)")
-    ap.add_argument("--only", choices=("adjoint", "book", "slv", "fixings"), default=None,
+    ap.add_argument("--only", choices=("adjoint", "book", "slv", "fixings", "lsm"), default=None,
                     help="run only this block and print its JSON (for profiling runs; the benchmark line needs all)")
     ap.add_argument("--no-bumps", action="store_true", help="with --only adjoint: skip the bump-and-revalue comparator")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -368,7 +422,8 @@ def main(argv=None) -> int:
         block = {"adjoint": lambda: bench_adjoint(args.adjoint_paths, bumps=not args.no_bumps),
                  "book": lambda: bench_book(args.book_scale),
                  "slv": lambda: bench_slv(args.slv_particles),
-                 "fixings": lambda: bench_fixings()}[args.only]()
+                 "fixings": lambda: bench_fixings(),
+                 "lsm": lambda: bench_lsm(cpu=not args.no_cpu)}[args.only]()
         print(json.dumps({args.only: block}), flush=True)
         return 0
     ms32, kern32, p32, se32 = _time_gpu(lib, 1, args.steps, args.warmup)
@@ -443,6 +498,8 @@ def main(argv=None) -> int:
         print("bench: slv block done", file=sys.stderr, flush=True)
     line["fixings"] = bench_fixings()
     print("bench: fixings block done", file=sys.stderr, flush=True)
+    line["lsm"] = bench_lsm(cpu=not args.no_cpu)
+    print("bench: lsm block done", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
     if args.copy_mib:
         src = torch.ones(args.copy_mib << 20, dtype=torch.uint8, device="cuda")

@@ -299,7 +299,10 @@ typedef struct {
     int min_itm;                /* regress only with more than this many paths in the money (50) */
     double price, se;
     double* coefs;              /* host or NULL: n_obs x 7 regression coefficients, NaN at dates that exercise nothing */
-    double paths_ms, regression_ms, apply_ms, total_ms;
+    double paths_ms;            /* both path passes (device events around the path kernels) */
+    double regression_ms;       /* the backward sweep: two launches a date (device events) */
+    double apply_ms;            /* the rule applied to pass 2 (device events around that kernel) */
+    double total_ms;            /* the whole call, host wall clock */
 } optlab_lsm_request;
 
 /* ---- entry points ------------------------------------------------------------------------------------------- */

@@ -5,12 +5,16 @@
 // independent set.
 //
 // MI355X design: the path engine writes the spots (and the variance factor) at the exercise dates into HBM,
-// structure of arrays (a date's paths contiguous). Each date of the backward sweep is three launches on one stream,
-// with no host round trip: a grid-stride pass forms the normal equations of the in-the-money paths (X'X, X'y and
-// the count: 15 or 36 FP64 sums) as per-workgroup rows, one workgroup adds the rows in a fixed order and solves the
-// system by an FP64 Cholesky factorisation, and a second pass moves the exercised paths' values. The rule is then
-// applied to the second set of paths in one pass that reduces pair averages. Every sum has a fixed order, so a
-// result does not depend on anything but the request.
+// structure of arrays (a date's paths contiguous). Each date of the backward sweep is two launches on one stream,
+// with no host round trip: a grid-stride pass first brings every path's discounted value to the later date (its
+// payoff at maturity, or that date's exercise decision by the rule just fitted), then forms the normal equations of
+// the in-the-money paths (X'X, X'y and the count: 15 or 36 FP64 sums) as per-workgroup rows; one workgroup adds the
+// rows in a fixed order and solves the system by an FP64 Cholesky factorisation. The rule is then applied to the
+// second set of paths in one pass that reduces pair averages. Every sum has a fixed order, so a result does not
+// depend on anything but the request.
+//
+// The kernel boundary publishes the rows to the solve: a device-scope fence in every workgroup (to let the last one
+// solve) writes back the L2 of its XCD each time, which cost more than the launch it saves.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -60,169 +64,254 @@ __device__ __forceinline__ void basis(double s, double v, double (&X)[NB]) {
     }
 }
 
-// Sum over a workgroup of K values per lane; thread k < K of the workgroup returns total k (others: 0).
+// Sum over a workgroup of K values per lane; thread k < K of the workgroup returns total k (others: 0). In a wave
+// the K values (padded to P, a power of two) are reduce-scattered by halving: at the level of xor mask h a lane
+// keeps the half of its values whose index has bit h equal to its lane's and adds its partner's copy of them, so
+// after log2 P levels lane l holds value l & (P - 1) summed over its group of P lanes; the remaining levels add the
+// groups. P - 1 + log2(64 / P) shuffles instead of 6 K. Then wave w's lane k < K writes slot (w, k), and thread k
+// adds the waves in order. The order of every addition is fixed.
 template <int K>
 __device__ __forceinline__ double workgroup_sums(double (&acc)[K], double* lds /* [waves][K] */) {
+    static_assert(K >= 1 && K <= 64, "at most 64 sums");
+    constexpr int P = K <= 1 ? 1 : K <= 2 ? 2 : K <= 4 ? 4 : K <= 8 ? 8 : K <= 16 ? 16 : K <= 32 ? 32 : 64;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
+    double v[P];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        double v = acc[k];
+    for (int i = 0; i < P; ++i) v[i] = i < K ? acc[i] : 0.0;
 #pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) v += shfl_xor_d(v, m);
-        acc[k] = v;
+    for (int half = P / 2; half >= 1; half >>= 1) {
+        const bool upper = (lane & half) != 0;
+#pragma unroll
+        for (int i = 0; i < half; ++i) {
+            const double keep = upper ? v[i + half] : v[i];
+            const double give = upper ? v[i] : v[i + half];
+            v[i] = keep + shfl_xor_d(give, half);
+        }
     }
-    if (lane == 0) {
+    double t = v[0];
 #pragma unroll
-        for (int k = 0; k < K; ++k) lds[wave * K + k] = acc[k];
-    }
+    for (int m = P; m < 64; m <<= 1) t += shfl_xor_d(t, m);
+    if (lane < K) lds[wave * K + lane] = t;
     __syncthreads();
-    double t = 0.0;
+    double r = 0.0;
     if (threadIdx.x < K)
-        for (int w = 0; w < n_waves; ++w) t += lds[w * K + threadIdx.x];
-    return t;
+        for (int w = 0; w < n_waves; ++w) r += lds[w * K + threadIdx.x];
+    return r;
 }
 
-// The normal equations of date j over the in-the-money paths: one row of n_sums per workgroup.
+// FP64 Cholesky solve of X'X beta = X'y from the sums (lower triangle of X'X, X'y, the count), by one lane; one
+// division a column (the reciprocal of the diagonal, then products). False when the date has min_itm paths or fewer
+// in the money, or X'X is not numerically positive definite.
 template <int NB>
-__global__ void __launch_bounds__(kLsmBlock) lsm_sums_kernel(LsmArgs a, int j) {
+__device__ bool cholesky_solve(const double* tot, int min_itm, double (&beta)[NB]) {
+    double L[NB][NB], b[NB], inv[NB];
+    int m = 0;
+#pragma unroll
+    for (int r = 0; r < NB; ++r)
+#pragma unroll
+        for (int c = 0; c <= r; ++c) L[r][c] = tot[m++];
+#pragma unroll
+    for (int r = 0; r < NB; ++r) b[r] = tot[m++];
+    if (!(tot[m] > static_cast<double>(min_itm))) return false;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {  // A = L L' in place (lower triangle)
+        double d = L[c][c];
+#pragma unroll
+        for (int k = 0; k < c; ++k) d -= L[c][k] * L[c][k];
+        if (!(d > 0.0)) return false;
+        L[c][c] = sqrt(d);
+        inv[c] = 1.0 / L[c][c];
+#pragma unroll
+        for (int r = c + 1; r < NB; ++r) {
+            double e = L[r][c];
+#pragma unroll
+            for (int k = 0; k < c; ++k) e -= L[r][k] * L[c][k];
+            L[r][c] = e * inv[c];
+        }
+    }
+    double y[NB];
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+        double e = b[r];
+#pragma unroll
+        for (int k = 0; k < r; ++k) e -= L[r][k] * y[k];
+        y[r] = e * inv[r];
+    }
+#pragma unroll
+    for (int r = NB - 1; r >= 0; --r) {
+        double e = y[r];
+#pragma unroll
+        for (int k = r + 1; k < NB; ++k) e -= L[k][r] * beta[k];
+        beta[r] = e * inv[r];
+    }
+    return true;
+}
+
+constexpr int kLsmUnroll = 4;  // rows a lane loads before it uses any: the pass is bound by memory latency
+
+// Date j of the backward sweep (products.py:151-158), first launch. Each path's discounted value is brought to date
+// j + 1 first: the payoff at maturity when j + 1 is the last date, else the exercise decision of date j + 1 by the
+// rule the previous solve fitted (an in-the-money path exercises when its exercise value beats the fitted
+// continuation). Then the in-the-money paths of date j add their terms of the normal equations to a per-workgroup
+// row.
+template <int NB>
+__global__ void __launch_bounds__(kLsmBlock) lsm_date_kernel(LsmArgs a, int j) {
     constexpr int K = n_sums<NB>();
+    constexpr int U = kLsmUnroll;
     __shared__ double lds[(kLsmBlock / 64) * K];
+    const int nx = j + 1;
+    const bool at_maturity = nx == a.n_obs - 1;
+    const bool rule = !at_maturity && ldc(a.valid, nx);
+    double bn[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) bn[k] = rule ? ldc(a.coefs, nx * kCoefStride + k) : 0.0;
+    const double* Sn = a.S + static_cast<long long>(nx) * a.rows;
+    const double* Vn = a.V ? a.V + static_cast<long long>(nx) * a.rows : nullptr;
+    const double* Sj = a.S + static_cast<long long>(j) * a.rows;
+    const double* Vj = a.V ? a.V + static_cast<long long>(j) * a.rows : nullptr;
+    const double dn = ldc(a.disc, nx), dj = ldc(a.disc, j);
     double acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = 0.0;
-    const double* Sj = a.S + static_cast<long long>(j) * a.rows;
-    const double* Vj = a.V ? a.V + static_cast<long long>(j) * a.rows : nullptr;
-    const double dj = ldc(a.disc, j);
-    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
-    for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.rows; i += stride) {
-        const double s = Sj[i] * a.inv_strike;
-        const double ex = fmax(a.phi * (s - 1.0), 0.0);
-        if (ex > 0.0) {
-            double X[NB];
-            basis<NB>(s, Vj ? Vj[i] : 0.0, X);
-            const double y = a.value[i] / dj;
-            int m = 0;
+    const long long tile = static_cast<long long>(kLsmBlock) * U;
+    for (long long base = blockIdx.x * tile; base < a.rows; base += gridDim.x * tile) {
+        double sn[U], sj[U], wn[U], wj[U], val[U];
 #pragma unroll
-            for (int r = 0; r < NB; ++r)
+        for (int u = 0; u < U; ++u) {
+            const long long i = base + u * kLsmBlock + threadIdx.x;
+            const long long k = i < a.rows ? i : 0;
+            sn[u] = Sn[k];
+            sj[u] = Sj[k];
+            wn[u] = Vn ? Vn[k] : 0.0;
+            wj[u] = Vj ? Vj[k] : 0.0;
+            val[u] = at_maturity ? 0.0 : a.value[k];
+        }
 #pragma unroll
-                for (int c = 0; c <= r; ++c) acc[m++] += X[r] * X[c];
+        for (int u = 0; u < U; ++u) {
+            const long long i = base + u * kLsmBlock + threadIdx.x;
+            if (i >= a.rows) continue;
+            const double s1 = sn[u] * a.inv_strike;
+            const double exn = fmax(a.phi * (s1 - 1.0), 0.0);
+            double v = val[u];
+            if (at_maturity) {
+                v = exn * dn;
+                a.value[i] = v;
+            } else if (rule && exn > 0.0) {
+                double X[NB];
+                basis<NB>(s1, wn[u], X);
+                double cont = 0.0;
 #pragma unroll
-            for (int r = 0; r < NB; ++r) acc[m++] += X[r] * y;
-            acc[m] += 1.0;
+                for (int k = 0; k < NB; ++k) cont += X[k] * bn[k];
+                if (exn > cont) {
+                    v = exn * dn;
+                    a.value[i] = v;
+                }
+            }
+            const double s = sj[u] * a.inv_strike;
+            if (a.phi * (s - 1.0) > 0.0) {
+                double X[NB];
+                basis<NB>(s, wj[u], X);
+                const double y = v / dj;
+                int m = 0;
+#pragma unroll
+                for (int r = 0; r < NB; ++r)
+#pragma unroll
+                    for (int c = 0; c <= r; ++c) acc[m++] += X[r] * X[c];
+#pragma unroll
+                for (int r = 0; r < NB; ++r) acc[m++] += X[r] * y;
+                acc[m] += 1.0;
+            }
         }
     }
     const double t = workgroup_sums<K>(acc, lds);
     if (threadIdx.x < K) a.partial[static_cast<long long>(blockIdx.x) * K + threadIdx.x] = t;
 }
 
-// The rows added in order, then (one lane) the FP64 Cholesky solve of X'X beta = X'y.
+// Date j, second launch (one workgroup): the n_rows rows added in a fixed order (lane l takes rows l, l + 256, ...,
+// then a fixed tree), the solve, the coefficients of date j (NaN when not regressed) and its flag.
 template <int NB>
-__global__ void __launch_bounds__(64) lsm_solve_kernel(LsmArgs a, int j, int n_rows) {
+__global__ void __launch_bounds__(kLsmBlock) lsm_solve_kernel(LsmArgs a, int j, int n_rows) {
     constexpr int K = n_sums<NB>();
+    __shared__ double lds[(kLsmBlock / 64) * K];
     __shared__ double tot[K];
-    for (int k = threadIdx.x; k < K; k += blockDim.x) {
-        double t = 0.0;
-        for (int r = 0; r < n_rows; ++r) t += a.partial[static_cast<long long>(r) * K + k];
-        tot[k] = t;
-    }
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = 0.0;
+    for (int r = threadIdx.x; r < n_rows; r += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] += a.partial[static_cast<long long>(r) * K + k];
+    const double total = workgroup_sums<K>(acc, lds);
+    if (threadIdx.x < K) tot[threadIdx.x] = total;
     __syncthreads();
     if (threadIdx.x != 0) return;
-    double L[NB][NB], b[NB];
-    int m = 0;
-    for (int r = 0; r < NB; ++r)
-        for (int c = 0; c <= r; ++c) L[r][c] = tot[m++];
-    for (int r = 0; r < NB; ++r) b[r] = tot[m++];
-    bool ok = tot[m] > static_cast<double>(a.min_itm);
-    for (int c = 0; c < NB && ok; ++c) {  // A = L L' in place (lower triangle)
-        double d = L[c][c];
-        for (int k = 0; k < c; ++k) d -= L[c][k] * L[c][k];
-        if (!(d > 0.0)) {
-            ok = false;
-            break;
-        }
-        L[c][c] = sqrt(d);
-        for (int r = c + 1; r < NB; ++r) {
-            double e = L[r][c];
-            for (int k = 0; k < c; ++k) e -= L[r][k] * L[c][k];
-            L[r][c] = e / L[c][c];
-        }
-    }
     double beta[NB];
-    if (ok) {
-        double y[NB];
-        for (int r = 0; r < NB; ++r) {
-            double e = b[r];
-            for (int k = 0; k < r; ++k) e -= L[r][k] * y[k];
-            y[r] = e / L[r][r];
-        }
-        for (int r = NB - 1; r >= 0; --r) {
-            double e = y[r];
-            for (int k = r + 1; k < NB; ++k) e -= L[k][r] * beta[k];
-            beta[r] = e / L[r][r];
-        }
-    }
-    for (int k = 0; k < kCoefStride; ++k) a.coefs[j * kCoefStride + k] = ok && k < NB ? beta[k] : (ok ? 0.0 : NAN);
+    const bool ok = cholesky_solve<NB>(tot, a.min_itm, beta);
+    for (int k = 0; k < kCoefStride; ++k) a.coefs[j * kCoefStride + k] = ok ? (k < NB ? beta[k] : 0.0) : NAN;
     a.valid[j] = ok ? 1 : 0;
 }
 
-// products.py:155-158: the in-the-money paths whose exercise value beats the fitted continuation exercise at j.
-template <int NB>
-__global__ void __launch_bounds__(kLsmBlock) lsm_update_kernel(LsmArgs a, int j) {
-    if (!ldc(a.valid, j)) return;
-    double beta[NB];
-#pragma unroll
-    for (int k = 0; k < NB; ++k) beta[k] = ldc(a.coefs, j * kCoefStride + k);
-    const double* Sj = a.S + static_cast<long long>(j) * a.rows;
-    const double* Vj = a.V ? a.V + static_cast<long long>(j) * a.rows : nullptr;
-    const double dj = ldc(a.disc, j);
-    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
-    for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.rows; i += stride) {
-        const double s = Sj[i] * a.inv_strike;
-        const double ex = fmax(a.phi * (s - 1.0), 0.0);
-        if (ex > 0.0) {
-            double X[NB];
-            basis<NB>(s, Vj ? Vj[i] : 0.0, X);
-            double cont = 0.0;
-#pragma unroll
-            for (int k = 0; k < NB; ++k) cont += X[k] * beta[k];
-            if (ex > cont) a.value[i] = ex * dj;
-        }
-    }
-}
+constexpr int kApplyDates = 4;  // dates of both legs a lane loads before it evaluates them
 
 // products.py:137-150 on the second set of paths: each leg exercises at the first date whose rule says so (the
 // reference's backward loop keeps the earliest), else at maturity; the pair average of K cash D(t_tau) and its
-// square, summed per workgroup.
+// square, summed per workgroup. The two legs of a pair walk the dates together, kApplyDates at a time.
 template <int NB>
 __global__ void __launch_bounds__(kLsmBlock) lsm_apply_kernel(LsmArgs a, double strike, double* sums) {
+    constexpr int D = kApplyDates;
     __shared__ double lds[(kLsmBlock / 64) * 2];
     const long long n_pairs = a.rows / 2;
+    const int last = a.n_obs - 1;
     double acc[2] = {0.0, 0.0};
     const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
     for (long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; p < n_pairs; p += stride) {
-        double pay = 0.0;
-        for (int leg = 0; leg < 2; ++leg) {
-            const long long i = leg ? n_pairs + p : p;
-            const int last = a.n_obs - 1;
-            double cash = fmax(a.phi * (a.S[static_cast<long long>(last) * a.rows + i] * a.inv_strike - 1.0), 0.0);
-            int tau = last;
-            for (int j = 0; j < last; ++j) {
-                if (!ldc(a.valid, j)) continue;
-                const double s = a.S[static_cast<long long>(j) * a.rows + i] * a.inv_strike;
-                const double ex = fmax(a.phi * (s - 1.0), 0.0);
-                if (!(ex > 0.0)) continue;
-                double X[NB];
-                basis<NB>(s, a.V ? a.V[static_cast<long long>(j) * a.rows + i] : 0.0, X);
-                double cont = 0.0;
+        const long long row[2] = {p, n_pairs + p};
+        double cash[2] = {0.0, 0.0};
+        int tau[2] = {last, last};
+        bool done[2] = {false, false};
+        for (int j0 = 0; j0 < last && !(done[0] && done[1]); j0 += D) {
+            double s[2][D], w[2][D];
 #pragma unroll
-                for (int k = 0; k < NB; ++k) cont += X[k] * ldc(a.coefs, j * kCoefStride + k);
-                if (ex > cont) {
-                    cash = ex;
-                    tau = j;
-                    break;
+            for (int d = 0; d < D; ++d) {
+                const long long at = static_cast<long long>(min(j0 + d, last)) * a.rows;
+#pragma unroll
+                for (int leg = 0; leg < 2; ++leg) {
+                    s[leg][d] = a.S[at + row[leg]];
+                    w[leg][d] = a.V ? a.V[at + row[leg]] : 0.0;
                 }
             }
-            pay += strike * cash * ldc(a.disc, tau);
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int j = j0 + d;
+                if (j >= last || !ldc(a.valid, j)) continue;
+                double beta[NB];
+#pragma unroll
+                for (int k = 0; k < NB; ++k) beta[k] = ldc(a.coefs, j * kCoefStride + k);
+#pragma unroll
+                for (int leg = 0; leg < 2; ++leg) {
+                    if (done[leg]) continue;
+                    const double x = s[leg][d] * a.inv_strike;
+                    const double ex = fmax(a.phi * (x - 1.0), 0.0);
+                    if (!(ex > 0.0)) continue;
+                    double X[NB];
+                    basis<NB>(x, w[leg][d], X);
+                    double cont = 0.0;
+#pragma unroll
+                    for (int k = 0; k < NB; ++k) cont += X[k] * beta[k];
+                    if (ex > cont) {
+                        cash[leg] = ex;
+                        tau[leg] = j;
+                        done[leg] = true;
+                    }
+                }
+            }
+        }
+        double pay = 0.0;
+#pragma unroll
+        for (int leg = 0; leg < 2; ++leg) {
+            if (!done[leg])
+                cash[leg] =
+                    fmax(a.phi * (a.S[static_cast<long long>(last) * a.rows + row[leg]] * a.inv_strike - 1.0), 0.0);
+            pay += strike * cash[leg] * ldc(a.disc, tau[leg]);
         }
         const double d = 0.5 * pay;
         acc[0] += d;
@@ -232,17 +321,8 @@ __global__ void __launch_bounds__(kLsmBlock) lsm_apply_kernel(LsmArgs a, double 
     if (threadIdx.x < 2) sums[blockIdx.x * 2 + threadIdx.x] = t;
 }
 
-// The value of every path of pass 1 at the last date, discounted (products.py:147).
-__global__ void lsm_init_kernel(LsmArgs a) {
-    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= a.rows) return;
-    const int last = a.n_obs - 1;
-    const double s = a.S[static_cast<long long>(last) * a.rows + i] * a.inv_strike;
-    a.value[i] = fmax(a.phi * (s - 1.0), 0.0) * ldc(a.disc, last);
-}
-
 struct Events {
-    hipEvent_t e[4];
+    hipEvent_t e[5];
     Events() {
         for (auto& x : e) hip_check(hipEventCreate(&x), "event");
     }
@@ -263,8 +343,13 @@ void sweep(optlab_lsm_request& rq, int prec) {
     const int n_obs = rq.paths.n_obs;
     const long long n_pairs = rq.paths.n_pairs, rows = 2 * n_pairs;
     constexpr int K = n_sums<NB>();
-    const int blocks = static_cast<int>(std::max<long long>(
-        1, std::min<long long>((rows + kLsmBlock - 1) / kLsmBlock, 4LL * device_cu_count())));
+    // the sweep: two workgroups a CU, kLsmUnroll rows a lane at a time (the solve adds two rows a lane; four
+    // workgroups a CU measured slower: the reduction is per workgroup); the apply pass: four a CU over the pairs
+    const long long tile = static_cast<long long>(kLsmBlock) * kLsmUnroll;
+    const int blocks = static_cast<int>(
+        std::max<long long>(1, std::min<long long>((rows + tile - 1) / tile, 2LL * device_cu_count())));
+    const int apply_blocks = static_cast<int>(std::max<long long>(
+        1, std::min<long long>((n_pairs + kLsmBlock - 1) / kLsmBlock, 4LL * device_cu_count())));
     auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t s_bytes = up(static_cast<size_t>(n_obs) * rows * sizeof(double));
     const size_t v_bytes = NB == 7 ? s_bytes : 0;
@@ -273,7 +358,7 @@ void sweep(optlab_lsm_request& rq, int prec) {
     const size_t coef_bytes = up(static_cast<size_t>(n_obs) * kCoefStride * sizeof(double));
     const size_t valid_bytes = up(static_cast<size_t>(n_obs) * sizeof(int));
     const size_t disc_bytes = up(static_cast<size_t>(n_obs) * sizeof(double));
-    const size_t sums_bytes = up(static_cast<size_t>(blocks) * 2 * sizeof(double));
+    const size_t sums_bytes = up(static_cast<size_t>(apply_blocks) * 2 * sizeof(double));
     unsigned char* m = static_cast<unsigned char*>(scratch(
         kSlotLsm, s_bytes + v_bytes + val_bytes + part_bytes + coef_bytes + valid_bytes + disc_bytes + sums_bytes));
     LsmArgs a{};
@@ -302,13 +387,9 @@ void sweep(optlab_lsm_request& rq, int prec) {
     hip_check(hipEventRecord(ev.e[0], st), "event");
     double paths_ms = launch_paths(rq.paths, prec, S, st, V);
     hip_check(hipEventRecord(ev.e[1], st), "event");
-    hipLaunchKernelGGL(lsm_init_kernel, dim3(static_cast<unsigned>((rows + 255) / 256)), dim3(256), 0, st, a);
-    hip_check(hipGetLastError(), "lsm init launch");
-    hip_check(hipMemsetAsync(a.valid, 0, n_obs * sizeof(int), st), "clear valid");
-    for (int j = n_obs - 2; j >= 0; --j) {
-        hipLaunchKernelGGL((lsm_sums_kernel<NB>), dim3(blocks), dim3(kLsmBlock), 0, st, a, j);
-        hipLaunchKernelGGL((lsm_solve_kernel<NB>), dim3(1), dim3(64), 0, st, a, j, blocks);
-        hipLaunchKernelGGL((lsm_update_kernel<NB>), dim3(blocks), dim3(kLsmBlock), 0, st, a, j);
+    for (int j = n_obs - 2; j >= 0; --j) {  // every date's flag and coefficients are written before they are read
+        hipLaunchKernelGGL((lsm_date_kernel<NB>), dim3(blocks), dim3(kLsmBlock), 0, st, a, j);
+        hipLaunchKernelGGL((lsm_solve_kernel<NB>), dim3(1), dim3(kLsmBlock), 0, st, a, j, blocks);
     }
     hip_check(hipGetLastError(), "lsm sweep launch");
     hip_check(hipEventRecord(ev.e[2], st), "event");
@@ -316,10 +397,11 @@ void sweep(optlab_lsm_request& rq, int prec) {
     optlab_paths_request p2 = rq.paths;
     p2.first_pair = rq.paths.first_pair + n_pairs;
     paths_ms += launch_paths(p2, prec, S, st, V);
-    hipLaunchKernelGGL((lsm_apply_kernel<NB>), dim3(blocks), dim3(kLsmBlock), 0, st, a, rq.strike, sums);
-    hip_check(hipGetLastError(), "lsm apply launch");
     hip_check(hipEventRecord(ev.e[3], st), "event");
-    std::vector<double> h(static_cast<size_t>(blocks) * 2);
+    hipLaunchKernelGGL((lsm_apply_kernel<NB>), dim3(apply_blocks), dim3(kLsmBlock), 0, st, a, rq.strike, sums);
+    hip_check(hipGetLastError(), "lsm apply launch");
+    hip_check(hipEventRecord(ev.e[4], st), "event");
+    std::vector<double> h(static_cast<size_t>(apply_blocks) * 2);
     hip_check(hipMemcpyAsync(h.data(), sums, h.size() * sizeof(double), hipMemcpyDeviceToHost, st), "copy sums");
     std::vector<double> coefs(static_cast<size_t>(n_obs) * kCoefStride);
     hip_check(hipMemcpyAsync(coefs.data(), a.coefs, coefs.size() * sizeof(double), hipMemcpyDeviceToHost, st),
@@ -327,7 +409,7 @@ void sweep(optlab_lsm_request& rq, int prec) {
     hip_check(hipStreamSynchronize(st), "synchronize");
 
     double s1 = 0.0, s2 = 0.0;
-    for (int b = 0; b < blocks; ++b) s1 += h[2 * b], s2 += h[2 * b + 1];
+    for (int b = 0; b < apply_blocks; ++b) s1 += h[2 * b], s2 += h[2 * b + 1];
     const double n = static_cast<double>(n_pairs);
     rq.price = s1 / n;
     rq.se = n > 1 ? std::sqrt(std::max((s2 - s1 * s1 / n) / (n - 1.0), 0.0) / n) : 0.0;
@@ -337,7 +419,7 @@ void sweep(optlab_lsm_request& rq, int prec) {
     if (rq.coefs) std::memcpy(rq.coefs, coefs.data(), coefs.size() * sizeof(double));
     rq.paths_ms = paths_ms;
     rq.regression_ms = ev.ms(1, 2);
-    rq.apply_ms = ev.ms(2, 3);
+    rq.apply_ms = ev.ms(3, 4);
     rq.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
 }
 
