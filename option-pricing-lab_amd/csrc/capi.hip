@@ -138,7 +138,13 @@ void check_paths_request(const optlab_paths_request& c) {  // requests.cpp:168-2
         if (c.obs_step[j] < prev || c.obs_step[j] > c.n_steps) invalid("obs_step must be nondecreasing in [0, n_steps]");
         prev = c.obs_step[j];
     }
-    if (c.mixer && (c.mixer != 1 || !c.low_root || !(c.mix[4] > 0))) invalid("bad local correlation");
+    if (c.mixer < 0 || c.mixer > 2 || (c.mixer == 1 && (!c.low_root || !(c.mix[4] > 0))))
+        invalid("bad local correlation");
+    // mixer 2 (an addition: the reference engine refuses the stochastic weight): w0, theta in [0, 1], nu >= 0,
+    // |rho| <= 1, kappa >= 0
+    if (c.mixer == 2 && (!c.low_root || !(c.mix[0] >= 0 && c.mix[0] <= 1) || !(c.mix[1] >= 0 && c.mix[1] <= 1) ||
+                         !(c.mix[2] >= 0) || !(std::fabs(c.mix[3]) <= 1) || !(c.mix[4] >= 0)))
+        invalid("bad stochastic mix");
     if (c.n_windows < 0 || c.n_windows > 2 || (c.n_windows && (!c.win_lo || !c.win_hi)))
         invalid("n_windows must be 0 to 2, with win_lo and win_hi");
     if (c.n_obs && !c.col_of) invalid("null col_of");

@@ -97,6 +97,9 @@ __device__ __forceinline__ void gauss2(std::uint32_t a, std::uint32_t b, double&
 }
 
 constexpr std::uint32_t kAuxTag = 0x80000000u;
+// The stochastic mixing weight's own driver: a third stream of the pair, so that the spot normals of a basket are
+// those of the constant matrix whatever the weight does.
+constexpr std::uint32_t kMixTag = 0x40000000u;
 
 // Normals of one pair on one stream, four per Philox block (paths.hpp:213-233). The consumption pattern is the
 // same on every lane of a launch, so `used` and `block` stay wave-uniform.

@@ -129,11 +129,18 @@ __device__ __forceinline__ void record(const PathArgs<R>& s, const long long (&r
 }
 
 // paths.hpp:274-361: lane p carries both legs of pair p (output rows p and n_pairs + p).
-// CV: every name has a constant variance and there is no local-correlation mixer (Black-Scholes names and baskets),
-// so the model dispatch, the variance factor and the table staging compile away: the same arithmetic on fewer
-// scalar instructions a step.
-template <typename R, int N, bool CV>
+// The kernel's form, by what the request needs:
+// kFormCV: every name has a constant variance and there is no mixer (Black-Scholes names and baskets), so the
+// model dispatch, the variance factor and the table staging compile away: the same arithmetic on fewer scalar
+// instructions a step;
+// kFormGeneric: tables, Heston and SLV names, no mixer;
+// kFormMixed: the generic form with a mixer (local correlation, or the stochastic weight with its state and its
+// driver's stream), kept apart so that the other forms carry none of its registers.
+constexpr int kFormCV = 0, kFormGeneric = 1, kFormMixed = 2;
+
+template <typename R, int N, int FORM>
 __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
+    constexpr bool CV = FORM == kFormCV;
     extern __shared__ unsigned char lds_raw[];
     R* lds = reinterpret_cast<R*>(lds_raw);
     const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -152,14 +159,18 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
         x[0][a] = x[1][a] = R(0);
         v[0][a] = v[1][a] = (a < n && hv) ? ldc(s.hes, a * 5) : R(0);
     }
-    const bool staged = !CV && s.staged, has_var = !CV && s.has_var, mixer = !CV && s.mixer;
+    const bool staged = !CV && s.staged, has_var = !CV && s.has_var;
+    constexpr bool mixer = FORM == kFormMixed;
     Extremes<R, N> ext;
     int j = 0;
     while (j < s.n_obs && ldc(s.obs_step, j) == 0) record<R, N>(s, row, active, j++, x, v, ext);
 
-    NormalStream<R> rng, aux;
+    NormalStream<R> rng, aux, drv;
     rng.init(pair, s.key0, s.key1, 0u);
     aux.init(pair, s.key0, s.key1, kAuxTag);
+    if (mixer) drv.init(pair, s.key0, s.key1, kMixTag);
+    const bool jacobi = mixer && s.mixer == 2;  // the stochastic weight: per-leg state, its own driver
+    R wst[2] = {s.mix[0], s.mix[0]};
     const int stage_len = s.n_tab * s.n_x;
     if (staged && s.n_steps > 0) stage_rows(s, 0, lds);
 
@@ -179,7 +190,9 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
             for (int a = 0; a < N; ++a)
                 if (a < n) zr[a] = s.has_slv ? aux.next() : rng.next();
         }
+        R drive = R(0);
         if (mixer) fresh = rng.next();
+        if (jacobi) drive = drv.next();
         // root . zs (and root . zr for Heston/SLV names with the spot normals' correlation) do not depend on the
         // leg: the - leg takes their negatives
         R zc[N], wc[N];
@@ -210,14 +223,19 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
             const R sign = leg ? R(-1) : R(1);
             R z[N], w[N];
             if (mixer) {
-                R m = R(0);
+                R wt;
+                if (jacobi) {
+                    wt = wst[leg];
+                } else {
+                    R m = R(0);
 #pragma unroll
-                for (int a = 0; a < N; ++a)
-                    if (a < n) m += r_exp(x[leg][a]);
-                const R y = r_log(m / static_cast<R>(n));
-                const R u = y / (s.mix[4] * r_sqrt(r_max(t0, R(1.0 / 12))));
-                R wt = s.mix[0] + s.mix[1] + s.mix[2] * r_max(-u, R(0)) - s.mix[3] * r_max(u, R(0));
-                wt = r_min(r_max(wt, R(0)), R(1));
+                    for (int a = 0; a < N; ++a)
+                        if (a < n) m += r_exp(x[leg][a]);
+                    const R y = r_log(m / static_cast<R>(n));
+                    const R u = y / (s.mix[4] * r_sqrt(r_max(t0, R(1.0 / 12))));
+                    wt = s.mix[0] + s.mix[1] + s.mix[2] * r_max(-u, R(0)) - s.mix[3] * r_max(u, R(0));
+                    wt = r_min(r_max(wt, R(0)), R(1));
+                }
                 R zlast = zs[0];
 #pragma unroll
                 for (int a = 0; a < N; ++a)
@@ -236,6 +254,12 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
                     for (int b = 0; b < N; ++b)
                         if (b < n) acc += ldc(s.low_root, a * n + b) * other[b];
                     z[a] = sw * common + sl * acc;
+                }
+                if (jacobi) {  // localcorr.py:96-108: dw = kappa (theta - w) dt + nu sqrt(w (1 - w)) dB, clipped
+                    const R rho = s.mix[3];
+                    const R zb = rho * market + r_sqrt(R(1) - rho * rho) * (sign * drive);
+                    const R w = wt + s.mix[4] * (s.mix[1] - wt) * dt + s.mix[2] * r_sqrt(r_max(wt * (R(1) - wt), R(0)) * dt) * zb;
+                    wst[leg] = r_min(r_max(w, R(0)), R(1));
                 }
             } else {
 #pragma unroll
@@ -301,18 +325,20 @@ struct Arena {
     size_t ints(const int* src, long long n) { return put(src, src ? static_cast<size_t>(n) * sizeof(int) : 0); }
 };
 
-template <typename R, int N, bool CV>
-void launch_cv(const PathArgs<R>& s, size_t lds, hipStream_t stream) {
+template <typename R, int N, int FORM>
+void launch_form(const PathArgs<R>& s, size_t lds, hipStream_t stream) {
     const unsigned grid = static_cast<unsigned>((s.n_pairs + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL((paths_kernel<R, N, CV>), dim3(grid), dim3(kBlock), lds, stream, s);
+    hipLaunchKernelGGL((paths_kernel<R, N, FORM>), dim3(grid), dim3(kBlock), lds, stream, s);
 }
 
 template <typename R, int N>
 void launch_n(const PathArgs<R>& s, size_t lds, hipStream_t stream) {
-    if (s.n_tab == 0 && !s.has_var && !s.mixer)
-        launch_cv<R, N, true>(s, lds, stream);
+    if (s.mixer)
+        launch_form<R, N, kFormMixed>(s, lds, stream);
+    else if (s.n_tab == 0 && !s.has_var)
+        launch_form<R, N, kFormCV>(s, lds, stream);
     else
-        launch_cv<R, N, false>(s, lds, stream);
+        launch_form<R, N, kFormGeneric>(s, lds, stream);
 }
 
 template <typename R>

@@ -207,8 +207,9 @@ _KNOWN = (BlackScholesModel, LocalVolModel, SLVModel, HestonModel)
 
 
 def unsupported(model) -> str | None:
-    """Why the engine cannot simulate this model or basket, or None (native/__init__.py:190-212)."""
-    from optlab.structured.localcorr import LocalCorrelation
+    """Why the engine cannot simulate this model or basket, or None (native/__init__.py:190-212). An addition: the
+    stochastic mixing weight (`StochasticMix`, localcorr.py:87-108), which the reference engine refuses, runs here."""
+    from optlab.structured.localcorr import LocalCorrelation, StochasticMix
     from optlab.structured.multiasset import Basket
 
     if isinstance(model, Basket):
@@ -217,8 +218,8 @@ def unsupported(model) -> str | None:
         bad = [n for n, m in model.models.items() if not isinstance(m, _KNOWN)]
         if bad:
             return f"{', '.join(type(model.models[n]).__name__ for n in bad)} not in the path engine"
-        if model.mixer is not None and not isinstance(model.mixer, LocalCorrelation):
-            return f"{type(model.mixer).__name__} is not in the path engine (local correlation only)"
+        if model.mixer is not None and not isinstance(model.mixer, (LocalCorrelation, StochasticMix)):
+            return f"{type(model.mixer).__name__} is not in the path engine (local correlation and StochasticMix only)"
         return None
     if not isinstance(model, _KNOWN):
         return f"{type(model).__name__} is not in the path engine"
@@ -293,9 +294,14 @@ def plan(model, market, times, dt_max=1 / 252, dt_min=1 / 2000, n_x: int = N_X) 
     if isinstance(model, Basket) and model.mixer is not None:
         from optlab.structured.localcorr import aligned_factors
 
+        from optlab.structured.localcorr import StochasticMix
+
         mx = model.mixer
         low, sign, turn = aligned_factors(model.corr, model.root, mx.w0)
-        mixer, mix = 1, (mx.w0, mx.level, mx.down, mx.up, mx.sigma, sign, np.cos(turn), np.sin(turn))
+        if isinstance(mx, StochasticMix):  # mixer 2: (w0, theta, nu, rho, kappa, market sign, cos, sin of the turn)
+            mixer, mix = 2, (mx.w0, mx.theta, mx.nu, mx.rho, mx.kappa, sign, np.cos(turn), np.sin(turn))
+        else:
+            mixer, mix = 1, (mx.w0, mx.level, mx.down, mx.up, mx.sigma, sign, np.cos(turn), np.sin(turn))
         low = np.ascontiguousarray(low, dtype=float)
     return Plan(
         names,

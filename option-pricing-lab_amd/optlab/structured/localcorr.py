@@ -1,4 +1,4 @@
-"""Local correlation for the path engine's mixer (packages/structured/src/optlab/structured/localcorr.py:37-84).
+"""Local correlation for the path engine's mixer (packages/structured/src/optlab/structured/localcorr.py:37-114).
 
 Z_i = sqrt(w) Z_0 + sqrt(1 - w) (L_low e)_i with w = clip(w0 + level + down max(-u, 0) - up max(u, 0), 0, 1) and
 u = y / (sigma sqrt(max(t, 1 month))), y = ln(mean S_i / F_i) at the start of the step. L_low factors the matrix that
@@ -49,10 +49,17 @@ class LocalCorrelation:
 
 @dataclass(frozen=True)
 class StochasticMix:
-    """A Jacobi-process weight: not in the path engine (the reference engine refuses it too)."""
+    """The weight as a Jacobi process (localcorr.py:87-114): dw = kappa (theta - w) dt + nu sqrt(w (1 - w)) dB,
+    clipped to [0, 1], with dB = rho (market factor) + sqrt(1 - rho^2) (the weight's own driver); the step's
+    normals are mixed with the weight at the step's start. In the path engine as mixer 2 (the reference engine
+    refuses it); the driver is a stream of its own (tag 0x40000000 of the pair's Philox counter)."""
 
     theta: float = W0
     nu: float = 0.5
     rho: float = -0.5
     kappa: float = 2.0
     w0: float = W0
+
+    def bumped(self, slope: float) -> StochasticMix:
+        """A steeper correlation skew for a positive slope: rho moved down by it, within [-0.99, 0.99]."""
+        return replace(self, rho=float(np.clip(self.rho - slope, -0.99, 0.99)))

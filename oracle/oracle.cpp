@@ -152,25 +152,33 @@ void pair_paths(const Spec& s, long long p) {
     Extremes ext;
     int j = 0;
     while (j < r.n_obs && r.obs_step[j] == 0) observe(s, p, j++, x[0], x[1], ext);
-    double zs[kNames], zr[kNames], fresh = 0.0;
+    double zs[kNames], zr[kNames], fresh = 0.0, drive = 0.0;
     Draws main{pair, s.k0, s.k1};
     Draws aux{pair, s.k0, s.k1, kAuxTag};
+    Draws drv{pair, s.k0, s.k1, kMixTag};
+    double wst[2] = {r.mix[0], r.mix[0]};  // the stochastic weight of each leg (mixer 2)
     for (int i = 1; i <= r.n_steps; ++i) {
         const double dt = r.dt[i - 1], t0 = r.t0[i - 1];
         for (int a = 0; a < n; ++a) zs[a] = main.next();
         if (has_var)
             for (int a = 0; a < n; ++a) zr[a] = has_slv ? aux.next() : main.next();
         if (r.mixer) fresh = main.next();
+        if (r.mixer == 2) drive = drv.next();
         for (int leg = 0; leg < 2; ++leg) {
             const double sign = leg ? -1.0 : 1.0;
             double z[kNames], w[kNames];
             if (r.mixer) {
-                double m = 0.0;
-                for (int a = 0; a < n; ++a) m += std::exp(x[leg][a]);
-                const double y = std::log(m / static_cast<double>(n));
-                const double u = y / (r.mix[4] * std::sqrt(dmax(t0, 1.0 / 12)));
-                double wt = r.mix[0] + r.mix[1] + r.mix[2] * dmax(-u, 0.0) - r.mix[3] * dmax(u, 0.0);
-                wt = dmin(dmax(wt, 0.0), 1.0);
+                double wt;
+                if (r.mixer == 2) {  // localcorr.py:87-105: the weight is the state
+                    wt = wst[leg];
+                } else {
+                    double m = 0.0;
+                    for (int a = 0; a < n; ++a) m += std::exp(x[leg][a]);
+                    const double y = std::log(m / static_cast<double>(n));
+                    const double u = y / (r.mix[4] * std::sqrt(dmax(t0, 1.0 / 12)));
+                    wt = r.mix[0] + r.mix[1] + r.mix[2] * dmax(-u, 0.0) - r.mix[3] * dmax(u, 0.0);
+                    wt = dmin(dmax(wt, 0.0), 1.0);
+                }
                 const double market = r.mix[5] * sign * zs[n - 1], fr = sign * fresh;
                 const double common = r.mix[6] * market + r.mix[7] * fr;
                 double other[kNames];
@@ -181,6 +189,16 @@ void pair_paths(const Spec& s, long long p) {
                     double acc = 0.0;
                     for (int b = 0; b < n; ++b) acc += r.low_root[a * n + b] * other[b];
                     z[a] = sw * common + sl * acc;
+                }
+                if (r.mixer == 2) {
+                    // localcorr.py:99-108 (advance), called after the step's normals are mixed
+                    // (multiasset.py:122-124): z_B = rho market + sqrt(1 - rho^2) e, e the driver's (antithetic)
+                    // normal; w += kappa (theta - w) dt + nu sqrt(max(w (1 - w), 0) dt) z_B, clipped to [0, 1].
+                    const double rho = r.mix[3];
+                    const double zb = rho * market + std::sqrt(1.0 - rho * rho) * (sign * drive);
+                    const double w =
+                        wt + r.mix[4] * (r.mix[1] - wt) * dt + r.mix[2] * std::sqrt(dmax(wt * (1.0 - wt), 0.0) * dt) * zb;
+                    wst[leg] = dmin(dmax(w, 0.0), 1.0);
                 }
             } else {
                 for (int a = 0; a < n; ++a) {
@@ -254,7 +272,10 @@ void check_paths(const optlab_paths_request& c) {
         if (c.obs_step[j] < prev || c.obs_step[j] > c.n_steps) bad("obs_step must be nondecreasing in [0, n_steps]");
         prev = c.obs_step[j];
     }
-    if (c.mixer && (c.mixer != 1 || !c.low_root || !(c.mix[4] > 0))) bad("bad local correlation");
+    if (c.mixer < 0 || c.mixer > 2 || (c.mixer == 1 && (!c.low_root || !(c.mix[4] > 0)))) bad("bad local correlation");
+    if (c.mixer == 2 && (!c.low_root || !(c.mix[0] >= 0 && c.mix[0] <= 1) || !(c.mix[1] >= 0 && c.mix[1] <= 1) ||
+                         !(c.mix[2] >= 0) || !(std::fabs(c.mix[3]) <= 1) || !(c.mix[4] >= 0)))
+        bad("bad stochastic mix");
     if (c.n_windows < 0 || c.n_windows > kWindows || (c.n_windows && (!c.win_lo || !c.win_hi)))
         bad("n_windows must be 0 to 2, with win_lo and win_hi");
     if (c.n_obs && !c.col_of) bad("null col_of");

@@ -47,6 +47,9 @@ inline double unit(std::uint32_t i) { return static_cast<double>(i) * 2.32830643
 void gauss_pair(std::uint32_t a, std::uint32_t b, double& z0, double& z1);
 
 constexpr std::uint32_t kAuxTag = 0x80000000u;  // paths.hpp:209
+// The stochastic mixing weight's driver (no counterpart in the reference engine, which refuses that mixer): a
+// third stream of the pair, so that the spot normals are those of the constant matrix whatever the weight does.
+constexpr std::uint32_t kMixTag = 0x40000000u;
 
 // paths.hpp:213-233: the normals of one pair on one stream, four per Philox block, none wasted.
 struct Draws {

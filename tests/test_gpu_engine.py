@@ -74,10 +74,14 @@ def _models():
     market, models = _golden_models()
     b, _ = _basket()
     bl, _ = _basket(mixer=True, slv=True)
-    return market, {**models, "basket": b, "basket_slv_localcorr": bl}
+    from optlab.structured.localcorr import StochasticMix
+    from optlab.structured.multiasset import Basket
+
+    bs = Basket(bl.names, bl.models, bl.markets, bl.corr, bl.vol_corr, StochasticMix(0.5, 1.0, -0.8, 2.0, 0.3))
+    return market, {**models, "basket": b, "basket_slv_localcorr": bl, "basket_slv_stochmix": bs}
 
 
-CASES = ["bs", "lv", "heston", "basket", "basket_slv_localcorr"]
+CASES = ["bs", "lv", "heston", "basket", "basket_slv_localcorr", "basket_slv_stochmix"]
 
 
 @pytest.mark.parametrize("case", CASES)

@@ -220,10 +220,18 @@ def test_unsupported_models_are_named():
     from optlab.structured.localcorr import StochasticMix
     from optlab.structured.multiasset import Basket
 
+    class OtherMixer:
+        w0 = 0.3
+
     market, surface, heston = simulated_context()
-    b = Basket(("A", "B"), {"A": HestonModel(heston), "B": BlackScholesModel(0.2)}, {"A": market, "B": market},
-               np.eye(2), mixer=StochasticMix())
-    assert native.unsupported(b) == "StochasticMix is not in the path engine (local correlation only)"
+    pair = ("A", "B"), {"A": HestonModel(heston), "B": BlackScholesModel(0.2)}, {"A": market, "B": market}
+    b = Basket(*pair, np.eye(2), mixer=OtherMixer())
+    assert native.unsupported(b) == "OtherMixer is not in the path engine (local correlation and StochasticMix only)"
     assert native.unsupported(object()) == "object is not in the path engine"
-    with pytest.raises(engines.EngineError, match="StochasticMix"):
+    with pytest.raises(engines.EngineError, match="OtherMixer"):
         engines.choose("cpu", b)
+    # an addition: the stochastic weight runs in this engine (the reference's refuses it)
+    s = Basket(*pair, np.array([[1.0, 0.5], [0.5, 1.0]]), mixer=StochasticMix(0.5, 1.0, -0.8))
+    assert native.unsupported(s) is None
+    p = native.plan(s, None, [0.5, 1.0])
+    assert p.mixer == 2 and p.mix[:5] == (0.3, 0.5, 1.0, -0.8, 2.0) and p.low_root.shape == (2, 2)
