@@ -2,17 +2,19 @@
 // Semantics: packages/native/cuda/src/leverage.cpp:17-186 (the host algorithm and the CPU particles) and
 // include/optlab_cuda/leverage.hpp:13-53.
 //
-// Per step i, four kernels on one stream, no host round trip (the reference copies the bins to the host and the
+// Per step i, five kernels on one stream, no host round trip (the reference copies the bins to the host and the
 // leverage row back, four synchronous copies per step):
 //   bin   -- every block first reduces the previous step's per-block ranges (min x, max x, max |v|) in a fixed
 //            order, then bins its particles in LDS: a fine histogram (u32) for the quantiles and, on the regression
 //            bins, count and four fixed-point sums (v, d, d^2, d v with d = (x - centre) / width in [-1/2, 1/2]),
 //            added with 64-bit integer LDS atomics, so the totals do not depend on the order of the additions;
 //   reduce-- adds the per-block bins in a fixed order;
-//   row   -- one workgroup: quantiles by a block scan of the fine histogram, the 81-point grid, the Gaussian
-//            Nadaraya-Watson (or local-linear) regression of E[v | x] on the bin centres in FP64 (one wave per grid
-//            point, fixed-order wave reductions), the FP64 division L = clip(sqrt(sigma_loc^2 / E[v | x])), and the
-//            leverage row for the step kernel;
+//   prep  -- one workgroup: quantiles by a block scan of the fine histogram, the 81-point grid, the bandwidth
+//            and each regression bin's moments;
+//   fit   -- one wave per grid point, across the chip: the Gaussian Nadaraya-Watson (or local-linear) regression
+//            of E[v | x] on the bin centres in FP64 (fixed-order wave reductions), the FP64 division
+//            L = clip(sqrt(sigma_loc^2 / E[v | x])), and the leverage row for the step kernel (the regression is
+//            most of a step's FP64 work: on one CU it was the longest kernel of the step);
 //   step  -- one pair per lane, the leverage row staged in LDS, one Euler step of both legs (Philox counter
 //            (pair, i + 1, 0), leverage.cpp:197-208), block min / max / max |v| for the next bin.
 // Particles are FP32 (backend 1) or FP64 (backend 0); everything after the bins is FP64.
@@ -32,6 +34,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kRowThreads = 1024;
+constexpr int kFitThreads = 256;  // four grid points a workgroup
 constexpr int kMaxFine = 8192;
 constexpr int kMaxBins = 512;
 constexpr int kMaxX = 1024;
@@ -40,6 +43,7 @@ struct Shared {  // device-resident state of the calibration between kernels
     double lo, hi, vmax;  // range of x and max |v| of the particles before this step's binning
     double a, b;          // ends of the current leverage row's grid
     int shift_d, shift_v;
+    double bw, fallback;  // this step's kernel bandwidth and E[v] (the regression's fallback)
 };
 
 struct LevArgs {
@@ -50,6 +54,7 @@ struct LevArgs {
     double atm_vol, bandwidth_scale, l_min, l_max, q_lo, q_hi;
     int bw_rule;     // 0: the reference's bandwidth; 1: bw_coef std(x) N^-0.2 over the particles
     double bw_coef;
+    double n_pow;    // N^-0.2, N the number of particles (formed on the host, as leverage.cpp:63 does)
     const double* dt;
     const double* t0;
     const double* lv_lo;
@@ -65,6 +70,8 @@ struct LevArgs {
     int n_bin_blocks;
     unsigned long long* fine_tot;  // [n_fine]
     long long* sum_tot;            // [5 n_bins]
+    double* x_eval;                // [n_x]: this step's grid
+    double* bins;                  // [6][n_bins]: centre, count, sum v, sum x, sum x^2, sum x v of each bin
 };
 
 template <typename R>
@@ -80,26 +87,34 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Reduces the step kernel's per-block (min x, max x, max |v|) in a fixed order.
+constexpr int kBinThreads = 1024;  // four waves a SIMD: the LDS atomics' and loads' latencies overlap
+
+// Reduces the step kernel's per-block (min x, max x, max |v|): minima and maxima, the same in any order (wave
+// butterflies, then the waves' results in LDS).
+template <int T>
 __device__ void reduce_range(const LevArgs& s, double& lo, double& hi, double& vmax) {
-    __shared__ double rmin[kBlock], rmax[kBlock], rv[kBlock];
+    __shared__ double red[3][T / 64];
     double mn = INFINITY, mx = -INFINITY, mv = 0.0;
     for (int b = threadIdx.x; b < s.n_step_blocks; b += blockDim.x) {
         mn = fmin(mn, s.range_part[3 * b]);
         mx = fmax(mx, s.range_part[3 * b + 1]);
         mv = fmax(mv, s.range_part[3 * b + 2]);
     }
-    rmin[threadIdx.x] = mn, rmax[threadIdx.x] = mx, rv[threadIdx.x] = mv;
-    __syncthreads();
-    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
-        if (static_cast<int>(threadIdx.x) < h) {
-            rmin[threadIdx.x] = fmin(rmin[threadIdx.x], rmin[threadIdx.x + h]);
-            rmax[threadIdx.x] = fmax(rmax[threadIdx.x], rmax[threadIdx.x + h]);
-            rv[threadIdx.x] = fmax(rv[threadIdx.x], rv[threadIdx.x + h]);
-        }
-        __syncthreads();
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        mn = fmin(mn, __shfl_xor(mn, m, 64));
+        mx = fmax(mx, __shfl_xor(mx, m, 64));
+        mv = fmax(mv, __shfl_xor(mv, m, 64));
     }
-    lo = rmin[0], hi = rmax[0], vmax = rv[0];
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[0][wave] = mn, red[1][wave] = mx, red[2][wave] = mv;
+    __syncthreads();
+    for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) {
+        mn = fmin(mn, red[0][w]);
+        mx = fmax(mx, red[1][w]);
+        mv = fmax(mv, red[2][w]);
+    }
+    lo = mn, hi = mx, vmax = mv;
     __syncthreads();
 }
 
@@ -110,11 +125,11 @@ __device__ __forceinline__ int shift_bits(long long n_particles) {
 }
 
 template <typename R>
-__global__ void __launch_bounds__(kBlock) bin_kernel(LevArgs s, Particles<R> P) {
+__global__ void __launch_bounds__(kBinThreads) bin_kernel(LevArgs s, Particles<R> P) {
     __shared__ unsigned fine[kMaxFine];
     __shared__ long long sums[5 * kMaxBins];
     double lo, hi, vmax;
-    reduce_range(s, lo, hi, vmax);
+    reduce_range<kBinThreads>(s, lo, hi, vmax);
     const int nf = s.n_fine, nb = s.n_bins;
     const long long n = 2 * s.n_pairs;
     // fixed-point scales: n |d| < 2^62 with |d| <= 1/2 + eps, and n max|v| < 2^61
@@ -165,11 +180,13 @@ __global__ void __launch_bounds__(kBlock) bin_kernel(LevArgs s, Particles<R> P) 
     for (int k = threadIdx.x; k < 5 * nb; k += blockDim.x) sp[k] = sums[k];
 }
 
-// Totals of the workgroups' bins: a workgroup takes 64 slots, its 4 waves each add every fourth workgroup's row
-// (loads overlapped: integer sums give the same bits in any order), then wave 0 adds the 4 partials.
-__global__ void __launch_bounds__(kBlock) reduce_bins_kernel(LevArgs s) {
-    __shared__ unsigned long long part[kBlock / 64][64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = kBlock / 64;
+// Totals of the workgroups' bins: a workgroup takes 64 slots, its 16 waves each add every sixteenth workgroup's row
+// (a short chain of loads a wave; integer sums give the same bits in any order), then wave 0 adds the partials.
+constexpr int kReduceThreads = 1024;
+
+__global__ void __launch_bounds__(kReduceThreads) reduce_bins_kernel(LevArgs s) {
+    __shared__ unsigned long long part[kReduceThreads / 64][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, n_waves = kReduceThreads / 64;
     const int slot = blockIdx.x * 64 + lane;
     const int nf = s.n_fine, ns = 5 * s.n_bins;
     unsigned long long t = 0;
@@ -203,153 +220,180 @@ __device__ __forceinline__ double interp_uniform(double x, double a, double b, c
     return y[i] * (1 - w) + y[i + 1] * w;
 }
 
-// leverage.cpp:97-148: the leverage row of step i from the totals of the bins (or the flat start at step 0).
+// leverage.cpp:26-38, 56-66: the grid of step i from the quantiles of the fine histogram (or the flat start at
+// step 0), the bandwidth, E[v] and each regression bin's moments, for the fit kernel.
 template <typename R>
-__global__ void __launch_bounds__(kRowThreads) row_kernel(LevArgs s, Particles<R> P, int i) {
-    __shared__ double x_eval[kMaxX], cond[kMaxX];
+__global__ void __launch_bounds__(kRowThreads) prep_kernel(LevArgs s, int i) {
+    constexpr int kPer = kMaxFine / kRowThreads;  // fine bins a thread owns, at most
     __shared__ double scan[kRowThreads];
     __shared__ double qval[2];
     __shared__ int first_hit[2];
     const int nx = s.n_x, nb = s.n_bins, nf = s.n_fine;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, n_waves = blockDim.x >> 6;
     const double n_particles = 2.0 * static_cast<double>(s.n_pairs);
-    const double t0 = s.t0[i], h = s.dt[i];
+    const double h = s.dt[i];
     if (i == 0) {
-        for (int g = tid; g < nx; g += blockDim.x) {
-            x_eval[g] = (-6.0 + 12.0 * g / (nx - 1)) * s.atm_vol * sqrt(fmax(h, 1e-4));
-            cond[g] = s.v0;
+        for (int g = tid; g < nx; g += blockDim.x) s.x_eval[g] = (-6.0 + 12.0 * g / (nx - 1)) * s.atm_vol * sqrt(fmax(h, 1e-4));
+        if (tid == 0) {
+            s.shared->a = (-6.0) * s.atm_vol * sqrt(fmax(h, 1e-4));
+            s.shared->b = (-6.0 + 12.0 * (nx - 1) / (nx - 1)) * s.atm_vol * sqrt(fmax(h, 1e-4));
         }
-        __syncthreads();
-    } else {
-        const double lo = s.shared->lo, hi = s.shared->hi;
-        // leverage.cpp:26-38 by a block scan: each thread owns a contiguous run of fine bins; the cumulative counts
-        // are integers below 2^53, so every grouping of the additions gives the reference's sequential values.
-        const int per = (nf + blockDim.x - 1) / blockDim.x;
-        const int k0 = min(nf, tid * per), k1 = min(nf, k0 + per);
-        double run = 0.0;
-        for (int k = k0; k < k1; ++k) run += static_cast<double>(s.fine_tot[k]);
-        scan[tid] = run;
-        if (tid < 2) first_hit[tid] = INT_MAX;
-        __syncthreads();
-        for (int off = 1; off < static_cast<int>(blockDim.x); off <<= 1) {
-            const double add = tid >= off ? scan[tid - off] : 0.0;
-            __syncthreads();
-            scan[tid] += add;
-            __syncthreads();
-        }
-        const double start = tid > 0 ? scan[tid - 1] : 0.0;
-        const double total = scan[blockDim.x - 1];
-        const double target[2] = {s.q_lo * total, s.q_hi * total};
-        for (int q = 0; q < 2; ++q) {
-            double cum = start;
-            for (int k = k0; k < k1; ++k) {
-                const double c = static_cast<double>(s.fine_tot[k]);
-                if (cum + c >= target[q] && c > 0) {
-                    atomicMin(&first_hit[q], k);
-                    break;
-                }
+        return;
+    }
+    const double lo = s.shared->lo, hi = s.shared->hi, t0 = s.t0[i];
+    // leverage.cpp:26-38 by a block scan: each thread owns a contiguous run of fine bins; the cumulative counts are
+    // integers below 2^53, so every grouping of the additions gives the reference's sequential values.
+    const int per = (nf + blockDim.x - 1) / blockDim.x;  // <= kPer
+    const int k0 = min(nf, tid * per), k1 = min(nf, k0 + per);
+    double fc[kPer];  // this thread's fine counts, read once
+#pragma unroll
+    for (int m = 0; m < kPer; ++m) fc[m] = k0 + m < k1 ? static_cast<double>(s.fine_tot[k0 + m]) : 0.0;
+    double run = 0.0;
+#pragma unroll
+    for (int m = 0; m < kPer; ++m)
+        if (k0 + m < k1) run += fc[m];
+    // inclusive scan of the runs: in the wave by shuffles, then the waves' totals in order
+    double incl = run;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) scan[wave] = incl;
+    if (tid < 2) first_hit[tid] = INT_MAX;
+    __syncthreads();
+    double before = 0.0, total = 0.0;
+    for (int w = 0; w < n_waves; ++w) {
+        if (w < wave) before += scan[w];
+        total += scan[w];
+    }
+    const double start = before + incl - run;  // the counts before this thread's first fine bin
+    __syncthreads();
+    const double target[2] = {s.q_lo * total, s.q_hi * total};
+    for (int q = 0; q < 2; ++q) {
+        double cum = start;
+        bool hit = false;
+#pragma unroll
+        for (int m = 0; m < kPer; ++m) {
+            if (hit || k0 + m >= k1) continue;
+            const double c = fc[m];
+            if (cum + c >= target[q] && c > 0) {
+                atomicMin(&first_hit[q], k0 + m);
+                hit = true;
+            } else {
                 cum += c;
             }
         }
-        __syncthreads();
-        const double width = (hi - lo) / static_cast<double>(nf);
-        for (int q = 0; q < 2; ++q) {
-            const int k = first_hit[q];
-            if (k == INT_MAX) {
-                if (tid == 0) qval[q] = hi;
-            } else if (k >= k0 && k < k1) {
-                double cum = start;
-                for (int m = k0; m < k; ++m) cum += static_cast<double>(s.fine_tot[m]);
-                const double c = static_cast<double>(s.fine_tot[k]);
-                qval[q] = lo + (static_cast<double>(k) + (target[q] - cum) / c) * width;
-            }
-        }
-        __syncthreads();
-        const double qv[2] = {qval[0], qval[1]};
-        const double a = qv[0], b = qv[1];
-        for (int g = tid; g < nx; g += blockDim.x) x_eval[g] = a + (b - a) * g / (nx - 1);
-        // regression on the bin centres, one wave per grid point
-        const double we = (hi + 1e-12 - lo) / nb;
-        const double inv_d = ldexp(1.0, -s.shared->shift_d), inv_v = ldexp(1.0, -s.shared->shift_v);
-        double bw = s.bandwidth_scale * s.atm_vol * sqrt(fmax(t0, 0.15)) * pow(n_particles, -0.2);
-        if (s.bw_rule == 1) {
-            // std(x) over the particles from the bins: sum x = n c + w S_d, sum x^2 = n c^2 + 2 c w S_d + w^2 S_dd
-            double sx = 0.0, sxx = 0.0;
-            for (int k = tid; k < nb; k += blockDim.x) {
-                const double centre = lo + (k + 0.5) * we, cnt = static_cast<double>(s.sum_tot[k]);
-                const double sd = static_cast<double>(s.sum_tot[2 * nb + k]) * inv_d;
-                const double sdd = static_cast<double>(s.sum_tot[3 * nb + k]) * inv_d;
-                sx += cnt * centre + we * sd;
-                sxx += cnt * centre * centre + 2.0 * centre * we * sd + we * we * sdd;
-            }
-            sx = wave_sum(sx);
-            sxx = wave_sum(sxx);
-            if (lane == 0) scan[wave] = sx, scan[n_waves + wave] = sxx;
-            __syncthreads();
-            double tx = 0.0, txx = 0.0;
-            for (int w = 0; w < n_waves; ++w) tx += scan[w], txx += scan[n_waves + w];
-            __syncthreads();
-            const double mean = tx / n_particles, var = fmax(txx / n_particles - mean * mean, 0.0);
-            bw = s.bw_coef * sqrt(var) * pow(n_particles, -0.2);
-        }
-        double vs = 0.0;
-        for (int k = lane; k < nb; k += 64) vs += static_cast<double>(s.sum_tot[nb + k]) * inv_v;
-        const double fallback = wave_sum(vs) / n_particles;
-        __syncthreads();
-        for (int g = wave; g < nx; g += n_waves) {
-            const double x0 = x_eval[g];
-            double S0 = 0, T0 = 0, Ax = 0, Axx = 0, Axv = 0;
-            for (int k = lane; k < nb; k += 64) {
-                const double centre = lo + (k + 0.5) * we;
-                const double d = (x0 - centre) / bw, w = exp(-0.5 * d * d);
-                const double cnt = static_cast<double>(s.sum_tot[k]);
-                const double sv = static_cast<double>(s.sum_tot[nb + k]) * inv_v;
-                S0 += w * cnt;
-                T0 += w * sv;
-                if (s.local_linear) {
-                    const double sdd = static_cast<double>(s.sum_tot[2 * nb + k]) * inv_d;
-                    const double sd2 = static_cast<double>(s.sum_tot[3 * nb + k]) * inv_d;
-                    const double sdv = static_cast<double>(s.sum_tot[4 * nb + k]) * inv_v;
-                    const double sx = cnt * centre + we * sdd;
-                    const double sxx = cnt * centre * centre + 2.0 * centre * we * sdd + we * we * sd2;
-                    const double sxv = centre * sv + we * sdv;
-                    Ax += w * sx;
-                    Axx += w * sxx;
-                    Axv += w * sxv;
-                }
-            }
-            S0 = wave_sum(S0);
-            T0 = wave_sum(T0);
-            if (s.local_linear) {
-                Ax = wave_sum(Ax);
-                Axx = wave_sum(Axx);
-                Axv = wave_sum(Axv);
-            }
-            if (lane == 0) {
-                const double nw = S0 > 1e-8 ? T0 / fmax(S0, 1e-300) : fallback;
-                double c = nw;
-                if (s.local_linear) {
-                    const double S1 = Ax - x0 * S0, S2 = Axx - 2 * x0 * Ax + x0 * x0 * S0, T1 = Axv - x0 * T0;
-                    const double det = S0 * S2 - S1 * S1;
-                    const bool ok = S0 >= 50 && det > 1e-12 * S0 * fabs(S2);
-                    c = fmax(ok ? (S2 * T0 - S1 * T1) / det : nw, 1e-8);
-                }
-                cond[g] = c;
-            }
-        }
-        __syncthreads();
     }
-    // leverage.cpp:141-148
+    __syncthreads();
+    const double width = (hi - lo) / static_cast<double>(nf);
+    for (int q = 0; q < 2; ++q) {
+        const int k = first_hit[q];
+        if (k == INT_MAX) {
+            if (tid == 0) qval[q] = hi;
+        } else if (k >= k0 && k < k1) {
+            double cum = start, c = 0.0;
+#pragma unroll
+            for (int m = 0; m < kPer; ++m) {
+                if (k0 + m < k) cum += fc[m];
+                if (k0 + m == k) c = fc[m];
+            }
+            qval[q] = lo + (static_cast<double>(k) + (target[q] - cum) / c) * width;
+        }
+    }
+    __syncthreads();
+    const double a = qval[0], b = qval[1];
+    for (int g = tid; g < nx; g += blockDim.x) s.x_eval[g] = a + (b - a) * g / (nx - 1);
+    if (tid == 0) s.shared->a = a, s.shared->b = a + (b - a) * (nx - 1) / (nx - 1);
+    // each regression bin's moments from the fixed-point totals:
+    // sum x = n c + w S_d, sum x^2 = n c^2 + 2 c w S_d + w^2 S_dd, sum x v = c S_v + w S_dv
+    const double we = (hi + 1e-12 - lo) / nb;
+    const double inv_d = ldexp(1.0, -s.shared->shift_d), inv_v = ldexp(1.0, -s.shared->shift_v);
+    const bool moments = s.local_linear || s.bw_rule == 1;
+    double* B = s.bins;
+    double sx = 0.0, sxx = 0.0, vs = 0.0;
+    for (int k = tid; k < nb; k += blockDim.x) {
+        const double centre = lo + (k + 0.5) * we, cnt = static_cast<double>(s.sum_tot[k]);
+        const double sv = static_cast<double>(s.sum_tot[nb + k]) * inv_v;
+        B[k] = centre, B[nb + k] = cnt, B[2 * nb + k] = sv;
+        if (moments) {
+            const double sd = static_cast<double>(s.sum_tot[2 * nb + k]) * inv_d;
+            const double sdd = static_cast<double>(s.sum_tot[3 * nb + k]) * inv_d;
+            const double sdv = static_cast<double>(s.sum_tot[4 * nb + k]) * inv_v;
+            const double bx = cnt * centre + we * sd;
+            const double bxx = cnt * centre * centre + 2.0 * centre * we * sd + we * we * sdd;
+            B[3 * nb + k] = bx, B[4 * nb + k] = bxx, B[5 * nb + k] = centre * sv + we * sdv;
+            sx += bx;
+            sxx += bxx;
+        }
+    }
+    // E[v] (the fallback of an empty neighbourhood): wave 0, lanes over the bins in the fit's order
+    if (wave == 0) {
+        for (int k = lane; k < nb; k += 64) vs += static_cast<double>(s.sum_tot[nb + k]) * inv_v;
+        vs = wave_sum(vs);
+    }
+    double bw = s.bandwidth_scale * s.atm_vol * sqrt(fmax(t0, 0.15)) * s.n_pow;
+    if (s.bw_rule == 1) {  // std(x) over the particles from the bins
+        sx = wave_sum(sx);
+        sxx = wave_sum(sxx);
+        if (lane == 0) scan[wave] = sx, scan[n_waves + wave] = sxx;
+        __syncthreads();
+        double tx = 0.0, txx = 0.0;
+        for (int w = 0; w < n_waves; ++w) tx += scan[w], txx += scan[n_waves + w];
+        const double mean = tx / n_particles, var = fmax(txx / n_particles - mean * mean, 0.0);
+        bw = s.bw_coef * sqrt(var) * s.n_pow;
+    }
+    if (tid == 0) s.shared->bw = bw, s.shared->fallback = vs / n_particles;
+}
+
+// leverage.cpp:67-92, 141-148: one wave per grid point g of step i: the regression of E[v | x] at x_g over the
+// bins (lane l takes bins l, l + 64, ..., then a fixed butterfly), and the leverage there.
+template <typename R>
+__global__ void __launch_bounds__(kFitThreads) fit_kernel(LevArgs s, Particles<R> P, int i) {
+    const int lane = threadIdx.x & 63;
+    const int g = blockIdx.x * (kFitThreads / 64) + (threadIdx.x >> 6);
+    const int nx = s.n_x, nb = s.n_bins;
+    if (g >= nx) return;
+    const double x0 = s.x_eval[g];
+    double c = s.v0;
+    if (i > 0) {
+        const double bw = s.shared->bw;
+        const double* B = s.bins;
+        double S0 = 0, T0 = 0, Ax = 0, Axx = 0, Axv = 0;
+        for (int k = lane; k < nb; k += 64) {
+            const double d = (x0 - B[k]) / bw, w = exp(-0.5 * d * d);
+            S0 += w * B[nb + k];
+            T0 += w * B[2 * nb + k];
+            if (s.local_linear) {
+                Ax += w * B[3 * nb + k];
+                Axx += w * B[4 * nb + k];
+                Axv += w * B[5 * nb + k];
+            }
+        }
+        S0 = wave_sum(S0);
+        T0 = wave_sum(T0);
+        if (s.local_linear) {
+            Ax = wave_sum(Ax);
+            Axx = wave_sum(Axx);
+            Axv = wave_sum(Axv);
+        }
+        const double nw = S0 > 1e-8 ? T0 / fmax(S0, 1e-300) : s.shared->fallback;
+        c = nw;
+        if (s.local_linear) {
+            const double S1 = Ax - x0 * S0, S2 = Axx - 2 * x0 * Ax + x0 * x0 * S0, T1 = Axv - x0 * T0;
+            const double det = S0 * S2 - S1 * S1;
+            const bool ok = S0 >= 50 && det > 1e-12 * S0 * fabs(S2);
+            c = fmax(ok ? (S2 * T0 - S1 * T1) / det : nw, 1e-8);
+        }
+    }
+    if (lane != 0) return;
     const double* row = s.lv_rows + static_cast<long long>(i) * s.n_lv;
     const double lv_lo = s.lv_lo[i], lv_hi = lv_lo + (s.n_lv - 1) / s.lv_inv_dx[i];
-    for (int g = tid; g < nx; g += blockDim.x) {
-        const double loc = interp_uniform(x_eval[g], lv_lo, lv_hi, row, s.n_lv);
-        const double L = fmin(fmax(sqrt(loc / fmax(cond[g], 1e-8)), s.l_min), s.l_max);
-        s.x_grids[static_cast<long long>(i) * nx + g] = x_eval[g];
-        s.values[static_cast<long long>(i) * nx + g] = L;
-        P.lev[g] = static_cast<R>(L);
-    }
-    if (tid == 0) s.shared->a = x_eval[0], s.shared->b = x_eval[nx - 1];
+    const double loc = interp_uniform(x0, lv_lo, lv_hi, row, s.n_lv);
+    const double L = fmin(fmax(sqrt(loc / fmax(c, 1e-8)), s.l_min), s.l_max);
+    s.x_grids[static_cast<long long>(i) * nx + g] = x0;
+    s.values[static_cast<long long>(i) * nx + g] = L;
+    P.lev[g] = static_cast<R>(L);
 }
 
 // leverage.cpp:189-214: one Euler step of both legs of every pair on the leverage row.
@@ -447,8 +491,9 @@ void run_typed(optlab_leverage_request& rq, int bw_rule, double bw_coef) {
     s.atm_vol = rq.atm_vol, s.bandwidth_scale = rq.bandwidth_scale, s.l_min = rq.l_min, s.l_max = rq.l_max;
     s.q_lo = rq.q_lo, s.q_hi = rq.q_hi;
     s.bw_rule = bw_rule, s.bw_coef = bw_coef;
+    s.n_pow = std::pow(static_cast<double>(n), -0.2);
     s.n_step_blocks = static_cast<int>(std::max<long long>(1, std::min<long long>((np + kBlock - 1) / kBlock, 4LL * cus)));
-    s.n_bin_blocks = static_cast<int>(std::max<long long>(1, std::min<long long>((n + kBlock - 1) / kBlock, cus)));
+    s.n_bin_blocks = static_cast<int>(std::max<long long>(1, std::min<long long>((n + kBinThreads - 1) / kBinThreads, cus)));
     const size_t rows = static_cast<size_t>(rq.n_steps);
     // one device block of memory for everything, carved by offsets
     size_t off = 0;
@@ -465,6 +510,7 @@ void run_typed(optlab_leverage_request& rq, int bw_rule, double bw_coef) {
     const size_t o_ft = carve(static_cast<size_t>(nf) * 8), o_st = carve(static_cast<size_t>(5 * nb) * 8);
     const size_t o_x = carve(static_cast<size_t>(n) * sizeof(R)), o_v = carve(static_cast<size_t>(n) * sizeof(R));
     const size_t o_lev = carve(static_cast<size_t>(nx) * sizeof(R));
+    const size_t o_xe = carve(static_cast<size_t>(nx) * 8), o_bins = carve(static_cast<size_t>(6 * nb) * 8);
     unsigned char* base = static_cast<unsigned char*>(scratch(kSlotParticles, off));
     hip_check(hipMemcpy(base + o_dt, rq.dt, rows * 8, hipMemcpyHostToDevice), "copy dt");
     hip_check(hipMemcpy(base + o_t0, rq.t0, rows * 8, hipMemcpyHostToDevice), "copy t0");
@@ -478,6 +524,7 @@ void run_typed(optlab_leverage_request& rq, int bw_rule, double bw_coef) {
     s.shared = reinterpret_cast<Shared*>(base + o_sh), s.range_part = reinterpret_cast<double*>(base + o_rp);
     s.fine_part = reinterpret_cast<unsigned*>(base + o_fp), s.sum_part = reinterpret_cast<long long*>(base + o_spart);
     s.fine_tot = reinterpret_cast<unsigned long long*>(base + o_ft), s.sum_tot = reinterpret_cast<long long*>(base + o_st);
+    s.x_eval = reinterpret_cast<double*>(base + o_xe), s.bins = reinterpret_cast<double*>(base + o_bins);
     Particles<R> P{reinterpret_cast<R*>(base + o_x), reinterpret_cast<R*>(base + o_v), reinterpret_cast<R*>(base + o_lev)};
 
     // The 4 kernels of every step, 4 n_steps launches that depend each on the last, are captured in one HIP graph
@@ -487,16 +534,18 @@ void run_typed(optlab_leverage_request& rq, int bw_rule, double bw_coef) {
     hipLaunchKernelGGL(init_kernel<R>, dim3(init_grid), dim3(kBlock), 0, st, P, n, static_cast<R>(rq.v0));
     hip_check(hipGetLastError(), "init kernel launch");
     const unsigned red_grid = static_cast<unsigned>((nf + 5 * nb + 63) / 64);  // 64 slots a workgroup
+    const unsigned fit_grid = static_cast<unsigned>((nx + kFitThreads / 64 - 1) / (kFitThreads / 64));
     hip_check(hipStreamSynchronize(st), "synchronize");
     hipGraph_t graph;
     hipGraphExec_t exec;
     hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
     for (int i = 0; i < rq.n_steps; ++i) {
         if (i > 0) {
-            hipLaunchKernelGGL(bin_kernel<R>, dim3(s.n_bin_blocks), dim3(kBlock), 0, st, s, P);
-            hipLaunchKernelGGL(reduce_bins_kernel, dim3(red_grid), dim3(kBlock), 0, st, s);
+            hipLaunchKernelGGL(bin_kernel<R>, dim3(s.n_bin_blocks), dim3(kBinThreads), 0, st, s, P);
+            hipLaunchKernelGGL(reduce_bins_kernel, dim3(red_grid), dim3(kReduceThreads), 0, st, s);
         }
-        hipLaunchKernelGGL(row_kernel<R>, dim3(1), dim3(kRowThreads), 0, st, s, P, i);
+        hipLaunchKernelGGL(prep_kernel<R>, dim3(1), dim3(kRowThreads), 0, st, s, i);
+        hipLaunchKernelGGL(fit_kernel<R>, dim3(fit_grid), dim3(kFitThreads), 0, st, s, P, i);
         hipLaunchKernelGGL(step_kernel<R>, dim3(s.n_step_blocks), dim3(kBlock), 0, st, s, P, i);
     }
     hip_check(hipStreamEndCapture(st, &graph), "end capture");
