@@ -39,6 +39,7 @@ enum Slot {
     kSlotBook,         // book: the tables of every (scenario, name) and the book's small arrays
     kSlotBookQueue,    // book: the work queue, trades, per-item sums and clocks
     kSlotLsm,          // Longstaff-Schwartz: spots and variances at the dates, values, partial sums, coefficients
+    kSlotChunks,       // fixed products and the strip: compensated sums of chunks of the per-block rows
     kSlotCount
 };
 

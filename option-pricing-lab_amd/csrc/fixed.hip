@@ -13,7 +13,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <string>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -476,28 +478,62 @@ __global__ void __launch_bounds__(kBlock) strip_kernel_f64(StripArgs<double> s, 
     strip_body<double, MODEL>(s, n_pairs, k0, k1, partials);
 }
 
-// totals[slot] = sum over rows r of partials[r n_slots + slot], in a fixed order. Block g takes slots
-// g 64 .. g 64 + 63; wave w of the block adds rows w, w + 4, ... reading 64 consecutive doubles per row (one
-// coalesced 512 B load), then the four waves are added in order.
-__global__ void __launch_bounds__(256) reduce_kernel(const double* partials, long long n_rows, int n_slots,
-                                                      double* totals) {
+// The per-block rows (n_rows x n_slots) reduced to totals in two fixed-order stages, Neumaier-compensated:
+// stage 1, one workgroup per (group of 64 slots, chunk of kChunkRows rows): wave w adds rows w, w + 16, ... of the
+// chunk, then the 16 waves are combined in order; stage 2, one thread per slot combines the chunks in order. A
+// result depends only on the rows (the strip writes 2048 rows of 512 slots: one chain of 512 rows a wave took
+// 163 us, the chunks take a few).
+constexpr int kChunkRows = 64;
+constexpr int kChunkThreads = 1024;
+
+__global__ void __launch_bounds__(kChunkThreads) reduce_chunks_kernel(const double* partials, long long n_rows,
+                                                                       int n_slots, double* chunks) {
+    constexpr int W = kChunkThreads / 64;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int slot = blockIdx.x * 64 + lane;
+    const long long r0 = static_cast<long long>(blockIdx.y) * kChunkRows;
+    const long long r1 = r0 + kChunkRows < n_rows ? r0 + kChunkRows : n_rows;
     Comp acc{0.0, 0.0};
     if (slot < n_slots)
-        for (long long r = w; r < n_rows; r += 4) acc.add(partials[r * n_slots + slot]);
-    __shared__ double hi[4][64], lo[4][64];
+        for (long long r = r0 + w; r < r1; r += W) acc.add(partials[r * n_slots + slot]);
+    __shared__ double hi[W][64], lo[W][64];
     hi[w][lane] = acc.s;
     lo[w][lane] = acc.c;
     __syncthreads();
     if (w == 0 && slot < n_slots) {
         Comp a{hi[0][lane], lo[0][lane]};
-        for (int k = 1; k < 4; ++k) {
+        for (int k = 1; k < W; ++k) {
             a.add(hi[k][lane]);
             a.c += lo[k][lane];
         }
-        totals[slot] = a.value();
+        const long long at = (static_cast<long long>(blockIdx.y) * n_slots + slot) * 2;
+        chunks[at] = a.s;
+        chunks[at + 1] = a.c;
     }
+}
+
+__global__ void __launch_bounds__(256) reduce_totals_kernel(const double* chunks, int n_chunks, int n_slots,
+                                                            double* totals) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= n_slots) return;
+    Comp a{chunks[2 * slot], chunks[2 * slot + 1]};
+#pragma unroll 8
+    for (int c = 1; c < n_chunks; ++c) {
+        const long long at = (static_cast<long long>(c) * n_slots + slot) * 2;
+        a.add(chunks[at]);
+        a.c += chunks[at + 1];
+    }
+    totals[slot] = a.value();
+}
+
+void reduce_rows(const double* partials, long long n_rows, int n_slots, double* totals, hipStream_t st) {
+    const int n_chunks = static_cast<int>((n_rows + kChunkRows - 1) / kChunkRows);
+    double* chunks = static_cast<double*>(scratch(kSlotChunks, static_cast<size_t>(n_chunks) * n_slots * 2 * sizeof(double)));
+    hipLaunchKernelGGL(reduce_chunks_kernel, dim3((n_slots + 63) / 64, n_chunks), dim3(kChunkThreads), 0, st, partials,
+                       n_rows, n_slots, chunks);
+    hipLaunchKernelGGL(reduce_totals_kernel, dim3((n_slots + 255) / 256), dim3(256), 0, st, chunks, n_chunks, n_slots,
+                       totals);
+    hip_check(hipGetLastError(), "reduce kernel launch");
 }
 
 void estimate(double sum, double sumsq, long long n_pairs, double& price, double& se) {  // cpu_mc.cpp:14-18
@@ -529,11 +565,39 @@ double geometric_asian(double S, double K, double T, double r, double q, double 
 // the uneven wave times of QE-M's branches (measured: one resident round with two pairs per lane is 4 % slower).
 template <typename K>
 unsigned resident_grid(K kernel, size_t lds, long long n_pairs) {
-    int per_cu = 0;
-    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, lds), "occupancy");
+    // the occupancy of (kernel, LDS bytes) on a device does not change: asked once
+    static std::map<std::tuple<const void*, size_t, int>, int> known;
+    int device = 0;
+    hip_check(hipGetDevice(&device), "hipGetDevice");
+    const auto key = std::make_tuple(reinterpret_cast<const void*>(kernel), lds, device);
+    auto it = known.find(key);
+    if (it == known.end()) {
+        int per_cu = 0;
+        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, lds), "occupancy");
+        it = known.emplace(key, per_cu).first;
+    }
+    const int per_cu = it->second;
     const long long want = (n_pairs + kBlock - 1) / kBlock;
     const long long cap = static_cast<long long>(device_cu_count()) * std::max(per_cu, 1) * 8;
     return static_cast<unsigned>(std::max<long long>(1, std::min(want, cap)));
+}
+
+// Pinned host memory for a call's totals (per device, grown on demand): the totals come back by an asynchronous
+// copy on the call's stream, so the one wait at the end of the call covers the kernels and the copy.
+double* pinned_totals(size_t n) {
+    static std::vector<std::pair<double*, size_t>> bufs;  // per device
+    int device = 0;
+    hip_check(hipGetDevice(&device), "hipGetDevice");
+    if (static_cast<int>(bufs.size()) <= device) bufs.resize(device + 1, {nullptr, 0});
+    auto& b = bufs[device];
+    if (b.second < n) {
+        if (b.first) hip_check(hipHostFree(b.first), "free pinned totals");
+        b.first = nullptr;
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&b.first), n * sizeof(double), hipHostMallocDefault),
+                  "pinned totals");
+        b.second = n;
+    }
+    return b.first;
 }
 
 // HIP events around the main kernel of a call (mark() after it); stop() waits for the stream and returns the
@@ -603,7 +667,7 @@ size_t strip_lds_bytes(int n_mats) {
 // the timer's events, then the fixed-order reduction. Returns the main kernel's time in ms.
 template <typename R>
 double price_on_device(const HostConsts& hc, int model, int product, long long n_pairs, std::uint64_t seed,
-                       double* totals) {
+                       double* totals, double* host_totals) {
     const Consts<R> c = hc.as<R>();
     const std::uint32_t k0 = static_cast<std::uint32_t>(seed), k1 = static_cast<std::uint32_t>(seed >> 32);
     unsigned grid = 1;
@@ -615,13 +679,14 @@ double price_on_device(const HostConsts& hc, int model, int product, long long n
     });
     hip_check(hipGetLastError(), "fixed kernel launch");
     timer.mark();
-    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, nullptr, partials, static_cast<long long>(grid), 16, totals);
-    hip_check(hipGetLastError(), "reduce kernel launch");
+    reduce_rows(partials, static_cast<long long>(grid), 16, totals, nullptr);
+    hip_check(hipMemcpyAsync(host_totals, totals, 16 * sizeof(double), hipMemcpyDeviceToHost, nullptr), "copy totals");
     return timer.stop();
 }
 
 template <typename R>
-double strip_on_device(const StripArgs<R>& s, long long n_pairs, std::uint64_t seed, double* totals) {
+double strip_on_device(const StripArgs<R>& s, long long n_pairs, std::uint64_t seed, double* totals,
+                       double* host_totals) {
     const std::uint32_t k0 = static_cast<std::uint32_t>(seed), k1 = static_cast<std::uint32_t>(seed >> 32);
     const size_t lds = strip_lds_bytes<R>(s.n_mats);
     const int n_slots = s.n_mats * 64;
@@ -641,9 +706,10 @@ double strip_on_device(const StripArgs<R>& s, long long n_pairs, std::uint64_t s
     });
     hip_check(hipGetLastError(), "strip kernel launch");
     timer.mark();
-    hipLaunchKernelGGL(reduce_kernel, dim3((n_slots + 63) / 64), dim3(256), 0, nullptr, partials,
-                       static_cast<long long>(grid), n_slots, totals);
-    hip_check(hipGetLastError(), "reduce kernel launch");
+    reduce_rows(partials, static_cast<long long>(grid), n_slots, totals, nullptr);
+    hip_check(hipMemcpyAsync(host_totals, totals, static_cast<size_t>(n_slots) * sizeof(double), hipMemcpyDeviceToHost,
+                             nullptr),
+              "copy totals");
     return timer.stop();
 }
 
@@ -713,10 +779,11 @@ void run_price(const optlab_request& c, int prec, optlab_result& out) {
                                       c.H, c.legs, c.strikes, 1.5);
     const long long n_pairs = c.n_paths / 2;
     double* totals = static_cast<double*>(scratch(kSlotTotals, 16 * sizeof(double)));
-    const double kernel_ms = prec == 0 ? price_on_device<double>(hc, c.model, c.product, n_pairs, c.seed, totals)
-                                       : price_on_device<float>(hc, c.model, c.product, n_pairs, c.seed, totals);
+    double* host = pinned_totals(16);
+    const double kernel_ms = prec == 0 ? price_on_device<double>(hc, c.model, c.product, n_pairs, c.seed, totals, host)
+                                       : price_on_device<float>(hc, c.model, c.product, n_pairs, c.seed, totals, host);
     double t[16];
-    hip_check(hipMemcpy(t, totals, sizeof t, hipMemcpyDeviceToHost), "copy totals");
+    std::memcpy(t, host, sizeof t);
     std::memset(&out, 0, sizeof out);
     for (int k = 0; k < 4; ++k) estimate(t[k], t[5 + k], n_pairs, out.price[k], out.se[k]);
     if (c.product == 1) {
@@ -760,10 +827,10 @@ void run_strip(optlab_strip_request& rq, int prec) {
     const long long n_pairs = rq.n_paths / 2;
     const int n_slots = rq.n_mats * 64;
     double* totals = static_cast<double*>(scratch(kSlotTotals, static_cast<size_t>(n_slots) * sizeof(double)));
-    rq.kernel_ms = prec == 0 ? strip_on_device(strip_args<double>(rq), n_pairs, rq.seed, totals)
-                             : strip_on_device(strip_args<float>(rq), n_pairs, rq.seed, totals);
-    std::vector<double> t(n_slots);
-    hip_check(hipMemcpy(t.data(), totals, n_slots * sizeof(double), hipMemcpyDeviceToHost), "copy totals");
+    double* host = pinned_totals(static_cast<size_t>(n_slots));
+    rq.kernel_ms = prec == 0 ? strip_on_device(strip_args<double>(rq), n_pairs, rq.seed, totals, host)
+                             : strip_on_device(strip_args<float>(rq), n_pairs, rq.seed, totals, host);
+    const std::vector<double> t(host, host + n_slots);
     const int width = rq.n_strikes + 1;
     for (int e = 0; e < rq.n_mats; ++e)
         for (int k = 0; k < width; ++k)
