@@ -305,6 +305,36 @@ typedef struct {
     double total_ms;            /* the whole call, host wall clock */
 } optlab_lsm_request;
 
+/* ---- addition: the issuer's call regression on a pilot's states --------------------------------------------- */
+/* packages/structured/src/optlab/structured/exercise.py:191-238 (fit) and :150-171 (Rule): the least-squares fit of
+   y on the terms of a state x (n_vars variables), with z = (x - mean) / scale: polynomial terms prod_i z_i^p_i
+   (powers, exponents 0 to 3, the constant included) then hinges max(z_v - k, 0) (knots), at most 25 terms. The
+   reference solves with NumPy's lstsq; here the normal equations are formed in FP64 on the device (fixed-order
+   sums) and solved by a Cholesky factorisation. ok = 0 when the normal matrix is not numerically positive definite,
+   a pivot below 1e-13 of its diagonal (a repeated term, a hinge no state reaches): coef is then left as it was and
+   the caller decides (the reference's lstsq would return a minimum-norm solution). With n_eval > 0 the fitted continuation is evaluated on a
+   second set of states (exercise.py:170-171: Rule.continuation), in the same terms. Both backends compute in
+   FP64. */
+typedef struct {
+    long long n_rows;           /* rows of the fit, >= 1 */
+    int n_vars;                 /* 1 to 8 */
+    const double* state;        /* host, n_rows x n_vars, row major */
+    const double* y;            /* host, n_rows */
+    const double* mean;         /* host, n_vars */
+    const double* scale;        /* host, n_vars, nonzero */
+    int n_powers;               /* polynomial terms */
+    const int* powers;          /* host, n_powers x n_vars exponents in 0..3 */
+    int n_knots;                /* hinge terms; n_powers + n_knots in 1..25 */
+    const int* knot_var;        /* host, n_knots variable indices */
+    const double* knot;         /* host, n_knots levels, in z */
+    double* coef;               /* host, n_powers + n_knots (out) */
+    int ok;                     /* out: 1 when solved */
+    long long n_eval;           /* states to evaluate the continuation on (0: none) */
+    const double* eval_state;   /* host, n_eval x n_vars */
+    double* eval_out;           /* host, n_eval (out) */
+    double fit_ms, eval_ms, total_ms;
+} optlab_regress_request;
+
 /* ---- entry points ------------------------------------------------------------------------------------------- */
 
 /* requests.hpp:97-101 */
@@ -362,6 +392,10 @@ OPTLAB_API int optlab_book(optlab_book_request* rq, int backend);
 /* Addition: the American option above (backend 0 double paths, 1 float paths; the regressions in double either way).
    Replaces core/products.py:112-169 (NumPy paths, lstsq per date). */
 OPTLAB_API int optlab_lsm(optlab_lsm_request* rq, int backend);
+
+/* Addition: the issuer's call regression in FP64 normal equations (exercise.py:191-238); the reference calls
+   np.linalg.lstsq on the host. */
+OPTLAB_API int optlab_regress(optlab_regress_request* rq, int backend);
 
 /* Addition, for tests of the integer stream: the Philox4x32-10 words of counters (c0 + i, c1, c2, c3), i < n,
    key (k0, k1), computed by the device function the kernels use; out_dev holds 4 n uint32 (device memory). */

@@ -351,6 +351,16 @@ int optlab_lsm(optlab_lsm_request* rq, int backend) {
     });
 }
 
+int optlab_regress(optlab_regress_request* rq, int backend) {
+    return guarded([&] {
+        if (!rq) invalid("null request");
+        (void)precision_of(backend);  // either backend: the regression is FP64
+        need_device();
+        check_regress_request(*rq);
+        run_regress(*rq);
+    });
+}
+
 int optlab_philox_dev(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1, long long n,
                       unsigned* out_dev, void* stream) {
     return guarded([&] {

@@ -40,6 +40,7 @@ enum Slot {
     kSlotBookQueue,    // book: the work queue, trades, per-item sums and clocks
     kSlotLsm,          // Longstaff-Schwartz: spots and variances at the dates, values, partial sums, coefficients
     kSlotChunks,       // fixed products and the strip: compensated sums of chunks of the per-block rows
+    kSlotRegress,      // the issuer's call regression: terms, states, sums, coefficients
     kSlotCount
 };
 
@@ -80,6 +81,8 @@ void run_book(optlab_book_request& rq, int prec);
 // ---- American options by Longstaff-Schwartz (lsm.hip) ----------------------------------------------------------
 void check_lsm_request(const optlab_lsm_request& rq);
 void run_lsm(optlab_lsm_request& rq, int prec);
+void check_regress_request(const optlab_regress_request& rq);
+void run_regress(optlab_regress_request& rq);
 
 // ---- test entry points (paths.hip) -----------------------------------------------------------------------------
 void launch_philox(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1, long long n,

@@ -465,6 +465,54 @@ def american_lsm(model, market, K, T, cp="put", n_exercise=50, n_paths=100_000, 
                       "total_ms": rq.total_ms, "precision": "fp64" if prec == 0 else "fp32"})
 
 
+@dataclass
+class RegressResult:
+    """regress's result: the coefficients (None when the normal matrix is not positive definite), the fitted
+    continuation on eval_state (None without one) and the device timings."""
+
+    coef: np.ndarray | None
+    values: np.ndarray | None
+    timing: dict
+
+
+def regress(state, y, mean, scale, powers, knots=(), eval_state=None, backend: str = "auto") -> RegressResult:
+    """The issuer's call regression of exercise.py:191-238 on the device: the least-squares fit of y on the terms of
+    the state (exercise.py:150-171, Rule.features): z = (state - mean) / scale, polynomial terms prod_i z_i^p_i for
+    each exponent tuple of `powers` (0 to 3 each), then hinges max(z_v - k, 0) for each (v, k) of `knots`; FP64
+    normal equations summed in a fixed order and a Cholesky solve (the reference: np.linalg.lstsq). With
+    `eval_state`, the fitted continuation there (Rule.continuation). `backend` is accepted for symmetry: both
+    compute in FP64."""
+    lib = _need_lib()
+    prec = precision_code(backend)
+    x = np.ascontiguousarray(np.atleast_2d(np.asarray(state, dtype=float).T).T)
+    n, d = x.shape
+    yv = np.ascontiguousarray(y, dtype=float)
+    if yv.shape != (n,):
+        raise ValueError("y must have one value per state row")
+    mu = np.ascontiguousarray(np.broadcast_to(np.asarray(mean, dtype=float), (d,)))
+    sc = np.ascontiguousarray(np.broadcast_to(np.asarray(scale, dtype=float), (d,)))
+    pw = np.ascontiguousarray(np.asarray(powers, dtype=np.int32).reshape(-1, d))
+    kv = np.ascontiguousarray([int(v) for v, _ in knots], dtype=np.int32)
+    kl = np.ascontiguousarray([float(k) for _, k in knots], dtype=float)
+    coef = np.zeros(len(pw) + len(kl))
+    rq = _abi.RegressRequest()
+    rq.n_rows, rq.n_vars = n, d
+    rq.state, rq.y, rq.mean, rq.scale = _abi.ptr(x), _abi.ptr(yv), _abi.ptr(mu), _abi.ptr(sc)
+    rq.n_powers, rq.powers = len(pw), _abi.ptr(pw, _abi._PI)
+    rq.n_knots, rq.knot_var, rq.knot = len(kl), _abi.ptr(kv, _abi._PI), _abi.ptr(kl)
+    rq.coef = _abi.ptr(coef)
+    ev = out = None
+    if eval_state is not None:
+        ev = np.ascontiguousarray(np.atleast_2d(np.asarray(eval_state, dtype=float).T).T)
+        if ev.shape[1] != d:
+            raise ValueError("eval_state must have the state's variables")
+        out = np.empty(len(ev))
+        rq.n_eval, rq.eval_state, rq.eval_out = len(ev), _abi.ptr(ev), _abi.ptr(out)
+    _check(lib, lib.optlab_regress(C.byref(rq), prec))
+    return RegressResult(coef if rq.ok else None, out if rq.ok else None,
+                         {"fit_ms": rq.fit_ms, "eval_ms": rq.eval_ms, "total_ms": rq.total_ms})
+
+
 def leverage_request(params: HestonParams, surface, T_max: float, mix: float, n_particles: int, dt_max, dt_min,
                      n_x, bandwidth_scale, rng, l_bounds, regression, positive_part, n_bins=400, n_fine=8192,
                      atm_vol=None, bandwidth=None):

@@ -314,6 +314,32 @@ class LsmRequest(C.Structure):
     ]
 
 
+class RegressRequest(C.Structure):
+    """optlab_regress_request (addition: the issuer's call regression, exercise.py:191-238)."""
+
+    _fields_ = [
+        ("n_rows", _LL),
+        ("n_vars", _I),
+        ("state", _PD),
+        ("y", _PD),
+        ("mean", _PD),
+        ("scale", _PD),
+        ("n_powers", _I),
+        ("powers", _PI),
+        ("n_knots", _I),
+        ("knot_var", _PI),
+        ("knot", _PD),
+        ("coef", _PD),
+        ("ok", _I),
+        ("n_eval", _LL),
+        ("eval_state", _PD),
+        ("eval_out", _PD),
+        ("fit_ms", _D),
+        ("eval_ms", _D),
+        ("total_ms", _D),
+    ]
+
+
 def ptr(a, kind=_PD):
     """A ctypes pointer to a NumPy array's data (None for None or an empty array)."""
     if a is None or a.size == 0:
@@ -350,6 +376,7 @@ _SIGNATURES = {
     "optlab_adjoint": (_I, [C.POINTER(AutocallRequest), _I]),
     "optlab_book": (_I, [C.POINTER(BookRequest), _I]),
     "optlab_lsm": (_I, [C.POINTER(LsmRequest), _I]),
+    "optlab_regress": (_I, [C.POINTER(RegressRequest), _I]),
     "optlab_philox_dev": (_I, [C.c_uint] * 6 + [_LL, C.c_void_p, C.c_void_p]),
     "optlab_normals_dev": (_I, [_LL, _LL, _I, C.c_uint, _ULL, _I, C.c_void_p, C.c_void_p]),
 }

@@ -55,6 +55,7 @@ def test_ctypes_layouts_match_the_c_header(tmp_path):
         "optlab_trade": _abi.Trade,
         "optlab_book_request": _abi.BookRequest,
         "optlab_lsm_request": _abi.LsmRequest,
+        "optlab_regress_request": _abi.RegressRequest,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for cname, py in structs.items():
@@ -135,6 +136,7 @@ def test_without_a_device_the_engine_refuses_loudly():
     assert lib.optlab_adjoint(C.byref(_abi.AutocallRequest()), 2) == _abi.EINVAL  # backend
     assert lib.optlab_book(C.byref(_abi.BookRequest()), 1) == _abi.EDEVICE
     assert lib.optlab_lsm(C.byref(_abi.LsmRequest()), 1) == _abi.EDEVICE
+    assert lib.optlab_regress(C.byref(_abi.RegressRequest()), 0) == _abi.EDEVICE
 
 
 def test_the_engine_is_found_by_name():
