@@ -361,6 +361,7 @@ def bench_lsm(n_paths: int = 1 << 20, n_ex: int = 50, reps: int = 5, cpu: bool =
             out[f"{name}_{backend}"] = med
     out["workload"] = (f"American put, {n_ex} exercise dates, {n_paths} paths for the rule + {n_paths} for the "
                        "estimate, FP64 normal equations on the device")
+    out["issuer_call_regression"] = bench_regress(cpu=cpu)
     if cpu:
         sys.path.insert(0, str(ROOT / "oracle"))
         import lsm_oracle
@@ -382,6 +383,46 @@ def bench_lsm(n_paths: int = 1 << 20, n_ex: int = 50, reps: int = 5, cpu: bool =
                                "cores": 1, "kind": "port",
                                "sample": f"oracle/lsm_oracle.py lsm_rule (NumPy) on {n_cpu} FP64 paths x {n_ex} dates"}
         out["sweep_gpu_path_dates_per_s"] = n_paths * (n_ex - 1) / (gpu * 1e-3)
+    return out
+
+
+def bench_regress(n: int = 100_000, n_eval: int = 1 << 20, reps: int = 5, cpu: bool = True) -> dict:
+    """The issuer's call regression of a worst-of note (exercise.py:191-238): 3 names' spots at a call date on n
+    pilot paths (lognormal, correlation 0.6), the terms exercise.py's rules give them (the quadratic and 5 hinges a
+    name: 25), FP64 normal equations on the device, and the fitted continuation on n_eval states. Medians of reps
+    calls after a warm-up; with cpu, the reference's np.linalg.lstsq on the oracle's terms of the same rows (and
+    the terms of the n_eval states times the coefficients) timed beside it."""
+    import numpy as np
+
+    from optlab import native
+
+    from optlab.structured.exercise import terms
+
+    rng = np.random.default_rng(7)
+    c = np.full((3, 3), 0.6) + 0.4 * np.eye(3)
+    s = np.exp(0.25 * rng.standard_normal((n + n_eval, 3)) @ np.linalg.cholesky(c).T - 0.03)
+    x, ev = s[:n], s[n:]
+    w = x.min(axis=1)
+    y = 0.9 + 0.05 * np.minimum(w, 1.2) - 0.8 * np.maximum(0.6 - w, 0.0) + 0.02 * rng.standard_normal(n)
+    mean, scale = x.mean(axis=0), x.std(axis=0)
+    pw, kn = terms((x - mean) / scale)
+    native.regress(x, y, mean, scale, pw, kn, eval_state=ev)
+    runs = [native.regress(x, y, mean, scale, pw, kn, eval_state=ev) for _ in range(reps)]
+    out = {k: float(np.median([r.timing[k] for r in runs])) for k in ("fit_ms", "eval_ms", "total_ms")}
+    out.update(terms=len(pw) + len(kn), rows=n, eval_rows=n_eval)
+    if cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import regress_oracle as ro
+
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            beta = ro.fit(ro.features(x, mean, scale, pw, kn), y, "lstsq")
+            ro.features(ev, mean, scale, pw, kn) @ beta
+            ts.append(time.perf_counter() - t0)
+        out["cpu_baseline"] = {"value": float(np.median(ts)) * 1e3, "unit": "ms", "cores": 1, "kind": "port",
+                               "sample": "oracle/regress_oracle.py terms + np.linalg.lstsq (the reference's solve) "
+                                         "on the same rows, and the continuation on the same states"}
     return out
 
 

@@ -53,3 +53,13 @@ def test_basis_follows_the_room_the_paths_allow():
     assert len(pw) == 10 and len(kn) == 15
     pw, kn = ro.basis(rng.standard_normal((60, 3)))  # room 3: degree 2 (10) and 1 (4) do not fit, the constant
     assert pw == [(0, 0, 0)] and len(kn) == 0
+
+
+def test_the_product_s_term_choice_is_the_oracle_s():
+    from optlab.structured.exercise import powers, terms
+
+    rng = np.random.default_rng(3)
+    for n, d in ((100, 1), (5000, 3), (60, 3), (20_000, 1), (50_000, 5), (800, 2)):
+        z = rng.standard_normal((n, d)) * rng.uniform(0.5, 2.0, d)
+        assert terms(z) == ro.basis(z)
+        assert powers(d, 2) == ro.powers(d, 2)
