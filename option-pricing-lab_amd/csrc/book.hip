@@ -84,13 +84,15 @@ struct BookArgs {
     unsigned long long* start;
 };
 
-// The rows of step `step` of the NM names of trade t under scenario s into LDS (NM x n_x), by the workgroup.
+// The rows of step `step` of the NM names into LDS (NM x n_x), by the workgroup; base[k] is name k's first row
+// under the trade's scenario.
 template <typename R, int NM>
-__device__ __forceinline__ void stage_rows(const BookArgs<R>& a, const Shape& sh, int step, R* dst) {
+__device__ __forceinline__ void stage_rows(const BookArgs<R>& a, const R* const (&base)[NM], int step, R* dst) {
+    const int off = step * a.n_x;
 #pragma unroll
     for (int k = 0; k < NM; ++k) {  // unrolled: the names stay in registers (a runtime index would put them on
                                     // the stack)
-        const R* src = a.tables + ((static_cast<long long>(sh.scen) * a.n_book + sh.names[k]) * a.n_steps + step) * a.n_x;
+        const R* src = base[k] + off;
         for (int c = threadIdx.x; c < a.n_x; c += blockDim.x) dst[k * a.n_x + c] = src[c];
     }
 }
@@ -109,6 +111,7 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
     R lsh[NM];
     int rec[NM];  // the name's first step record; its ln F row starts at name * (n_steps + 1)
     int lfr[NM];
+    const R* base[NM];  // the name's first table row under this scenario
 #pragma unroll
     for (int k = 0; k < NM; ++k) {
         x[0][k] = x[1][k] = R(0);
@@ -116,6 +119,7 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
         lsh[k] = ldc(a.lsh, scen * a.n_book + nm);
         rec[k] = nm * a.n_steps;
         lfr[k] = nm * stride;
+        base[k] = a.tables + (static_cast<long long>(scen) * a.n_book + nm) * a.n_steps * a.n_x;
     }
     // product state, per leg
     R acc[2] = {R(0), R(0)};        // Asian: sum of fixings; barrier: running max of ln S
@@ -136,11 +140,12 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
     const int n_obs = product == 3 ? n_steps / obs_every : 0;
     const int stage_len = n * a.n_x;
     __syncthreads();  // the previous pass has read its last rows
-    stage_rows<R, NM>(a, sh, 0, rows);
+    stage_rows<R, NM>(a, base, 0, rows);
+    int to_fix = obs_every;  // steps to the next fixing (a countdown: a runtime modulo costs a division a step)
     for (int step = 0; step < n_steps; ++step) {
         const R* cur = rows + (step & 1) * stage_len;
         __syncthreads();  // this step's rows are in; everyone is done with the last step's buffer
-        if (step + 1 < n_steps) stage_rows<R, NM>(a, sh, step + 1, rows + ((step + 1) & 1) * stage_len);
+        if (step + 1 < n_steps) stage_rows<R, NM>(a, base, step + 1, rows + ((step + 1) & 1) * stage_len);
         R zs[NM];
 #pragma unroll
         for (int k = 0; k < NM; ++k)
@@ -161,7 +166,8 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
         for (int k = 0; k < NM; ++k)
             if (k < n) g[k] = ldc(a.steps, rec[k] + step);
         const int t1 = step + 1;
-        const bool fix = t1 % obs_every == 0;
+        const bool fix = --to_fix == 0;  // t1 % obs_every == 0
+        if (fix) to_fix = obs_every;
 #pragma unroll
         for (int leg = 0; leg < 2; ++leg) {
             const R sign = leg ? R(-1) : R(1);
