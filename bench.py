@@ -298,6 +298,62 @@ def bench_slv(n_particles: int = 1 << 20, T: float = 3.0, n_paths: int = 1 << 20
     }
 
 
+# The reference's published GPU kernel times (packages/native/cuda/README.md:26-40, RTX 2070 Super, float paths)
+# on its own benchmark requests (bench/bench.cpp:67-77, params.hpp:10-49: S0 100, r .04, q .01, T 1, sigma .2, the
+# SPX Heston parameters, calls at K 100, barrier H 80, seed 20261029), and its SLV leverage time
+# (ACCELERATION.md:36: 1 y, 200 000 particles, GPU).
+REFERENCE_KERNEL_MS = {
+    "bs_european_1e8x1": 0.98,
+    "bs_down_and_out_1e7x252": 8.52,
+    "heston_european_1e7x252": 92.2,
+    "heston_down_and_out_1e7x252": 91.2,
+}
+REFERENCE_SLV_1Y_200K_S = 0.16
+
+
+def bench_reference_workloads(reps: int = 5) -> dict:
+    """The reference's own benchmark workloads, FP32 paths as its GPU engine: the kernel time by the library's HIP
+    events (median of reps after a warm-up) against the published one, and the SLV leverage of 1 y on 200 000
+    particles (the reference's 400 bins and bandwidth rule; on this benchmark's SSVI case, the reference's surface
+    for that row not being among its inputs) against the published wall time."""
+    import numpy as np
+
+    from optlab import native
+    from optlab.core.market import Market
+    from optlab.core.models import BlackScholesModel, HestonModel, HestonParams
+    from optlab.native import gpu
+
+    market = Market.flat(100.0, 0.04, 0.01)
+    bs = BlackScholesModel(0.20)
+    hes = HestonModel(HestonParams(0.022846642482989885, 3.0892635624407627, 0.05372458999932075, 1.307745170926575,
+                                   -0.6879606734541107))
+    cases = {  # name: (model, product, n_steps, n_paths)
+        "bs_european_1e8x1": (bs, 0, 1, 10**8),
+        "bs_down_and_out_1e7x252": (bs, 2, 252, 10**7),
+        "heston_european_1e7x252": (hes, 0, 252, 10**7),
+        "heston_down_and_out_1e7x252": (hes, 2, 252, 10**7),
+    }
+    out = {}
+    for name, (model, product, n_steps, n_paths) in cases.items():
+        rq = gpu.request(model, market, 1.0, "call", product, n_steps, n_paths, 20261029, 100.0, 80.0)
+        gpu.price(rq, "gpu")
+        runs = [gpu.price(rq, "gpu") for _ in range(reps)]
+        k = float(np.median([r.kernel_ms for r in runs]))
+        ref = REFERENCE_KERNEL_MS[name]
+        out[name] = {"kernel_ms": k, "total_ms": float(np.median([r.total_ms for r in runs])),
+                     "path_steps_per_s": n_paths * n_steps / (k * 1e-3), "reference_kernel_ms": ref,
+                     "speedup_vs_reference_gpu": ref / k, "price": runs[0].price[0], "se": runs[0].se[0]}
+    market_s, surface, heston = slv_inputs()
+    native.calibrate_leverage(heston, surface, market_s, 0.25, 1.0, 1 << 14, rng=17, backend="gpu")  # warm-up
+    t0 = time.perf_counter()
+    native.calibrate_leverage(heston, surface, market_s, 1.0, 1.0, 200_000, rng=17, backend="gpu")
+    t = time.perf_counter() - t0
+    out["slv_leverage_1y_200k"] = {"calibration_s": t, "reference_s": REFERENCE_SLV_1Y_200K_S,
+                                   "speedup_vs_reference_gpu": REFERENCE_SLV_1Y_200K_S / t}
+    out["hardware_of_the_reference"] = "RTX 2070 Super (TU104, 40 SMs, FP32 paths)"
+    return out
+
+
 def bench_fixings(n_paths: int = 1 << 18, reps: int = 5) -> dict:
     """The fixing-heavy pass of the path engine: Black-Scholes (S0 100, r .03, q .01, sigma .2), 252 daily steps,
     every close kept, 2^18 paths, output left in HBM (optlab_paths_dev): 8 B written per path, close and name, so the
@@ -438,7 +494,7 @@ def main(argv=None) -> int:
     ap.add_argument("--book-scale", type=int, default=16,
                     help="divide every trade's paths of the batched book by this (0: skip the book)")
     ap.add_argument("--slv-particles", type=int, default=1 << 20, help="SLV particles (0: skip the SLV block)")
-    ap.add_argument("--only", choices=("adjoint", "book", "slv", "fixings", "lsm"), default=None,
+    ap.add_argument("--only", choices=("adjoint", "book", "slv", "fixings", "lsm", "reference"), default=None,
                     help="run only this block and print its JSON (for profiling runs; the benchmark line needs all)")
     ap.add_argument("--no-bumps", action="store_true", help="with --only adjoint: skip the bump-and-revalue comparator")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -464,7 +520,8 @@ def main(argv=None) -> int:
                  "book": lambda: bench_book(args.book_scale),
                  "slv": lambda: bench_slv(args.slv_particles),
                  "fixings": lambda: bench_fixings(),
-                 "lsm": lambda: bench_lsm(cpu=not args.no_cpu)}[args.only]()
+                 "lsm": lambda: bench_lsm(cpu=not args.no_cpu),
+                 "reference": lambda: bench_reference_workloads()}[args.only]()
         print(json.dumps({args.only: block}), flush=True)
         return 0
     ms32, kern32, p32, se32 = _time_gpu(lib, 1, args.steps, args.warmup)
@@ -541,6 +598,8 @@ def main(argv=None) -> int:
     print("bench: fixings block done", file=sys.stderr, flush=True)
     line["lsm"] = bench_lsm(cpu=not args.no_cpu)
     print("bench: lsm block done", file=sys.stderr, flush=True)
+    line["reference_workloads"] = bench_reference_workloads()
+    print("bench: reference workloads done", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
     if args.copy_mib:
         src = torch.ones(args.copy_mib << 20, dtype=torch.uint8, device="cuda")

@@ -287,8 +287,11 @@ template <typename R, int MODEL, int PRODUCT>
 __global__ void __launch_bounds__(kBlock) fixed_kernel(Consts<R> c, long long n_pairs, std::uint32_t k0,
                                                         std::uint32_t k1, double* partials) {
     const int lane = threadIdx.x & 63;
-    const long long wave = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
-    Comp acc{0.0, 0.0};
+    // each lane's own pairs in FP64 (a few to a few dozen terms a lane), reduced over the wave once at the end: a
+    // wave reduction per pair cost more than a one-step product itself
+    double sum[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) sum[k] = 0.0;
     const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
     for (long long base = static_cast<long long>(blockIdx.x) * blockDim.x; base < n_pairs; base += stride) {
         const long long p = base + threadIdx.x;
@@ -297,19 +300,21 @@ __global__ void __launch_bounds__(kBlock) fixed_kernel(Consts<R> c, long long n_
         src.init(static_cast<std::uint64_t>(active ? p : 0), k0, k1, 0u);
         R out[5];
         fixed_unit<R, MODEL, PRODUCT>(c, src, out);
-        double vals[16];
+        if (active) {
 #pragma unroll
-        for (int k = 0; k < 16; ++k) vals[k] = 0.0;
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            const double yk = active ? static_cast<double>(out[k]) : 0.0;
-            vals[k] = yk;
-            vals[5 + k] = yk * yk;
+            for (int k = 0; k < 5; ++k) {
+                const double yk = static_cast<double>(out[k]);
+                sum[k] += yk;
+                sum[5 + k] += yk * yk;
+            }
         }
-        reduce_scatter16(vals, lane);
-        const int v = lane & 15;
-        if (lane < 16 && v < 10) acc.add(vals[0]);
     }
+    double vals[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) vals[k] = k < 10 ? sum[k] : 0.0;
+    reduce_scatter16(vals, lane);
+    const int v = lane & 15;
+    Comp acc{lane < 16 && v < 10 ? vals[0] : 0.0, 0.0};
     // the block's waves in a fixed order: one row of 16 partial sums per block
     __shared__ double part_s[kBlock / 64][16], part_c[kBlock / 64][16];
     const int wib = threadIdx.x >> 6;
@@ -323,7 +328,6 @@ __global__ void __launch_bounds__(kBlock) fixed_kernel(Consts<R> c, long long n_
         }
         partials[static_cast<long long>(blockIdx.x) * 16 + lane] = a.value();
     }
-    (void)wave;
 }
 
 // The strip: events at mat_step[e] and 32 value slots per event. Slot k < n_strikes is the discounted
