@@ -483,20 +483,21 @@ __global__ void __launch_bounds__(kBlock) strip_kernel_f64(StripArgs<double> s, 
 }
 
 // The per-block rows (n_rows x n_slots) reduced to totals in two fixed-order stages, Neumaier-compensated:
-// stage 1, one workgroup per (group of 64 slots, chunk of kChunkRows rows): wave w adds rows w, w + 16, ... of the
-// chunk, then the 16 waves are combined in order; stage 2, one thread per slot combines the chunks in order. A
-// result depends only on the rows (the strip writes 2048 rows of 512 slots: one chain of 512 rows a wave took
-// 163 us, the chunks take a few).
-constexpr int kChunkRows = 64;
+// stage 1, one workgroup per (group of 64 slots, chunk of rows): wave w adds rows w, w + 16, ... of the chunk, then
+// the 16 waves are combined in order; stage 2, one thread per slot combines the chunks in order. Chunks are 64 rows
+// or more, at most kMaxChunks of them. A result depends only on the rows (the strip writes 2048 rows of 512 slots:
+// one chain of 512 rows a wave took 163 us, the chunks take a few).
+constexpr int kMinChunkRows = 64, kMaxChunks = 32;
 constexpr int kChunkThreads = 1024;
 
 __global__ void __launch_bounds__(kChunkThreads) reduce_chunks_kernel(const double* partials, long long n_rows,
-                                                                       int n_slots, double* chunks) {
+                                                                       int n_slots, long long chunk_rows,
+                                                                       double* chunks) {
     constexpr int W = kChunkThreads / 64;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int slot = blockIdx.x * 64 + lane;
-    const long long r0 = static_cast<long long>(blockIdx.y) * kChunkRows;
-    const long long r1 = r0 + kChunkRows < n_rows ? r0 + kChunkRows : n_rows;
+    const long long r0 = static_cast<long long>(blockIdx.y) * chunk_rows;
+    const long long r1 = r0 + chunk_rows < n_rows ? r0 + chunk_rows : n_rows;
     Comp acc{0.0, 0.0};
     if (slot < n_slots)
         for (long long r = r0 + w; r < r1; r += W) acc.add(partials[r * n_slots + slot]);
@@ -531,10 +532,11 @@ __global__ void __launch_bounds__(256) reduce_totals_kernel(const double* chunks
 }
 
 void reduce_rows(const double* partials, long long n_rows, int n_slots, double* totals, hipStream_t st) {
-    const int n_chunks = static_cast<int>((n_rows + kChunkRows - 1) / kChunkRows);
+    const long long chunk_rows = std::max<long long>(kMinChunkRows, (n_rows + kMaxChunks - 1) / kMaxChunks);
+    const int n_chunks = static_cast<int>((n_rows + chunk_rows - 1) / chunk_rows);
     double* chunks = static_cast<double*>(scratch(kSlotChunks, static_cast<size_t>(n_chunks) * n_slots * 2 * sizeof(double)));
     hipLaunchKernelGGL(reduce_chunks_kernel, dim3((n_slots + 63) / 64, n_chunks), dim3(kChunkThreads), 0, st, partials,
-                       n_rows, n_slots, chunks);
+                       n_rows, n_slots, chunk_rows, chunks);
     hipLaunchKernelGGL(reduce_totals_kernel, dim3((n_slots + 255) / 256), dim3(256), 0, st, chunks, n_chunks, n_slots,
                        totals);
     hip_check(hipGetLastError(), "reduce kernel launch");
