@@ -5,16 +5,19 @@
 // independent set.
 //
 // MI355X design: the path engine writes the spots (and the variance factor) at the exercise dates into HBM,
-// structure of arrays (a date's paths contiguous). Each date of the backward sweep is two launches on one stream,
-// with no host round trip: a grid-stride pass first brings every path's discounted value to the later date (its
-// payoff at maturity, or that date's exercise decision by the rule just fitted), then forms the normal equations of
-// the in-the-money paths (X'X, X'y and the count: 15 or 36 FP64 sums) as per-workgroup rows; one workgroup adds the
-// rows in a fixed order and solves the system by an FP64 Cholesky factorisation. The rule is then applied to the
-// second set of paths in one pass that reduces pair averages. Every sum has a fixed order, so a result does not
-// depend on anything but the request.
+// structure of arrays (a date's paths contiguous). Each date of the backward sweep is one launch on one stream,
+// with no host round trip: every workgroup first solves the later date from the per-workgroup rows of sums the
+// previous launch wrote (the same fixed-order sums and FP64 Cholesky factorisation in each, so the same bits; the
+// first workgroup publishes them), then a grid-stride pass brings every path's discounted value to the later date
+// (its payoff at maturity, or that date's exercise decision) and forms the normal equations of the in-the-money
+// paths of this date (X'X, X'y and the count: 15 or 36 FP64 sums) as per-workgroup rows, in alternate halves of
+// one buffer. One workgroup solves the first date at the end. The rule is then applied to the second set of paths
+// in one pass that reduces pair averages. Every sum has a fixed order, so a result does not depend on anything but
+// the request.
 //
-// The kernel boundary publishes the rows to the solve: a device-scope fence in every workgroup (to let the last one
-// solve) writes back the L2 of its XCD each time, which cost more than the launch it saves.
+// The kernel boundary publishes the rows to the next launch: a device-scope fence in every workgroup (to let the
+// last one solve) writes back the L2 of its XCD each time, which cost more than the launch it saves; a separate
+// one-workgroup solve launch a date cost 6-9 us more than redoing the solve in every workgroup of the next one.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -47,7 +50,7 @@ struct LsmArgs {
     const double* disc;      // n_obs
     double* coefs;           // n_obs x kCoefStride
     int* valid;              // n_obs
-    double* partial;         // workgroups x n_sums
+    double* partial;         // 2 x workgroups x n_sums: the rows of date j in half j & 1
 };
 
 // products.py:105-109, with s^3 formed as s s s.
@@ -149,22 +152,66 @@ __device__ bool cholesky_solve(const double* tot, int min_itm, double (&beta)[NB
 
 constexpr int kLsmUnroll = 4;  // rows a lane loads before it uses any: the pass is bound by memory latency
 
-// Date j of the backward sweep (products.py:151-158), first launch. Each path's discounted value is brought to date
-// j + 1 first: the payoff at maturity when j + 1 is the last date, else the exercise decision of date j + 1 by the
-// rule the previous solve fitted (an in-the-money path exercises when its exercise value beats the fitted
-// continuation). Then the in-the-money paths of date j add their terms of the normal equations to a per-workgroup
-// row.
+// The solve of date d by one workgroup: the n_rows rows of its half of `partial` added in a fixed order (lane l
+// takes rows l, l + 256, ..., then a fixed tree), then lane 0's Cholesky; beta and ok land in LDS for the whole
+// workgroup. Every workgroup that calls it for a date computes the same bits.
+template <int NB>
+__device__ void solve_date(const LsmArgs& a, int d, int n_rows, double* lds, double* tot, double* beta_s, int* ok_s) {
+    constexpr int K = n_sums<NB>();
+    const double* rows = a.partial + static_cast<long long>(d & 1) * n_rows * K;
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = 0.0;
+    for (int r = threadIdx.x; r < n_rows; r += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] += rows[static_cast<long long>(r) * K + k];
+    const double total = workgroup_sums<K>(acc, lds);
+    if (threadIdx.x < K) tot[threadIdx.x] = total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double beta[NB];
+        const bool ok = cholesky_solve<NB>(tot, a.min_itm, beta);
+#pragma unroll
+        for (int k = 0; k < NB; ++k) beta_s[k] = ok ? beta[k] : 0.0;
+        *ok_s = ok ? 1 : 0;
+    }
+    __syncthreads();
+}
+
+// The coefficients of date d (NaN when not regressed) and its flag, for the apply pass and the caller.
+template <int NB>
+__device__ void publish_date(const LsmArgs& a, int d, const double* beta_s, int ok) {
+    for (int k = threadIdx.x; k < kCoefStride; k += blockDim.x)
+        a.coefs[d * kCoefStride + k] = ok ? (k < NB ? beta_s[k] : 0.0) : NAN;
+    if (threadIdx.x == 0) a.valid[d] = ok;
+}
+
+// Date j of the backward sweep (products.py:151-158), one launch. Unless j + 1 is the last date, the workgroup first
+// solves date j + 1 from the rows the previous launch wrote (every workgroup the same bits; workgroup 0 publishes
+// them). Each path's discounted value is then brought to date j + 1: the payoff at maturity when j + 1 is the last
+// date, else date j + 1's exercise decision (an in-the-money path exercises when its exercise value beats the
+// fitted continuation). Then the in-the-money paths of date j add their terms of the normal equations to a
+// per-workgroup row, in the half of `partial` of date j.
 template <int NB>
 __global__ void __launch_bounds__(kLsmBlock) lsm_date_kernel(LsmArgs a, int j) {
     constexpr int K = n_sums<NB>();
     constexpr int U = kLsmUnroll;
     __shared__ double lds[(kLsmBlock / 64) * K];
+    __shared__ double tot[K], beta_s[NB];
+    __shared__ int ok_s;
     const int nx = j + 1;
     const bool at_maturity = nx == a.n_obs - 1;
-    const bool rule = !at_maturity && ldc(a.valid, nx);
+    bool rule = false;
     double bn[NB];
 #pragma unroll
-    for (int k = 0; k < NB; ++k) bn[k] = rule ? ldc(a.coefs, nx * kCoefStride + k) : 0.0;
+    for (int k = 0; k < NB; ++k) bn[k] = 0.0;
+    if (!at_maturity) {
+        solve_date<NB>(a, nx, gridDim.x, lds, tot, beta_s, &ok_s);
+        rule = ok_s != 0;
+#pragma unroll
+        for (int k = 0; k < NB; ++k) bn[k] = beta_s[k];
+        if (blockIdx.x == 0) publish_date<NB>(a, nx, beta_s, ok_s);
+    }
     const double* Sn = a.S + static_cast<long long>(nx) * a.rows;
     const double* Vn = a.V ? a.V + static_cast<long long>(nx) * a.rows : nullptr;
     const double* Sj = a.S + static_cast<long long>(j) * a.rows;
@@ -224,30 +271,19 @@ __global__ void __launch_bounds__(kLsmBlock) lsm_date_kernel(LsmArgs a, int j) {
         }
     }
     const double t = workgroup_sums<K>(acc, lds);
-    if (threadIdx.x < K) a.partial[static_cast<long long>(blockIdx.x) * K + threadIdx.x] = t;
+    if (threadIdx.x < K)
+        a.partial[(static_cast<long long>(j & 1) * gridDim.x + blockIdx.x) * K + threadIdx.x] = t;
 }
 
-// Date j, second launch (one workgroup): the n_rows rows added in a fixed order (lane l takes rows l, l + 256, ...,
-// then a fixed tree), the solve, the coefficients of date j (NaN when not regressed) and its flag.
+// The first date's solve (one workgroup), after the last date kernel wrote its rows.
 template <int NB>
 __global__ void __launch_bounds__(kLsmBlock) lsm_solve_kernel(LsmArgs a, int j, int n_rows) {
     constexpr int K = n_sums<NB>();
     __shared__ double lds[(kLsmBlock / 64) * K];
-    __shared__ double tot[K];
-    double acc[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] = 0.0;
-    for (int r = threadIdx.x; r < n_rows; r += blockDim.x)
-#pragma unroll
-        for (int k = 0; k < K; ++k) acc[k] += a.partial[static_cast<long long>(r) * K + k];
-    const double total = workgroup_sums<K>(acc, lds);
-    if (threadIdx.x < K) tot[threadIdx.x] = total;
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    double beta[NB];
-    const bool ok = cholesky_solve<NB>(tot, a.min_itm, beta);
-    for (int k = 0; k < kCoefStride; ++k) a.coefs[j * kCoefStride + k] = ok ? (k < NB ? beta[k] : 0.0) : NAN;
-    a.valid[j] = ok ? 1 : 0;
+    __shared__ double tot[K], beta_s[NB];
+    __shared__ int ok_s;
+    solve_date<NB>(a, j, n_rows, lds, tot, beta_s, &ok_s);
+    publish_date<NB>(a, j, beta_s, ok_s);
 }
 
 constexpr int kApplyDates = 4;  // dates of both legs a lane loads before it evaluates them
@@ -354,7 +390,7 @@ void sweep(optlab_lsm_request& rq, int prec) {
     const size_t s_bytes = up(static_cast<size_t>(n_obs) * rows * sizeof(double));
     const size_t v_bytes = NB == 7 ? s_bytes : 0;
     const size_t val_bytes = up(static_cast<size_t>(rows) * sizeof(double));
-    const size_t part_bytes = up(static_cast<size_t>(blocks) * K * sizeof(double));
+    const size_t part_bytes = up(2 * static_cast<size_t>(blocks) * K * sizeof(double));
     const size_t coef_bytes = up(static_cast<size_t>(n_obs) * kCoefStride * sizeof(double));
     const size_t valid_bytes = up(static_cast<size_t>(n_obs) * sizeof(int));
     const size_t disc_bytes = up(static_cast<size_t>(n_obs) * sizeof(double));
@@ -387,10 +423,10 @@ void sweep(optlab_lsm_request& rq, int prec) {
     hip_check(hipEventRecord(ev.e[0], st), "event");
     double paths_ms = launch_paths(rq.paths, prec, S, st, V);
     hip_check(hipEventRecord(ev.e[1], st), "event");
-    for (int j = n_obs - 2; j >= 0; --j) {  // every date's flag and coefficients are written before they are read
+    // every date's rows are solved by the next launch (date 0's by the solve kernel) before they are overwritten
+    for (int j = n_obs - 2; j >= 0; --j)
         hipLaunchKernelGGL((lsm_date_kernel<NB>), dim3(blocks), dim3(kLsmBlock), 0, st, a, j);
-        hipLaunchKernelGGL((lsm_solve_kernel<NB>), dim3(1), dim3(kLsmBlock), 0, st, a, j, blocks);
-    }
+    hipLaunchKernelGGL((lsm_solve_kernel<NB>), dim3(1), dim3(kLsmBlock), 0, st, a, 0, blocks);
     hip_check(hipGetLastError(), "lsm sweep launch");
     hip_check(hipEventRecord(ev.e[2], st), "event");
     // pass 2: the next n_pairs pairs of the same seed, the rule applied
