@@ -228,9 +228,13 @@ __global__ void __launch_bounds__(256) node_derivative_kernel(SurfaceRows s, Ten
 
 // One workgroup per (row, surface); thread (b, na) sums over x, in order, the batch-b adjoint of the entry times its
 // derivative for active node na. Lanes of one batch read one adjoint value (a broadcast) and consecutive
-// derivatives (coalesced).
+// derivatives (coalesced). The workgroup has a thread per (b, na), 320 for 16 batches x 20 nodes: with 256, one
+// wave walked the row a second time while the others waited (686 against 614 us a call); two threads per (b, na),
+// each on half the row, measured slower (667 us: fewer workgroups resident).
+constexpr int kContractThreads = 320;
+
 template <typename Adj>
-__global__ void __launch_bounds__(256) node_contraction_kernel(SurfaceSet set, DerivSet dset, int n_active,
+__global__ void __launch_bounds__(kContractThreads) node_contraction_kernel(SurfaceSet set, DerivSet dset, int n_active,
                                                                const Adj* tbar, int n_batches, double unit,
                                                                double* out) {
     const long long row = blockIdx.x;
@@ -332,8 +336,10 @@ void contract(const SurfaceSet& set, const DerivSet& dset, const TentGrid& g, co
     const long long n_rows = set.s[0].n_rows;
     double* per_row = static_cast<double*>(
         scratch(kSlotNodes, static_cast<size_t>(set.n) * n_batches * n_rows * n_active * sizeof(double)));
-    hipLaunchKernelGGL(node_contraction_kernel<A>, dim3(static_cast<unsigned>(n_rows), set.n), dim3(256), 0, stream,
-                       set, dset, n_active, tbar, n_batches, unit, per_row);
+    const int threads = (n_batches * n_active + 63) / 64 * 64;  // a thread per (batch, node), whole waves
+    if (threads > kContractThreads) invalid("at most 320 (batch, node) pairs a row");
+    hipLaunchKernelGGL(node_contraction_kernel<A>, dim3(static_cast<unsigned>(n_rows), set.n), dim3(threads), 0,
+                       stream, set, dset, n_active, tbar, n_batches, unit, per_row);
     hip_check(hipGetLastError(), "node contraction kernel launch");
     hipLaunchKernelGGL(node_sum_kernel, dim3(g.n_strikes * g.n_tenors, n_batches, set.n), dim3(64), 0, stream, set, g,
                        per_row, n_batches, out);
