@@ -237,6 +237,11 @@ __device__ __forceinline__ long long to_fixed(R v) {
     return static_cast<long long>(rint(v * static_cast<R>(kFixed)));
 }
 
+// 1 / x for the adjoint's weights: the hardware reciprocal in FP32 (1 ulp; an IEEE division is ten instructions in
+// a kernel that is bound by its instruction issue), the division in FP64.
+__device__ __forceinline__ float adj_rcp(float x) { return hw_rcp(x); }
+__device__ __forceinline__ double adj_rcp(double x) { return 1.0 / x; }
+
 template <typename R, int N, int SEG>
 __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s, long long n_chunks, R* xbar,
                                                                long long* partial) {
@@ -290,7 +295,7 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
                     if constexpr (kKeep) {
                         lu[k][a] = l.u;
                         ls[k][a] = l.slope;
-                        lw[k][a] = s.dt * (R(-0.5) + R(0.5) * z[a] / sd);
+                        lw[k][a] = s.dt * (R(-0.5) + R(0.5) * z[a] * adj_rcp(sd));
                     } else {
                         xs[k][a] = x[a];
                         zz[k][a] = z[a];
@@ -329,7 +334,7 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
                         const Lerp<R> l = lerp_at(y, static_cast<R>(ldc(s.lo[a], step)),
                                                   static_cast<R>(ldc(s.inv[a], step)), s.n_x, xs[k][a]);
                         u = l.u, slope = l.slope;
-                        wgt = s.dt * (R(-0.5) + R(0.5) * zz[k][a] / r_sqrt(l.var * s.dt));
+                        wgt = s.dt * (R(-0.5) + R(0.5) * zz[k][a] * adj_rcp(r_sqrt(l.var * s.dt)));
                     }
                     const R vbar = xb[a] * wgt;
                     const int i0 = static_cast<int>(u);
