@@ -195,9 +195,10 @@ __device__ __forceinline__ void qe_pair(const Consts<float>& c, float (&y)[2], f
         m2[a] = m[a] * m[a];
         ex[a] = !(s2[a] <= c.psi_c * m2[a]);
     }
-    QeBranch o[2] = {};
-    if (!ex[0]) o[0] = qe_quadratic(c, m[0], s2[0], m2[0], zv);
-    if (!ex[1]) o[1] = qe_quadratic(c, m[1], s2[1], m2[1], -zv);
+    // The quadratic branch for both legs in every lane (a wave almost always has a lane that needs it, so a
+    // condition would not skip it; without one there are no defaults to set and no exec mask to save): a lane past
+    // psi_c overwrites it with the exponential branch below, and selects drop whatever the quadratic made there.
+    QeBranch o[2] = {qe_quadratic(c, m[0], s2[0], m2[0], zv), qe_quadratic(c, m[1], s2[1], m2[1], -zv)};
     if (ex[0] || ex[1]) {
         const bool first = ex[0];  // the leg this pass serves: 0 when leg 0 needs it, else 1
         const QeBranch r = qe_exponential(c, first ? m[0] : m[1], first ? s2[0] : s2[1], first ? m2[0] : m2[1],
