@@ -41,6 +41,10 @@ SLOTS_PER_PATH_STEP = 134
 # vector peak and the unpacked FP32 peak; packed FP32 is not used).
 LANE_SLOTS_PER_S = 256 * 64 * 2.4e9
 PEAK_TFLOPS = 78.6
+# HBM bytes a launch of the FP32 strip kernel moves, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this
+# workload (480.2 and 8192 KiB a dispatch), scaled by what the counters read for a 256 MiB device copy (x 2.0 and
+# x 1.0): the constants and strike table read, one row of 8 maturities x 64 sums written per workgroup.
+STRIP_TRAFFIC_BYTES = int((480.2 * 2.0 + 8192.0) * 1024)
 
 
 # ---- the SLV particle workload (SSVI surface, Heston factor, uniform daily grid) --------------------------------
@@ -567,7 +571,7 @@ def main(argv=None) -> int:
             "peak": PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": STRIP_TRAFFIC_BYTES,
             "kernel_ms": kern32,
             "slots_per_path_step": SLOTS_PER_PATH_STEP,
             "roofline_path_steps_per_s": LANE_SLOTS_PER_S / SLOTS_PER_PATH_STEP,
