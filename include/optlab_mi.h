@@ -299,9 +299,10 @@ typedef struct {
     int min_itm;                /* regress only with more than this many paths in the money (50) */
     double price, se;
     double* coefs;              /* host or NULL: n_obs x 7 regression coefficients, NaN at dates that exercise nothing */
-    double paths_ms;            /* both path passes (device events around the path kernels) */
-    double regression_ms;       /* the backward sweep: two launches a date (device events) */
-    double apply_ms;            /* the rule applied to pass 2 (device events around that kernel) */
+    double paths_ms;            /* pass 1's path kernel (device events) */
+    double regression_ms;       /* the backward sweep: one launch a date (device events) */
+    double apply_ms;            /* pass 2: its path kernel, which applies the rule as the paths reach each date and
+                                   writes no spots (device events) */
     double total_ms;            /* the whole call, host wall clock */
 } optlab_lsm_request;
 

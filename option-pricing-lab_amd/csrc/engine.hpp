@@ -58,6 +58,20 @@ double launch_paths(const optlab_paths_request& rq, int prec, double* d_out, hip
                     double* d_var = nullptr);
 void check_paths_request(const optlab_paths_request& rq);
 
+// An exercise rule for the path kernel of one name (the pricing pass of Longstaff-Schwartz): at observation j a leg
+// exercises when its exercise value max(phi (S / K - 1), 0) is positive and beats coefs[j] . basis(S / K, v), the
+// first such date or maturity; nothing is written but the sums. Device pointers.
+struct ExerciseRule {
+    const double* coefs;  // n_obs x 7 over (1, s, s^2, s^3, v, v^2, s v), 0 for a term not in the basis
+    const int* valid;     // n_obs: 1 where date j has a rule (never the last)
+    const double* disc;   // n_obs
+    double strike, phi;   // phi: +1 call, -1 put
+    double* sums;         // 2 x exercise_workgroups(n_pairs): the sums of the pairs' average pay and of its square
+};
+long long exercise_workgroups(long long n_pairs);
+// The path kernel with the rule applied at the observations (no output columns); the kernel time in ms.
+double launch_paths_exercise(const optlab_paths_request& rq, int prec, const ExerciseRule& rule, hipStream_t stream);
+
 // ---- fixed products and strips (fixed.hip) ---------------------------------------------------------------------
 void check_price_request(const optlab_request& rq);
 void run_price(const optlab_request& rq, int prec, optlab_result& out);

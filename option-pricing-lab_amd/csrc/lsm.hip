@@ -12,8 +12,9 @@
 // (its payoff at maturity, or that date's exercise decision) and forms the normal equations of the in-the-money
 // paths of this date (X'X, X'y and the count: 15 or 36 FP64 sums) as per-workgroup rows, in alternate halves of
 // one buffer. One workgroup solves the first date at the end. The rule is then applied to the second set of paths
-// in one pass that reduces pair averages. Every sum has a fixed order, so a result does not depend on anything but
-// the request.
+// inside the path kernel itself, at each date as the paths reach it (ExerciseRule), which writes only its
+// workgroups' sums of the pair averages: the second set's spots never reach HBM. Every sum has a fixed order, so a
+// result does not depend on anything but the request.
 //
 // The kernel boundary publishes the rows to the next launch: a device-scope fence in every workgroup (to let the
 // last one solve) writes back the L2 of its XCD each time, which cost more than the launch it saves; a separate
@@ -286,79 +287,8 @@ __global__ void __launch_bounds__(kLsmBlock) lsm_solve_kernel(LsmArgs a, int j, 
     publish_date<NB>(a, j, beta_s, ok_s);
 }
 
-constexpr int kApplyDates = 4;  // dates of both legs a lane loads before it evaluates them
-
-// products.py:137-150 on the second set of paths: each leg exercises at the first date whose rule says so (the
-// reference's backward loop keeps the earliest), else at maturity; the pair average of K cash D(t_tau) and its
-// square, summed per workgroup. The two legs of a pair walk the dates together, kApplyDates at a time.
-template <int NB>
-__global__ void __launch_bounds__(kLsmBlock) lsm_apply_kernel(LsmArgs a, double strike, double* sums) {
-    constexpr int D = kApplyDates;
-    __shared__ double lds[(kLsmBlock / 64) * 2];
-    const long long n_pairs = a.rows / 2;
-    const int last = a.n_obs - 1;
-    double acc[2] = {0.0, 0.0};
-    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
-    for (long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; p < n_pairs; p += stride) {
-        const long long row[2] = {p, n_pairs + p};
-        double cash[2] = {0.0, 0.0};
-        int tau[2] = {last, last};
-        bool done[2] = {false, false};
-        for (int j0 = 0; j0 < last && !(done[0] && done[1]); j0 += D) {
-            double s[2][D], w[2][D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const long long at = static_cast<long long>(min(j0 + d, last)) * a.rows;
-#pragma unroll
-                for (int leg = 0; leg < 2; ++leg) {
-                    s[leg][d] = a.S[at + row[leg]];
-                    w[leg][d] = a.V ? a.V[at + row[leg]] : 0.0;
-                }
-            }
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const int j = j0 + d;
-                if (j >= last || !ldc(a.valid, j)) continue;
-                double beta[NB];
-#pragma unroll
-                for (int k = 0; k < NB; ++k) beta[k] = ldc(a.coefs, j * kCoefStride + k);
-#pragma unroll
-                for (int leg = 0; leg < 2; ++leg) {
-                    if (done[leg]) continue;
-                    const double x = s[leg][d] * a.inv_strike;
-                    const double ex = fmax(a.phi * (x - 1.0), 0.0);
-                    if (!(ex > 0.0)) continue;
-                    double X[NB];
-                    basis<NB>(x, w[leg][d], X);
-                    double cont = 0.0;
-#pragma unroll
-                    for (int k = 0; k < NB; ++k) cont += X[k] * beta[k];
-                    if (ex > cont) {
-                        cash[leg] = ex;
-                        tau[leg] = j;
-                        done[leg] = true;
-                    }
-                }
-            }
-        }
-        double pay = 0.0;
-#pragma unroll
-        for (int leg = 0; leg < 2; ++leg) {
-            if (!done[leg])
-                cash[leg] =
-                    fmax(a.phi * (a.S[static_cast<long long>(last) * a.rows + row[leg]] * a.inv_strike - 1.0), 0.0);
-            pay += strike * cash[leg] * ldc(a.disc, tau[leg]);
-        }
-        const double d = 0.5 * pay;
-        acc[0] += d;
-        acc[1] += d * d;
-    }
-    const double t = workgroup_sums<2>(acc, lds);
-    if (threadIdx.x < 2) sums[blockIdx.x * 2 + threadIdx.x] = t;
-}
-
 struct Events {
-    hipEvent_t e[5];
+    hipEvent_t e[3];
     Events() {
         for (auto& x : e) hip_check(hipEventCreate(&x), "event");
     }
@@ -380,12 +310,11 @@ void sweep(optlab_lsm_request& rq, int prec) {
     const long long n_pairs = rq.paths.n_pairs, rows = 2 * n_pairs;
     constexpr int K = n_sums<NB>();
     // the sweep: two workgroups a CU, kLsmUnroll rows a lane at a time (the solve adds two rows a lane; four
-    // workgroups a CU measured slower: the reduction is per workgroup); the apply pass: four a CU over the pairs
+    // workgroups a CU measured slower: the reduction is per workgroup)
     const long long tile = static_cast<long long>(kLsmBlock) * kLsmUnroll;
     const int blocks = static_cast<int>(
         std::max<long long>(1, std::min<long long>((rows + tile - 1) / tile, 2LL * device_cu_count())));
-    const int apply_blocks = static_cast<int>(std::max<long long>(
-        1, std::min<long long>((n_pairs + kLsmBlock - 1) / kLsmBlock, 4LL * device_cu_count())));
+    const long long apply_blocks = exercise_workgroups(n_pairs);
     auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t s_bytes = up(static_cast<size_t>(n_obs) * rows * sizeof(double));
     const size_t v_bytes = NB == 7 ? s_bytes : 0;
@@ -429,14 +358,11 @@ void sweep(optlab_lsm_request& rq, int prec) {
     hipLaunchKernelGGL((lsm_solve_kernel<NB>), dim3(1), dim3(kLsmBlock), 0, st, a, 0, blocks);
     hip_check(hipGetLastError(), "lsm sweep launch");
     hip_check(hipEventRecord(ev.e[2], st), "event");
-    // pass 2: the next n_pairs pairs of the same seed, the rule applied
+    // pass 2: the next n_pairs pairs of the same seed, the rule applied as the paths reach each date
     optlab_paths_request p2 = rq.paths;
     p2.first_pair = rq.paths.first_pair + n_pairs;
-    paths_ms += launch_paths(p2, prec, S, st, V);
-    hip_check(hipEventRecord(ev.e[3], st), "event");
-    hipLaunchKernelGGL((lsm_apply_kernel<NB>), dim3(apply_blocks), dim3(kLsmBlock), 0, st, a, rq.strike, sums);
-    hip_check(hipGetLastError(), "lsm apply launch");
-    hip_check(hipEventRecord(ev.e[4], st), "event");
+    const ExerciseRule rule{a.coefs, a.valid, disc, rq.strike, a.phi, sums};
+    const double apply_ms = launch_paths_exercise(p2, prec, rule, st);
     std::vector<double> h(static_cast<size_t>(apply_blocks) * 2);
     hip_check(hipMemcpyAsync(h.data(), sums, h.size() * sizeof(double), hipMemcpyDeviceToHost, st), "copy sums");
     std::vector<double> coefs(static_cast<size_t>(n_obs) * kCoefStride);
@@ -445,7 +371,7 @@ void sweep(optlab_lsm_request& rq, int prec) {
     hip_check(hipStreamSynchronize(st), "synchronize");
 
     double s1 = 0.0, s2 = 0.0;
-    for (int b = 0; b < apply_blocks; ++b) s1 += h[2 * b], s2 += h[2 * b + 1];
+    for (long long b = 0; b < apply_blocks; ++b) s1 += h[2 * b], s2 += h[2 * b + 1];
     const double n = static_cast<double>(n_pairs);
     rq.price = s1 / n;
     rq.se = n > 1 ? std::sqrt(std::max((s2 - s1 * s1 / n) / (n - 1.0), 0.0) / n) : 0.0;
@@ -455,7 +381,7 @@ void sweep(optlab_lsm_request& rq, int prec) {
     if (rq.coefs) std::memcpy(rq.coefs, coefs.data(), coefs.size() * sizeof(double));
     rq.paths_ms = paths_ms;
     rq.regression_ms = ev.ms(1, 2);
-    rq.apply_ms = ev.ms(3, 4);
+    rq.apply_ms = apply_ms;
     rq.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
 }
 

@@ -54,6 +54,12 @@ struct PathArgs {
     int tab_slot[8];     // slot of name a among them (-1: none)
     int staged;          // 1: the step's rows are copied to LDS
     int has_var, has_slv;
+    // an exercise rule applied at the observations of one name (kFormExercise), in place of writing them
+    const double* ex_coefs;  // n_obs x 7: the continuation value's coefficients of each date
+    const int* ex_valid;     // n_obs: 1 where the date has a rule
+    const double* ex_disc;   // n_obs: D(t_j)
+    double ex_strike, ex_inv_strike, ex_phi;
+    double* ex_sums;         // 2 a workgroup: the sum of the pairs' average pay and of its square
 };
 
 template <typename R>
@@ -135,12 +141,81 @@ __device__ __forceinline__ void record(const PathArgs<R>& s, const long long (&r
 // instructions a step;
 // kFormGeneric: tables, Heston and SLV names, no mixer;
 // kFormMixed: the generic form with a mixer (local correlation, or the stochastic weight with its state and its
-// driver's stream), kept apart so that the other forms carry none of its registers.
-constexpr int kFormCV = 0, kFormGeneric = 1, kFormMixed = 2;
+// driver's stream), kept apart so that the other forms carry none of its registers;
+// kFormExercise, added to kFormCV or kFormGeneric (one name): nothing is written at the observations; an exercise
+// rule is applied there instead and the workgroup's sums of the pay are written.
+constexpr int kFormCV = 0, kFormGeneric = 1, kFormMixed = 2, kFormExercise = 4;
+
+// The exercise state of a lane's two legs: the exercise value (in units of the strike) and its date.
+struct Exercise {
+    double cash[2];
+    int tau[2];
+    bool done[2];
+};
+
+// products.py:137-150 at observation j, for both legs: a leg exercises at the first date whose rule says so (the
+// reference's backward loop keeps the earliest), else at maturity, where it takes its payoff. The spot is formed
+// as record forms it (f exp(x) in the path precision, then widened), so the rule sees the values its regressions
+// were fitted on; the basis is (1, s, s^2, s^3) in s = S / K, then (v, v^2, s v), with s^3 formed as s s s.
+template <typename R, int N>
+__device__ __forceinline__ void exercise(const PathArgs<R>& s, int j, const R (&x)[2][N], const R (&v)[2][N],
+                                         Exercise& e) {
+    const int last = s.n_obs - 1;
+    const R f = ldc(s.fwd, j);
+    const bool rule = j < last && ldc(s.ex_valid, j);
+    // every date has its row (unread where there is no rule): loaded unconditionally, the row's seven loads are
+    // issued together and waited for once, not one branch and one wait each
+    double beta[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) beta[k] = ldc(s.ex_coefs, j * 7 + k);
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+        if (e.done[l]) continue;
+        const double sx = static_cast<double>(f * r_exp(x[l][0])) * s.ex_inv_strike;
+        const double ex = fmax(s.ex_phi * (sx - 1.0), 0.0);
+        if (j == last) {
+            e.cash[l] = ex;
+            continue;
+        }
+        if (!rule || !(ex > 0.0)) continue;
+        const double w = static_cast<double>(v[l][0]);
+        const double X[7] = {1.0, sx, sx * sx, sx * sx * sx, w, w * w, sx * w};
+        // the coefficients of the variance terms are 0 with the basis without v: each adds an exact 0
+        double cont = 0.0;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) cont += X[k] * beta[k];
+        if (ex > cont) {
+            e.cash[l] = ex;
+            e.tau[l] = j;
+            e.done[l] = true;
+        }
+    }
+}
+
+// The workgroup's sums of a and b (every lane one value each), in a fixed order: a butterfly over each wave, then
+// the waves in order; written to out[2 blockIdx.x], out[2 blockIdx.x + 1].
+__device__ __forceinline__ void workgroup_pair_sums(double a, double b, double* out) {
+    __shared__ double part[kBlock / 64][2];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        a += __shfl_xor(a, m);
+        b += __shfl_xor(b, m);
+    }
+    if (threadIdx.x % 64 == 0) part[threadIdx.x / 64][0] = a, part[threadIdx.x / 64][1] = b;
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) t += part[w][threadIdx.x];
+        out[2 * blockIdx.x + threadIdx.x] = t;
+    }
+}
 
 template <typename R, int N, int FORM>
 __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
-    constexpr bool CV = FORM == kFormCV;
+    constexpr bool EXER = (FORM & kFormExercise) != 0;
+    constexpr bool CV = (FORM & 3) == kFormCV;
+    static_assert(!EXER || (N == 1 && (FORM & 3) != kFormMixed), "an exercise rule is on one name, no mixer");
     extern __shared__ unsigned char lds_raw[];
     R* lds = reinterpret_cast<R*>(lds_raw);
     const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -162,8 +237,15 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
     const bool staged = !CV && s.staged, has_var = !CV && s.has_var;
     constexpr bool mixer = FORM == kFormMixed;
     Extremes<R, N> ext;
+    Exercise exs{{0.0, 0.0}, {s.n_obs - 1, s.n_obs - 1}, {false, false}};
+    auto observe = [&](int jj) {
+        if constexpr (EXER)
+            exercise<R, N>(s, jj, x, v, exs);
+        else
+            record<R, N>(s, row, active, jj, x, v, ext);
+    };
     int j = 0;
-    while (j < s.n_obs && ldc(s.obs_step, j) == 0) record<R, N>(s, row, active, j++, x, v, ext);
+    while (j < s.n_obs && ldc(s.obs_step, j) == 0) observe(j++);
 
     NormalStream<R> rng, aux, drv;
     rng.init(pair, s.key0, s.key1, 0u);
@@ -302,7 +384,17 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
                 x[leg][a] += R(-0.5) * var * dt + r_sqrt(var * dt) * z[a];
             }
         }
-        while (j < s.n_obs && ldc(s.obs_step, j) == i) record<R, N>(s, row, active, j++, x, v, ext);
+        while (j < s.n_obs && ldc(s.obs_step, j) == i) observe(j++);
+    }
+    if constexpr (EXER) {  // products.py:150 and :30-35: K cash D(t_tau) of each leg, the pair's average
+        double d = 0.0;
+        if (active) {
+            double pay = 0.0;
+#pragma unroll
+            for (int l = 0; l < 2; ++l) pay += s.ex_strike * exs.cash[l] * ldc(s.ex_disc, exs.tau[l]);
+            d = 0.5 * pay;
+        }
+        workgroup_pair_sums(d, d * d, s.ex_sums);
     }
 }
 
@@ -333,6 +425,15 @@ void launch_form(const PathArgs<R>& s, size_t lds, hipStream_t stream) {
 
 template <typename R, int N>
 void launch_n(const PathArgs<R>& s, size_t lds, hipStream_t stream) {
+    if constexpr (N == 1) {
+        if (s.ex_sums) {
+            if (s.n_tab == 0 && !s.has_var)
+                launch_form<R, 1, kFormCV | kFormExercise>(s, lds, stream);
+            else
+                launch_form<R, 1, kFormGeneric | kFormExercise>(s, lds, stream);
+            return;
+        }
+    }
     if (s.mixer)
         launch_form<R, N, kFormMixed>(s, lds, stream);
     else if (s.n_tab == 0 && !s.has_var)
@@ -342,7 +443,8 @@ void launch_n(const PathArgs<R>& s, size_t lds, hipStream_t stream) {
 }
 
 template <typename R>
-double launch_typed(const optlab_paths_request& rq, double* d_out, double* d_var, hipStream_t stream) {
+double launch_typed(const optlab_paths_request& rq, double* d_out, double* d_var, hipStream_t stream,
+                    const ExerciseRule* rule = nullptr) {
     const int n = rq.n_names;
     const long long nsn = static_cast<long long>(n) * rq.n_steps;
     Arena ar;
@@ -387,6 +489,13 @@ double launch_typed(const optlab_paths_request& rq, double* d_out, double* d_var
     }
     s.out = d_out;
     s.out_var = d_var;
+    if (rule) {
+        if (n != 1 || rq.mixer) invalid("an exercise rule is on one name, no mixer");
+        s.out = s.out_var = nullptr;
+        s.ex_coefs = rule->coefs, s.ex_valid = rule->valid, s.ex_disc = rule->disc;
+        s.ex_strike = rule->strike, s.ex_inv_strike = 1.0 / rule->strike, s.ex_phi = rule->phi;
+        s.ex_sums = rule->sums;
+    }
     s.n_tab = 0;
     s.has_var = s.has_slv = 0;
     for (int a = 0; a < 8; ++a) s.tab_slot[a] = -1, s.tab_of[a] = 0;
@@ -440,6 +549,13 @@ __global__ void normals_kernel(long long first_pair, long long n_pairs, int n_no
 
 double launch_paths(const optlab_paths_request& rq, int prec, double* d_out, hipStream_t stream, double* d_var) {
     return prec == 0 ? launch_typed<double>(rq, d_out, d_var, stream) : launch_typed<float>(rq, d_out, d_var, stream);
+}
+
+long long exercise_workgroups(long long n_pairs) { return (n_pairs + kBlock - 1) / kBlock; }
+
+double launch_paths_exercise(const optlab_paths_request& rq, int prec, const ExerciseRule& rule, hipStream_t stream) {
+    return prec == 0 ? launch_typed<double>(rq, nullptr, nullptr, stream, &rule)
+                     : launch_typed<float>(rq, nullptr, nullptr, stream, &rule);
 }
 
 void launch_philox(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1, long long n,
