@@ -83,6 +83,7 @@ struct Spec {
     const optlab_paths_request* rq;
     long long rows;
     std::uint32_t k0, k1;
+    double* var = nullptr;  // or the variance factor of name 0 at the kept columns (n_keep x rows)
 };
 
 // paths.hpp:196-205: uniform-grid lerp, u clamped to [0, n_x - 1 - 1e-4].
@@ -150,8 +151,13 @@ void pair_paths(const Spec& s, long long p) {
         v[0][a] = v[1][a] = hv ? r.heston[a * 5] : 0.0;
     }
     Extremes ext;
+    auto obs = [&](int jj) {
+        const int c = r.col_of[jj];
+        if (s.var && c >= 0) s.var[c * s.rows + p] = v[0][0], s.var[c * s.rows + r.n_pairs + p] = v[1][0];
+        observe(s, p, jj, x[0], x[1], ext);
+    };
     int j = 0;
-    while (j < r.n_obs && r.obs_step[j] == 0) observe(s, p, j++, x[0], x[1], ext);
+    while (j < r.n_obs && r.obs_step[j] == 0) obs(j++);
     double zs[kNames], zr[kNames], fresh = 0.0, drive = 0.0;
     Draws main{pair, s.k0, s.k1};
     Draws aux{pair, s.k0, s.k1, kAuxTag};
@@ -238,7 +244,7 @@ void pair_paths(const Spec& s, long long p) {
                 x[leg][a] += -0.5 * var * dt + std::sqrt(var * dt) * z[a];
             }
         }
-        while (j < r.n_obs && r.obs_step[j] == i) observe(s, p, j++, x[0], x[1], ext);
+        while (j < r.n_obs && r.obs_step[j] == i) obs(j++);
     }
 }
 
@@ -527,11 +533,14 @@ void oracle_normals(long long first_pair, long long n_pairs, int n_normals, unsi
     }
 }
 
-int oracle_paths(optlab_paths_request* rq, int n_threads) {
+int oracle_paths(optlab_paths_request* rq, int n_threads) { return oracle_paths_var(rq, n_threads, nullptr); }
+
+int oracle_paths_var(optlab_paths_request* rq, int n_threads, double* var) {
     try {
         if (!rq) bad("null request");
         check_paths(*rq);
-        Spec s{rq, 2 * rq->n_pairs, static_cast<std::uint32_t>(rq->seed), static_cast<std::uint32_t>(rq->seed >> 32)};
+        Spec s{rq, 2 * rq->n_pairs, static_cast<std::uint32_t>(rq->seed), static_cast<std::uint32_t>(rq->seed >> 32),
+               var};
         const long long block = 512, n_blocks = (rq->n_pairs + block - 1) / block;
         for_blocks(n_blocks, n_threads, [&](long long k) {
             const long long a = k * block, b = std::min(rq->n_pairs, a + block);

@@ -22,6 +22,9 @@ void oracle_normals(long long first_pair, long long n_pairs, int n_normals, unsi
 
 /* requests.cpp:264-280 with backend 0 (paths_cpu.cpp): every pair in double on n_threads threads (0: all). */
 int oracle_paths(optlab_paths_request* rq, int n_threads);
+/* The same, also writing the variance factor v of name 0 (before its floor at 0) at the kept columns into var
+   (n_keep x 2 n_pairs doubles), as the device's Longstaff-Schwartz pass keeps it for the basis with v. */
+int oracle_paths_var(optlab_paths_request* rq, int n_threads, double* var);
 
 /* The fixed products on Philox stream 0 of each pair, in double (schemes.hpp + cpu_mc.cpp:14-18). */
 int oracle_price(const optlab_request* rq, int n_threads, optlab_result* out);

@@ -31,6 +31,7 @@ def oracle() -> C.CDLL:
         "oracle_philox": (None, [C.c_uint] * 6 + [C.c_longlong, C.c_void_p]),
         "oracle_normals": (None, [C.c_longlong, C.c_longlong, C.c_int, C.c_uint, C.c_ulonglong, C.c_void_p]),
         "oracle_paths": (C.c_int, [C.POINTER(_abi.PathsRequest), C.c_int]),
+        "oracle_paths_var": (C.c_int, [C.POINTER(_abi.PathsRequest), C.c_int, C.c_void_p]),
         "oracle_price": (C.c_int, [C.POINTER(_abi.Request), C.c_int, C.POINTER(_abi.Result)]),
         "oracle_strip": (C.c_int, [C.POINTER(_abi.StripRequest), C.c_int]),
         "oracle_leverage": (C.c_int, [C.POINTER(_abi.LeverageRequest), C.c_int, C.c_int]),
@@ -101,6 +102,21 @@ def oracle_paths(p, n_pairs: int, seed: int, first_pair: int = 0, keep=None, win
     assert status == 0
     del keepalive
     return out
+
+
+def oracle_paths_var(p, n_pairs: int, seed: int, first_pair: int = 0):
+    """(S, V): the oracle's spots of the plan's one name and its variance factor at every observation, each
+    (2 n_pairs, n_obs)."""
+    from optlab import native
+    from optlab.native import _abi
+
+    rq, keepalive, m = native.paths_request(p, n_pairs, seed, first_pair)
+    out = np.empty((1, rq.n_cols, 2 * n_pairs))
+    var = np.empty((rq.n_keep, 2 * n_pairs))
+    rq.out = _abi.ptr(out)
+    assert oracle().oracle_paths_var(C.byref(rq), 0, var.ctypes.data) == 0
+    del keepalive
+    return out[0, :m].T.copy(), var.T.copy()
 
 
 def calls(S, D, strikes):

@@ -10,7 +10,7 @@ import sys
 
 import numpy as np
 import pytest
-from _support import ROOT
+from _support import ROOT, oracle_paths_var
 from test_oracle import _golden_models
 
 sys.path.insert(0, str(ROOT / "oracle"))
@@ -69,6 +69,38 @@ def test_fp64_lsm_is_the_oracle_on_the_same_paths(case):
     # the reference's lstsq on the same paths: the same exercise decisions up to ties
     ref, _, _ = lo.lsm_price(S1, None, S2, None, disc, strike, spot, call=cp == "call", solver="lstsq")
     assert got.price == pytest.approx(ref, rel=1e-6)
+
+
+def test_fp64_lsm_with_the_variance_basis_is_the_oracle_on_its_paths():
+    """The basis with v (seven terms) against the oracle's LSM on the oracle's own Heston paths and variance factor:
+    the device's FP64 paths agree with those to about 1e-12, and on this seed a 1e-12 relative perturbation of S
+    and V moves the oracle's price by 1e-13 at most (no exercise decision flips), so the prices must agree to 1e-10."""
+    market, models = _golden_models()
+    model = models["heston"]
+    strike = market.spot * 1.1
+    n_paths, seed = 1 << 14, 29
+    times = np.linspace(T / N_EX, T, N_EX)
+    print("lsm fp64 heston, basis with v: engine", flush=True)
+    got = native.american_lsm(model, market, strike, T, "put", N_EX, n_paths, rng=seed, backend="fp64")
+    print("oracle", flush=True)
+    p = native.plan(model, market, times)
+    S1, V1 = oracle_paths_var(p, n_paths // 2, native.seed64(seed), 0)
+    S2, V2 = oracle_paths_var(p, n_paths // 2, native.seed64(seed), n_paths // 2)
+    want, want_se, coefs = lo.lsm_price(S1, V1, S2, V2, market.D(times), strike, market.spot)
+    assert got.price == pytest.approx(want, rel=1e-10)
+    assert got.stderr == pytest.approx(want_se, rel=1e-8)
+    s1 = S1 / strike
+    for j, c in enumerate(coefs):
+        if c is None:
+            assert np.all(np.isnan(got.coefs[j]))
+            continue
+        itm = s1[:, j] < 1.0
+        X = lo.basis(s1[itm, j], V1[itm, j])
+        A = X.T @ X
+        # backward error of the device's seven coefficients on the oracle's system; the two path sets differ by
+        # about 1e-12, so 1e-9 (a 1e-4 slip in one coefficient gives about 1e-5)
+        back = np.linalg.norm(A @ (got.coefs[j] - c)) / (np.linalg.norm(A) * np.linalg.norm(c))
+        assert back < 1e-9, (j, back)
 
 
 def test_fp32_bermudan_put_is_within_monte_carlo_error_of_the_tree():

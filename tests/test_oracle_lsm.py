@@ -1,6 +1,7 @@
 """The Longstaff-Schwartz oracle (oracle/lsm_oracle.py) pinned on the CPU: its binomial-tree reference against the
 Black-Scholes European and its own convergence; the LSM estimate on NumPy Black-Scholes paths against the tree (a
-lower bound within Monte Carlo error); the normal equations against lstsq; the immediate-exercise floor."""
+lower bound within Monte Carlo error); the normal equations against lstsq; the immediate-exercise floor; the oracle's
+variance factor at the observations (the basis with v) against its deterministic recursion."""
 
 from __future__ import annotations
 
@@ -8,7 +9,7 @@ import sys
 
 import numpy as np
 import pytest
-from _support import ROOT
+from _support import ROOT, oracle_paths, oracle_paths_var
 from scipy.stats import norm
 
 sys.path.insert(0, str(ROOT / "oracle"))
@@ -72,3 +73,29 @@ def test_the_basis_with_a_variance_factor_has_seven_columns():
     X = lo.basis(s, v)
     np.testing.assert_array_equal(X[:, 6], s * v)
     assert X.shape == (2, 7) and lo.basis(s).shape == (2, 4)
+
+
+def test_oracle_variance_factor_at_the_observations():
+    """oracle_paths_var (the paths with v kept for the basis with v): the spots are oracle_paths' own; with xi = 0
+    the variance factor is deterministic and follows the Euler recursion v += kappa (theta - max(v, 0)) dt, the same
+    on both legs; with xi > 0 the legs' variances differ and the antithetic pair's mean over many pairs stays near
+    the recursion's value (E[v] obeys it while v stays positive)."""
+    from optlab import native
+    from optlab.core.market import Market
+    from optlab.core.models import HestonModel, HestonParams
+
+    market = Market.flat(100.0, 0.03, 0.0)
+    times = np.linspace(0.1, 1.0, 10)
+    p = native.plan(HestonModel(HestonParams(0.09, 1.5, 0.04, 0.0, -0.7)), market, times)
+    S, V = oracle_paths_var(p, 64, 5)
+    assert np.array_equal(S, oracle_paths(p, 64, 5)[0].T)
+    v, want = 0.09, []
+    for i in range(1, len(p.dt) + 1):
+        v += 1.5 * (0.04 - max(v, 0.0)) * p.dt[i - 1]
+        want += [v] * int(np.sum(p.obs_step == i))
+    assert np.allclose(V, np.array(want)[None, :], rtol=1e-14, atol=0.0)
+    q = native.plan(HestonModel(HestonParams(0.09, 1.5, 0.04, 0.3, -0.7)), market, times)
+    S, V = oracle_paths_var(q, 20_000, 5)
+    assert np.array_equal(S, oracle_paths(q, 20_000, 5)[0].T)
+    assert not np.array_equal(V[:20_000], V[20_000:])
+    assert np.max(np.abs(V.mean(axis=0) - np.array(want))) < 2e-3
