@@ -32,15 +32,16 @@ def engine_paths(model, market, times, n_paths, seed, first_pair):
     return spots[0]
 
 
-@pytest.mark.parametrize("case", ["bs_put", "lv_put", "lv_call"])
+@pytest.mark.parametrize("case", ["bs_put", "bs_put_ragged", "lv_put", "lv_call"])
 def test_fp64_lsm_is_the_oracle_on_the_same_paths(case):
-    if case == "bs_put":
+    if case.startswith("bs_put"):
         model, market, strike, cp, spot = BlackScholesModel(SIGMA), Market.flat(S0, R, 0.0), K, "put", S0
     else:
         market, models = _golden_models()
         model, spot = models["lv"], market.spot
         strike, cp = (spot * 1.05, "put") if case == "lv_put" else (spot * 0.95, "call")
-    n_paths, seed = 1 << 14, 17
+    # ragged: 5001 pairs, so the last workgroup of the pricing pass and the sweep's last tile are partly empty
+    n_paths, seed = (2 * 5001 if case == "bs_put_ragged" else 1 << 14), 17
     times = np.linspace(T / N_EX, T, N_EX)
     print(f"lsm fp64 {case}: engine", flush=True)
     got = native.american_lsm(model, market, strike, T, cp, N_EX, n_paths, rng=seed, backend="fp64")
