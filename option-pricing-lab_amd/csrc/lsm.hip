@@ -151,7 +151,9 @@ __device__ bool cholesky_solve(const double* tot, int min_itm, double (&beta)[NB
     return true;
 }
 
-constexpr int kLsmUnroll = 4;  // rows a lane loads before it uses any: the pass is bound by memory latency
+// rows a lane loads before it uses any: at 2^20 rows and two workgroups a CU, a lane's whole share in one round
+// of loads (four rows at a time took 1 us more a date with the variance basis, the same without)
+constexpr int kLsmUnroll = 8;
 
 // The solve of date d by one workgroup: the n_rows rows of its half of `partial` added in a fixed order (lane l
 // takes rows l, l + 256, ..., then a fixed tree), then lane 0's Cholesky; beta and ok land in LDS for the whole
