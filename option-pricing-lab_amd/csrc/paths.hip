@@ -22,6 +22,17 @@ constexpr int kBlock = 256;
 constexpr int kMaxNames = 8;
 constexpr size_t kStageBytes = 64 * 1024;  // LDS for two buffers of the step's table rows
 
+// An exercise rule in device memory (one pointer in the kernel's arguments: the fields are loaded where the rule is
+// applied, so they hold no scalar registers over the step loop).
+struct ExerciseDev {
+    const double* coefs;  // n_obs x 7: the continuation value's coefficients of each date
+    const int* valid;     // n_obs: 1 where the date has a rule
+    const double* disc;   // n_obs: D(t_j)
+    double strike, inv_strike, phi;
+    double* sums;         // 2 a workgroup: the sum of the pairs' average pay and of its square
+};
+typedef const __attribute__((address_space(4))) ExerciseDev* ExerciseConst;
+
 template <typename R>
 struct PathArgs {
     int n_names, n_steps, n_obs, n_x;
@@ -54,12 +65,7 @@ struct PathArgs {
     int tab_slot[8];     // slot of name a among them (-1: none)
     int staged;          // 1: the step's rows are copied to LDS
     int has_var, has_slv;
-    // an exercise rule applied at the observations of one name (kFormExercise), in place of writing them
-    const double* ex_coefs;  // n_obs x 7: the continuation value's coefficients of each date
-    const int* ex_valid;     // n_obs: 1 where the date has a rule
-    const double* ex_disc;   // n_obs: D(t_j)
-    double ex_strike, ex_inv_strike, ex_phi;
-    double* ex_sums;         // 2 a workgroup: the sum of the pairs' average pay and of its square
+    const ExerciseDev* ex;  // kFormExercise: the rule applied at the observations of one name, in place of writing
 };
 
 template <typename R>
@@ -160,19 +166,20 @@ struct Exercise {
 template <typename R, int N>
 __device__ __forceinline__ void exercise(const PathArgs<R>& s, int j, const R (&x)[2][N], const R (&v)[2][N],
                                          Exercise& e) {
+    const ExerciseConst r = (ExerciseConst)s.ex;
     const int last = s.n_obs - 1;
     const R f = ldc(s.fwd, j);
-    const bool rule = j < last && ldc(s.ex_valid, j);
+    const bool rule = j < last && ldc(r->valid, j);
     // every date has its row (unread where there is no rule): loaded unconditionally, the row's seven loads are
     // issued together and waited for once, not one branch and one wait each
     double beta[7];
 #pragma unroll
-    for (int k = 0; k < 7; ++k) beta[k] = ldc(s.ex_coefs, j * 7 + k);
+    for (int k = 0; k < 7; ++k) beta[k] = ldc(r->coefs, j * 7 + k);
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
         if (e.done[l]) continue;
-        const double sx = static_cast<double>(f * r_exp(x[l][0])) * s.ex_inv_strike;
-        const double ex = fmax(s.ex_phi * (sx - 1.0), 0.0);
+        const double sx = static_cast<double>(f * r_exp(x[l][0])) * r->inv_strike;
+        const double ex = fmax(r->phi * (sx - 1.0), 0.0);
         if (j == last) {
             e.cash[l] = ex;
             continue;
@@ -387,14 +394,15 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
         while (j < s.n_obs && ldc(s.obs_step, j) == i) observe(j++);
     }
     if constexpr (EXER) {  // products.py:150 and :30-35: K cash D(t_tau) of each leg, the pair's average
+        const ExerciseConst r = (ExerciseConst)s.ex;
         double d = 0.0;
         if (active) {
             double pay = 0.0;
 #pragma unroll
-            for (int l = 0; l < 2; ++l) pay += s.ex_strike * exs.cash[l] * ldc(s.ex_disc, exs.tau[l]);
+            for (int l = 0; l < 2; ++l) pay += r->strike * exs.cash[l] * ldc(r->disc, exs.tau[l]);
             d = 0.5 * pay;
         }
-        workgroup_pair_sums(d, d * d, s.ex_sums);
+        workgroup_pair_sums(d, d * d, r->sums);
     }
 }
 
@@ -426,7 +434,7 @@ void launch_form(const PathArgs<R>& s, size_t lds, hipStream_t stream) {
 template <typename R, int N>
 void launch_n(const PathArgs<R>& s, size_t lds, hipStream_t stream) {
     if constexpr (N == 1) {
-        if (s.ex_sums) {
+        if (s.ex) {
             if (s.n_tab == 0 && !s.has_var)
                 launch_form<R, 1, kFormCV | kFormExercise>(s, lds, stream);
             else
@@ -461,6 +469,13 @@ double launch_typed(const optlab_paths_request& rq, double* d_out, double* d_var
     const size_t o_fwd = ar.reals<R>(rq.fwd, static_cast<long long>(n) * rq.n_obs);
     const size_t o_low = ar.reals<R>(rq.mixer ? rq.low_root : nullptr, n * n);
     const size_t o_col = ar.ints(rq.col_of, rq.n_obs);
+    size_t o_ex = 0;
+    if (rule) {
+        if (n != 1 || rq.mixer) invalid("an exercise rule is on one name, no mixer");
+        const ExerciseDev e{rule->coefs, rule->valid, rule->disc, rule->strike, 1.0 / rule->strike, rule->phi,
+                            rule->sums};
+        o_ex = ar.put(&e, sizeof(e));
+    }
     ar.put(nullptr, 0);
     unsigned char* dev = static_cast<unsigned char*>(scratch(kSlotSpec, std::max<size_t>(ar.host.size(), 256)));
     hip_check(hipMemcpyAsync(dev, ar.host.data(), ar.host.size(), hipMemcpyHostToDevice, stream), "copy spec");
@@ -490,11 +505,8 @@ double launch_typed(const optlab_paths_request& rq, double* d_out, double* d_var
     s.out = d_out;
     s.out_var = d_var;
     if (rule) {
-        if (n != 1 || rq.mixer) invalid("an exercise rule is on one name, no mixer");
         s.out = s.out_var = nullptr;
-        s.ex_coefs = rule->coefs, s.ex_valid = rule->valid, s.ex_disc = rule->disc;
-        s.ex_strike = rule->strike, s.ex_inv_strike = 1.0 / rule->strike, s.ex_phi = rule->phi;
-        s.ex_sums = rule->sums;
+        s.ex = reinterpret_cast<const ExerciseDev*>(at(o_ex));
     }
     s.n_tab = 0;
     s.has_var = s.has_slv = 0;
