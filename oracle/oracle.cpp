@@ -686,6 +686,11 @@ int oracle_leverage(optlab_leverage_request* rq, int n_threads, int n_parts) {
 }
 
 int oracle_leverage_ex(optlab_leverage_request* rq, const optlab_leverage_options* opt, int n_threads, int n_parts) {
+    return oracle_leverage_trace(rq, opt, n_threads, n_parts, nullptr, nullptr, -1, nullptr);
+}
+
+int oracle_leverage_trace(optlab_leverage_request* rq, const optlab_leverage_options* opt, int n_threads,
+                          int n_parts, double* bw_out, double* std_out, int x_step, double* x_out) {
     try {
         const int rule = opt ? opt->bandwidth_rule : 0;
         if (rule < 0 || rule > 1) bad("bandwidth_rule must be 0 or 1");
@@ -714,6 +719,9 @@ int oracle_leverage_ex(optlab_leverage_request* rq, const optlab_leverage_option
         double lo = 0, hi = 0;
         for (int i = 0; i < s.n_steps; ++i) {
             const double t0 = s.t0[i], h = s.dt[i];
+            if (x_out && i == x_step) std::copy(X.begin(), X.end(), x_out);
+            if (bw_out) bw_out[i] = 0.0;
+            if (std_out) std_out[i] = 0.0;
             if (i == 0) {
                 for (int g = 0; g < nx; ++g)
                     x_eval[g] = (-6.0 + 12.0 * g / (nx - 1)) * s.atm_vol * std::sqrt(std::max(h, 1e-4)), cond[g] = s.v0;
@@ -757,7 +765,9 @@ int oracle_leverage_ex(optlab_leverage_request* rq, const optlab_leverage_option
                     for (int k = 0; k < nb; ++k) sx += bins.acc[2 * nb + k], sxx += bins.acc[3 * nb + k];
                     const double mean = sx / n_particles, var = std::max(sxx / n_particles - mean * mean, 0.0);
                     bw = opt->bandwidth_coef * std::sqrt(var) * std::pow(n_particles, -0.2);
+                    if (std_out) std_out[i] = std::sqrt(var);
                 }
+                if (bw_out) bw_out[i] = bw;
                 const double width = (hi + 1e-12 - lo) / nb;
                 for (int k = 0; k < nb; ++k) centres[k] = lo + (k + 0.5) * width;
                 const double fallback = bins.v_sum / n_particles;

@@ -39,6 +39,13 @@ int oracle_leverage(optlab_leverage_request* rq, int n_threads, int n_parts);
 /* The same with the options of include/optlab_mi.h (opt NULL: the reference's bandwidth). */
 int oracle_leverage_ex(optlab_leverage_request* rq, const optlab_leverage_options* opt, int n_threads, int n_parts);
 
+/* oracle_leverage_ex, also reporting each step's kernel bandwidth (bw_out, n_steps; 0 at step 0, which has none)
+   and, under bandwidth rule 1, the standard deviation of the particles' log-spot offsets that set it (std_out,
+   n_steps; 0 under rule 0), and the 2 n_pairs log-spot offsets x the particles hold when step x_step starts
+   (x_out; rows p and n_pairs + p are pair p). Any output may be NULL. */
+int oracle_leverage_trace(optlab_leverage_request* rq, const optlab_leverage_options* opt, int n_threads,
+                          int n_parts, double* bw_out, double* std_out, int x_step, double* x_out);
+
 /* closed_forms.cpp:21-33: geometric Asian on n equally spaced fixings. */
 double oracle_geometric_asian(double S, double K, double T, double r, double q, double sigma, int n, int call);
 

@@ -37,6 +37,8 @@ def oracle() -> C.CDLL:
         "oracle_leverage": (C.c_int, [C.POINTER(_abi.LeverageRequest), C.c_int, C.c_int]),
         "oracle_leverage_ex": (C.c_int, [C.POINTER(_abi.LeverageRequest), C.POINTER(_abi.LeverageOptions), C.c_int,
                                          C.c_int]),
+        "oracle_leverage_trace": (C.c_int, [C.POINTER(_abi.LeverageRequest), C.POINTER(_abi.LeverageOptions), C.c_int,
+                                            C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
         "oracle_geometric_asian": (C.c_double, [C.c_double] * 6 + [C.c_int, C.c_int]),
     }
     return _abi.declare(lib, sig)

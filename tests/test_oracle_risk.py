@@ -161,3 +161,18 @@ def test_var_es_reads_an_actual_scenario():
     assert r["var"] == 50.0  # the 1 % "lower" quantile of 100 points is the worst one
     assert r["es"] == pytest.approx(np.mean([50.0, 49.0, 48.0]))  # beyond the 2.5 % quantile: 3 scenarios
     assert x[r["var_index"]] == -50.0
+
+
+def test_tents_equal_the_reference_grid_vectors():
+    """The reference's own strike tents, their smoothing width and the tenor tents on a (k, T) set that crosses every
+    node, the width's floor and the ends (tests/golden/make_tent_fixtures.py wrote them from risk/grid.py)."""
+    g = np.load(ROOT / "tests" / "golden" / "risk_tents.npz")
+    k, T = g["k"], g["T"]
+    np.testing.assert_allclose(np.stack(so.width(k, T)), g["smoothing_width"], rtol=1e-14, atol=1e-15)
+    got = so.strike_tents(k, T)
+    for d in range(4):
+        scale = np.abs(g["strike_tents"][d]).max()
+        np.testing.assert_allclose(got[d], g["strike_tents"][d], rtol=1e-14, atol=1e-14 * scale)
+    psi, dpsi = so.tenor_tents(g["tenor_T"])
+    np.testing.assert_allclose(psi, g["tenor_tents"][0], rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(dpsi, g["tenor_tents"][1], rtol=1e-14, atol=1e-14)

@@ -1,0 +1,72 @@
+"""The oracle's SLV bandwidth rule pinned without a GPU: rule 1 (coef std(ln S) N^-1/5 over the particles at each
+step) against NumPy's population standard deviation of the oracle's own particles, and against rule 0 (the
+reference's bandwidth, leverage.cpp:63) at the bandwidth scale that makes the two equal. A plausible slip -- the
+standard deviation of the wrong sample (the variance factor, or the particles after the step), pairs counted for
+particles, the sample instead of the population variance -- fails one of these."""
+
+from __future__ import annotations
+
+import ctypes as C
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+from _support import oracle
+
+sys.path.insert(0, str(Path(__file__).resolve().parents[1]))
+import bench  # noqa: E402
+
+from optlab import native  # noqa: E402
+from optlab.native import _abi  # noqa: E402
+
+COEF = 0.25
+
+
+def _request(n_particles, T, dt, bandwidth, atm_vol=None):
+    market, surface, heston = bench.slv_inputs()
+    return native.leverage_request(heston, surface, T, 1.0, n_particles, dt, None, 81, 1.0, 17, (0.05, 8.0),
+                                   "nadaraya_watson", False, n_bins=256, bandwidth=bandwidth, atm_vol=atm_vol)
+
+
+def _trace(rq, x_step=-1):
+    n = rq.n_steps
+    bw, sd = np.zeros(n), np.zeros(n)
+    x = np.zeros(2 * rq.n_pairs)
+    assert oracle().oracle_leverage_trace(C.byref(rq), C.byref(rq._options), 2, 2, _abi.ptr(bw), _abi.ptr(sd),
+                                          int(x_step), _abi.ptr(x)) == 0
+    return bw, sd, x
+
+
+@pytest.mark.parametrize("step", [1, 7, 36])
+def test_rule_one_bandwidth_is_coef_times_the_particles_population_std(step):
+    n_particles = 6_000
+    rq, _, _ = _request(n_particles, 0.1, 1 / 365, ("std", COEF))
+    assert rq.n_steps == 37
+    bw, sd, x = _trace(rq, step)
+    assert x.std() > 0 and np.all(x[: n_particles // 2] != x[n_particles // 2:])  # both legs of every pair
+    want = COEF * np.std(x) * float(n_particles) ** -0.2  # population std over all 2 n_pairs particles
+    np.testing.assert_allclose(sd[step], np.std(x), rtol=1e-11)
+    np.testing.assert_allclose(bw[step], want, rtol=1e-11)
+    # the slips this rules out, each far outside the tolerance
+    assert abs(np.std(x, ddof=1) / np.std(x) - 1) > 1e-5  # sample variance
+    assert abs((n_particles / 2) ** -0.2 / n_particles ** -0.2 - 1) > 0.1  # pairs for particles
+    assert bw[0] == 0.0 and np.all(bw[1:] > 0)
+
+
+def test_rule_one_equals_rule_zero_at_the_equivalent_scale_bit_for_bit():
+    """Two steps of a quarter year with atm_vol 2: rule 0's bandwidth at step 1 is scale * 2 * sqrt(0.25) * N^-0.2
+    = scale * N^-0.2 exactly, so scale = coef * std (the oracle's own std, a product by a power of two) must give
+    rule 1's leverage bit for bit."""
+    rq1, got1, _ = _request(4_000, 0.5, 0.25, ("std", COEF), atm_vol=2.0)
+    assert rq1.n_steps == 2 and list(np.asarray(got1["values"]).shape) == [2, 81]
+    bw1, sd1, _ = _trace(rq1)
+    rq0, got0, _ = _request(4_000, 0.5, 0.25, None, atm_vol=2.0)
+    rq0.bandwidth_scale = COEF * sd1[1]
+    bw0, _, _ = _trace(rq0)
+    assert bw0[1] == bw1[1]
+    np.testing.assert_array_equal(got0["x_grids"], got1["x_grids"])
+    np.testing.assert_array_equal(got0["values"], got1["values"])
+    rq0.bandwidth_scale = COEF * sd1[1] * (1 + 1e-6)  # and a different scale moves the leverage
+    _trace(rq0)
+    assert not np.array_equal(got0["values"][1], got1["values"][1])
