@@ -1,10 +1,14 @@
-"""Which engine simulates, asked for by name (packages/core/src/optlab/core/engines.py:1-137).
+"""Engine selection by name, as the reference's registry exposes it (packages/core/src/optlab/core/engines.py:1-137).
 
-"cpu" and "gpu" are the compiled engine; "auto" takes it when it can run the model; "numpy" is the library's own
-NumPy reference, which lives outside this build. In this build the compiled engine is the HIP engine of
-`optlab.native`: "gpu" runs its float mode and "cpu" its double mode -- the reference CPU engine's numbers -- both
-on the GPU. The registry first looks for an engine registered under the entry point group `optlab.engines` (name
-"native"), then for the in-tree `optlab.native`."""
+The interface kept from the reference: the names "numpy", "cpu", "gpu" and "auto"; the entry point group
+`optlab.engines` under which a compiled engine registers as "native"; the `Engine` protocol; `EngineError`; the
+functions `compiled`, `set_default`, `default_engine`, `choose`, `require`; and `seed64`, which turns a seed into
+the engine's 64-bit Philox key.
+
+How this build resolves them: the compiled engine is the HIP engine `optlab.native`, found through the entry point
+when a distribution registers one, else imported from the source tree. Both of its modes run on the GPU -- "gpu" is
+its FP32 mode and "cpu" its FP64 mode (the reference CPU engine's numbers) -- so any named engine other than
+"numpy" needs a device. "numpy" stays a name only: the reference's NumPy pricers are not part of this build."""
 
 from __future__ import annotations
 
@@ -16,7 +20,6 @@ import numpy as np
 
 ENGINES = ("numpy", "cpu", "gpu", "auto")
 GROUP = "optlab.engines"
-NOT_INSTALLED = "no compiled engine is installed (optlab-native)"
 
 
 class EngineError(RuntimeError):
@@ -44,78 +47,102 @@ class Engine(Protocol):
     def calibrate_leverage(self, params, surface, market, T_max: float, *args, **kwargs) -> Any: ...
 
 
-_DEFAULT: str | None = None
-_compiled: list = []
+def _engine_name(name: str) -> str:
+    key = str(name).strip().lower()
+    if key not in ENGINES:
+        raise ValueError(f"unknown engine {name!r}: the engine names are {', '.join(ENGINES)}")
+    return key
+
+
+class _Registry:
+    """The process's engine preference and the compiled engine, looked up once on first use."""
+
+    def __init__(self) -> None:
+        self.preferred: str | None = None
+        self._looked_up = False
+        self._engine: Engine | None = None
+
+    def engine(self) -> Engine | None:
+        if not self._looked_up:
+            self._engine = self._find()
+            self._looked_up = True
+        return self._engine
+
+    @staticmethod
+    def _find() -> Engine | None:
+        # A registered engine that fails to import raises here: a broken install must not pass for "none".
+        for ep in entry_points(group=GROUP):
+            if ep.name == "native":
+                return cast(Engine, ep.load())
+        try:
+            import optlab.native as in_tree
+        except ModuleNotFoundError:
+            return None
+        return cast(Engine, in_tree)
+
+    def why_not(self, model) -> str | None:
+        """None when the compiled engine can run `model` now, else the reason."""
+        eng = self.engine()
+        if eng is None:
+            return "the compiled engine optlab.native is not importable"
+        reason = eng.unsupported(model)
+        if reason is not None:
+            return reason
+        if not eng.available():
+            return "the compiled engine is not built"
+        if not eng.gpu_available():
+            return "no HIP device"
+        return None
+
+
+_REGISTRY = _Registry()
 
 
 def compiled() -> Engine | None:
-    """The engine registered as "native" in `optlab.engines`, else the in-tree `optlab.native`, else None. An engine
-    that is found but fails to import raises: a broken install is never silently replaced."""
-    if not _compiled:
-        found = next((ep for ep in entry_points(group=GROUP) if ep.name == "native"), None)
-        if found is not None:
-            _compiled.append(cast(Engine, found.load()))
-        else:
-            try:
-                import optlab.native as native
-            except ModuleNotFoundError:
-                native = None
-            _compiled.append(cast(Engine, native) if native is not None else None)
-    return _compiled[0]
+    """The engine registered as "native" in `optlab.engines`, else the in-tree `optlab.native`, else None."""
+    return _REGISTRY.engine()
 
 
 def set_default(engine: str | None) -> None:
-    global _DEFAULT
-    if engine is not None and engine.lower() not in ENGINES:
-        raise ValueError(f"engine must be one of {', '.join(ENGINES)}, not {engine!r}")
-    _DEFAULT = engine.lower() if engine else None
+    """The engine used where a call names none; None or "" restores $OPTLAB_ENGINE (else "numpy")."""
+    _REGISTRY.preferred = _engine_name(engine) if engine else None
 
 
 def default_engine() -> str:
-    if _DEFAULT:
-        return _DEFAULT
-    e = os.environ.get("OPTLAB_ENGINE", "numpy").lower()
-    return e if e in ENGINES else "numpy"
+    if _REGISTRY.preferred is not None:
+        return _REGISTRY.preferred
+    env = os.environ.get("OPTLAB_ENGINE", "").strip().lower()
+    return env if env in ENGINES else "numpy"
 
 
 def choose(engine: str | None, model) -> tuple[str, str | None]:
-    """(backend, reason) for this request, as engines.py:96-120: "auto" falls back to "numpy" with the reason, a
-    named engine that cannot run raises EngineError."""
-    engine = (engine or default_engine()).lower()
-    if engine not in ENGINES:
-        raise ValueError(f"engine must be one of {', '.join(ENGINES)}, not {engine!r}")
-    if engine == "numpy":
+    """(backend, reason) for simulating `model` on the engine named `engine` (None: the default). "numpy" is
+    returned as is; "auto" resolves to "gpu", or to "numpy" with the reason when the compiled engine cannot run the
+    model; "cpu" and "gpu" that cannot run raise EngineError with the reason."""
+    name = _engine_name(engine or default_engine())
+    if name == "numpy":
         return "numpy", None
-    eng = compiled()
-    if eng is None:
-        why = NOT_INSTALLED
-    else:
-        why = eng.unsupported(model)
-        if why is None and not eng.available():
-            why = "the compiled engine is not built"
-        if why is None and not eng.gpu_available():  # every mode of this engine runs on the GPU
-            why = "no HIP device"
-    if why is not None:
-        if engine == "auto":
-            return "numpy", why
-        raise EngineError(why)
-    assert eng is not None  # noqa: S101
-    if engine == "auto":
-        return "gpu", None
-    return engine, None
+    reason = _REGISTRY.why_not(model)
+    if name == "auto":
+        return ("numpy", reason) if reason is not None else ("gpu", None)
+    if reason is not None:
+        raise EngineError(reason)
+    return name, None
 
 
 def require() -> Engine:
+    """The compiled engine; EngineError when there is none."""
     eng = compiled()
     if eng is None:
-        raise EngineError(NOT_INSTALLED)
+        raise EngineError("the compiled engine optlab.native is not importable")
     return eng
 
 
 def seed64(seed) -> int:
-    """The engine's 64-bit Philox key from an int, a SeedSequence or None (engines.py:131-137)."""
-    if isinstance(seed, np.random.SeedSequence):
-        return int(seed.generate_state(1, np.uint64)[0])
+    """The engine's 64-bit Philox key: an int modulo 2^64, a SeedSequence's first 64-bit state word, or that of
+    fresh entropy for None (the reference's key derivation, engines.py:131-137)."""
     if seed is None:
-        return int(np.random.SeedSequence().generate_state(1, np.uint64)[0])
-    return int(seed) & (2**64 - 1)
+        seed = np.random.SeedSequence()
+    if isinstance(seed, np.random.SeedSequence):
+        return int(seed.generate_state(1, dtype=np.uint64)[0])
+    return int(seed) % (1 << 64)

@@ -129,103 +129,150 @@ def cpu_threads() -> int:
     return int(lib.optlab_cpu_threads()) if lib is not None else (os.cpu_count() or 1)
 
 
-# --- models to tables (native/__init__.py:112-187) ---------------------------------------------------------------
+# --- models to tables: the plan layer (native/__init__.py:112-302) ------------------------------------------------
+#
+# A plan is the arrays of include/optlab_mi.h's path request that do not depend on the pairs: the time grid, the
+# observation steps, each name's kind and parameters, the forwards, and the rows of every tabulated name stacked in
+# one table (a name's steps point at its own rows). Local-variance rows are the Dupire variance at each step start
+# on n_x points spanning [-8, 6] ATM standard deviations; SLV rows are the squared leverage resampled on n_x points.
 
-_TABLES: OrderedDict = OrderedDict()
-
-
-def _cached(key, obj, build):
-    hit = _TABLES.get(key)
-    if hit is not None and hit[0] is obj:
-        _TABLES.move_to_end(key)
-        return hit[1]
-    rows = build()
-    _TABLES[key] = (obj, rows)
-    while len(_TABLES) > 16:
-        _TABLES.popitem(last=False)
-    return rows
+SD_RANGE = (-8.0, 6.0)
+_NO_FACTOR = (0.0, 1.0, 1.0, 0.0, 0.0)  # Heston slot of a name without a variance factor
 
 
-def local_var_rows(surface, t0: np.ndarray, n_x: int, sd_range=(-8.0, 6.0)):
-    """(lo, inv_dx, rows): the Dupire local variance at each step start on n_x uniform points of x spanning
-    sd_range ATM standard deviations sqrt(w(0, T)), T = max(t, 1 day) (native/__init__.py:129-144)."""
+class _RowCache:
+    """Tables built from a surface or a leverage, kept for the last few (object, grid) keys while the object lives:
+    a plan rebuilt on the same model and grid reuses its rows."""
 
-    def build():
-        if hasattr(surface, "local_var_rows"):
-            return surface.local_var_rows(t0, n_x, sd_range)
-        lo, inv, rows = np.empty(len(t0)), np.empty(len(t0)), np.empty((len(t0), n_x))
-        for i, t in enumerate(t0):
-            T = max(float(t), 1.0 / 365)
-            sd = float(np.sqrt(surface.w(0.0, T)))
-            a, b = sd_range[0] * sd, sd_range[1] * sd
-            lo[i], inv[i] = a, (n_x - 1) / (b - a)
-            rows[i] = surface.local_var(np.linspace(a, b, n_x), T)
-        return lo, inv, rows
+    def __init__(self, size: int = 16):
+        self.size = size
+        self._entries: OrderedDict = OrderedDict()
 
-    return _cached((id(surface), "lv", n_x, t0.tobytes(), tuple(sd_range)), surface, build)
+    def get(self, obj, key, build):
+        key = (id(obj), *key)
+        entry = self._entries.get(key)
+        if entry is None or entry[0] is not obj:
+            entry = (obj, build())
+            self._entries[key] = entry
+            while len(self._entries) > self.size:
+                self._entries.popitem(last=False)
+        self._entries.move_to_end(key)
+        return entry[1]
+
+
+_ROWS = _RowCache()
+
+
+def _surface_rows_by_row(surface, t0, n_x: int, sd_range):
+    """Rows of a surface that only offers w(k, T) and local_var(k, T) at one maturity at a time."""
+    T = np.maximum(np.asarray(t0, dtype=float), 1.0 / 365)
+    ends = np.empty((len(T), 2))
+    rows = np.empty((len(T), n_x))
+    for i, Ti in enumerate(T):
+        sd = float(np.sqrt(surface.w(0.0, float(Ti))))
+        ends[i] = sd_range[0] * sd, sd_range[1] * sd
+        rows[i] = surface.local_var(np.linspace(ends[i, 0], ends[i, 1], n_x), float(Ti))
+    return ends[:, 0].copy(), (n_x - 1) / (ends[:, 1] - ends[:, 0]), rows
+
+
+def local_var_rows(surface, t0: np.ndarray, n_x: int, sd_range=SD_RANGE):
+    """(lo, inv_dx, rows) of the Dupire local variance at each step start (native/__init__.py:129-144). A surface
+    with its own `local_var_rows` (optlab.core's SVI and SSVI: every row in one vectorised pass) builds them
+    itself; any other is asked row by row."""
+    t0 = np.asarray(t0, dtype=float)
+    own = getattr(surface, "local_var_rows", None)
+    build = (lambda: own(t0, n_x, sd_range)) if own is not None else (
+        lambda: _surface_rows_by_row(surface, t0, n_x, sd_range))
+    return _ROWS.get(surface, ("lv", n_x, t0.tobytes(), tuple(sd_range)), build)
+
+
+def _leverage_curves(leverage):
+    """(x_first, x_last, x, L) of each time row of a leverage: calibrated rows (x_grids, values), or uniform rows
+    (x0, inv_dx, values) as fastmodels.UniformLeverage holds them."""
+    if hasattr(leverage, "rows"):
+        for x0, inv_dx, y in leverage.rows:
+            y = np.asarray(y, dtype=float)
+            x1 = x0 + (len(y) - 1) / inv_dx
+            yield x0, x1, np.linspace(x0, x1, len(y)), y
+    else:
+        for x, y in zip(leverage.x_grids, leverage.values):
+            yield x[0], x[-1], x, y
 
 
 def leverage_rows(leverage, t0: np.ndarray, n_x: int):
-    """(lo, inv_dx, squared leverage rows, row of each step) (native/__init__.py:147-167)."""
+    """(lo, inv_dx, squared leverage rows, row of each step) (native/__init__.py:147-167): each time row resampled
+    linearly on n_x points between its first and last x, squared; step i reads the last time row at or before t0[i]
+    (+1e-12)."""
 
     def build():
+        curves = list(_leverage_curves(leverage))
+        lo = np.array([c[0] for c in curves], dtype=float)
+        hi = np.array([c[1] for c in curves], dtype=float)
+        table = np.stack([np.interp(np.linspace(a, b, n_x), x, y) ** 2 for a, b, x, y in curves])
         times = np.asarray(leverage.times, dtype=float)
-        if hasattr(leverage, "rows"):
-            src = [(x0, x0 + (len(y) - 1) / inv, np.linspace(x0, x0 + (len(y) - 1) / inv, len(y)), y)
-                   for x0, inv, y in leverage.rows]
-        else:
-            src = [(xs[0], xs[-1], xs, vs) for xs, vs in zip(leverage.x_grids, leverage.values)]
-        lo, inv, rows = np.empty(len(src)), np.empty(len(src)), np.empty((len(src), n_x))
-        for r, (a, b, xs, vs) in enumerate(src):
-            grid = np.linspace(a, b, n_x)
-            lo[r], inv[r] = a, (n_x - 1) / (b - a)
-            rows[r] = np.interp(grid, xs, vs) ** 2
-        step = np.clip(np.searchsorted(times, t0 + 1e-12, side="right") - 1, 0, len(times) - 1).astype(np.int32)
-        return lo, inv, rows, step
+        which = np.searchsorted(times, np.asarray(t0) + 1e-12, side="right") - 1
+        return lo, (n_x - 1) / (hi - lo), table, np.clip(which, 0, len(times) - 1).astype(np.int32)
 
-    return _cached((id(leverage), "lev", n_x, t0.tobytes()), leverage, build)
+    return _ROWS.get(leverage, ("lev", n_x, np.asarray(t0).tobytes()), build)
 
 
-def _describe(model, t0: np.ndarray, n_x: int):
+@dataclass
+class _Name:
+    """One name of a plan: its kind code, constant variance, Heston slot, and its table rows (lo, inv_dx, rows) with
+    the row each step reads, when it reads a table."""
+
+    kind: int
+    cvar: float = 0.0
+    factor: tuple = _NO_FACTOR
+    rows: tuple | None = None
+    step_rows: np.ndarray | None = None
+
+
+def _name_of(model, t0: np.ndarray, n_x: int) -> _Name:
     if isinstance(model, BlackScholesModel):
-        return _KIND["const"], model.vol**2, (0.0, 1.0, 1.0, 0.0, 0.0), None, None
+        return _Name(_KIND["const"], cvar=model.vol**2)
     if isinstance(model, LocalVolModel):
-        sd = getattr(model, "sd_range", (-8.0, 6.0))
-        lo, inv, rows = local_var_rows(model.surface, t0, n_x, sd)
-        return _KIND["table"], 0.0, (0.0, 1.0, 1.0, 0.0, 0.0), (lo, inv, rows), np.arange(len(t0), dtype=np.int32)
-    if isinstance(model, SLVModel):
+        rows = local_var_rows(model.surface, t0, n_x, getattr(model, "sd_range", SD_RANGE))
+        return _Name(_KIND["table"], rows=rows, step_rows=np.arange(len(t0), dtype=np.int32))
+    if isinstance(model, (HestonModel, SLVModel)):
         h = model.heston
-        lo, inv, rows, step = leverage_rows(model.leverage, t0, n_x)
-        return _KIND["slv"], 0.0, (h.v0, h.kappa, h.theta, h.xi, h.rho), (lo, inv, rows), step
-    if isinstance(model, HestonModel):
-        h = model.heston
-        return _KIND["heston"], 0.0, (h.v0, h.kappa, h.theta, h.xi, h.rho), None, None
+        factor = (h.v0, h.kappa, h.theta, h.xi, h.rho)
+        if isinstance(model, HestonModel):
+            return _Name(_KIND["heston"], factor=factor)
+        lo, inv, table, step = leverage_rows(model.leverage, t0, n_x)
+        return _Name(_KIND["slv"], factor=factor, rows=(lo, inv, table), step_rows=step)
     raise NotImplementedError(f"{type(model).__name__} is not in the path engine")
 
 
 _KNOWN = (BlackScholesModel, LocalVolModel, SLVModel, HestonModel)
 
 
+def _model_problem(model) -> str | None:
+    if not isinstance(model, _KNOWN):
+        return f"{type(model).__name__} is not in the path engine"
+    if isinstance(model, SLVModel) and not (isinstance(model.leverage, Leverage) or hasattr(model.leverage, "rows")):
+        return "this leverage is not in the path engine"
+    return None
+
+
 def unsupported(model) -> str | None:
-    """Why the engine cannot simulate this model or basket, or None (native/__init__.py:190-212). An addition: the
-    stochastic mixing weight (`StochasticMix`, localcorr.py:87-108), which the reference engine refuses, runs here."""
+    """None when the engine can simulate the model or basket, else why not (native/__init__.py:190-212). An
+    addition: the stochastic mixing weight (`StochasticMix`, localcorr.py:87-108), which the reference engine
+    refuses, runs here as mixer 2."""
     from optlab.structured.localcorr import LocalCorrelation, StochasticMix
     from optlab.structured.multiasset import Basket
 
-    if isinstance(model, Basket):
-        if len(model.names) > MAX_NAMES:
-            return f"more than {MAX_NAMES} names"
-        bad = [n for n, m in model.models.items() if not isinstance(m, _KNOWN)]
-        if bad:
-            return f"{', '.join(type(model.models[n]).__name__ for n in bad)} not in the path engine"
-        if model.mixer is not None and not isinstance(model.mixer, (LocalCorrelation, StochasticMix)):
-            return f"{type(model.mixer).__name__} is not in the path engine (local correlation and StochasticMix only)"
+    if not isinstance(model, Basket):
+        return _model_problem(model)
+    if len(model.names) > MAX_NAMES:
+        return f"more than {MAX_NAMES} names"
+    foreign = [type(model.models[n]).__name__ for n in model.names if not isinstance(model.models[n], _KNOWN)]
+    if foreign:
+        return f"{', '.join(foreign)} not in the path engine"
+    mixer = model.mixer
+    if mixer is None or isinstance(mixer, (LocalCorrelation, StochasticMix)):
         return None
-    if not isinstance(model, _KNOWN):
-        return f"{type(model).__name__} is not in the path engine"
-    if isinstance(model, SLVModel) and not isinstance(model.leverage, Leverage) and not hasattr(model.leverage, "rows"):
-        return "this leverage is not in the path engine"
-    return None
+    return f"{type(mixer).__name__} is not in the path engine (local correlation and StochasticMix only)"
 
 
 @dataclass
@@ -260,82 +307,95 @@ class Plan:
         return len(self.dt)
 
 
-def plan(model, market, times, dt_max=1 / 252, dt_min=1 / 2000, n_x: int = N_X) -> Plan:
-    """The simulation of one model or a Basket as arrays for the engine (native/__init__.py:243-302)."""
+def _members(model, market):
+    """(names, models, markets, correlation root, same-variance-driver flag) of a model or a Basket."""
     from optlab.structured.multiasset import Basket
 
-    why = unsupported(model)
-    if why:
-        raise NotImplementedError(why)
-    times = np.asarray(times, dtype=float)
-    if isinstance(model, Basket):
-        names, models = tuple(model.names), [model.models[n] for n in model.names]
-        markets = [model.markets[n] for n in model.names]
-        root, same = np.asarray(model.root, dtype=float), int(model.vol_corr == "same")
+    if not isinstance(model, Basket):
+        return ("_",), [model], [market], np.eye(1), 1
+    names = tuple(model.names)
+    return (names, [model.models[n] for n in names], [model.markets[n] for n in names],
+            np.asarray(model.root, dtype=float), int(model.vol_corr == "same"))
+
+
+def _mixer_fields(model):
+    """(mixer code, the 8 mixer values, low-matrix factors) of a basket's mixer (none: code 0). Code 1, local
+    correlation: (w0, level, down, up, sigma, market sign, cos, sin of the common factor's turn); code 2, the
+    stochastic weight: (w0, theta, nu, rho, kappa, market sign, cos, sin)."""
+    from optlab.structured.localcorr import StochasticMix, aligned_factors
+
+    mx = getattr(model, "mixer", None)
+    if mx is None:
+        return 0, (0.0,) * 8, None
+    low, sign, turn = aligned_factors(model.corr, model.root, mx.w0)
+    turn_cs = (np.cos(turn), np.sin(turn))
+    if isinstance(mx, StochasticMix):
+        code, head = 2, (mx.w0, mx.theta, mx.nu, mx.rho, mx.kappa)
     else:
-        names, models, markets, root, same = ("_",), [model], [market], np.eye(1), 1
-    if all(isinstance(m, BlackScholesModel) for m in models):
-        dt_max, dt_min = np.inf, None
-    grid = time_grid(times, dt_max, dt_min)
-    obs = np.searchsorted(grid, times).astype(np.int32)
+        code, head = 1, (mx.w0, mx.level, mx.down, mx.up, mx.sigma)
+    return code, tuple(float(v) for v in (*head, sign, *turn_cs)), np.ascontiguousarray(low, dtype=float)
+
+
+def plan(model, market, times, dt_max=1 / 252, dt_min=1 / 2000, n_x: int = N_X) -> Plan:
+    """The simulation of one model (`models.simulate`) or of a Basket (`simulate_basket`) as the engine's arrays
+    (native/__init__.py:243-302). A grid of Black-Scholes names only steps straight from one observation to the
+    next (their step is exact)."""
+    problem = unsupported(model)
+    if problem:
+        raise NotImplementedError(problem)
+    times = np.asarray(times, dtype=float)
+    names, models, markets, root, same = _members(model, market)
+    exact = all(isinstance(m, BlackScholesModel) for m in models)
+    grid = time_grid(times, np.inf, None) if exact else time_grid(times, dt_max, dt_min)
     dt, t0 = np.diff(grid), grid[:-1].copy()
-    k = len(models)
-    kind, cvar, hes = np.empty(k, np.int32), np.empty(k), np.empty((k, 5))
-    lo, inv, tab, rows_of = [], [], [], np.zeros((k, len(dt)), np.int32)
-    offset = 0
-    for a, m in enumerate(models):
-        kind[a], cvar[a], hes[a], rows, step = _describe(m, t0, n_x)
-        if rows is not None:
-            lo.append(rows[0]), inv.append(rows[1]), tab.append(rows[2])
-            rows_of[a] = step + offset
-            offset += len(rows[0])
-    fwd = np.stack([np.asarray(mk.F(times), dtype=float) for mk in markets]) if len(times) else np.zeros((k, 0))
-    mixer, mix, low = 0, (0.0,) * 8, None
-    if isinstance(model, Basket) and model.mixer is not None:
-        from optlab.structured.localcorr import aligned_factors
-
-        from optlab.structured.localcorr import StochasticMix
-
-        mx = model.mixer
-        low, sign, turn = aligned_factors(model.corr, model.root, mx.w0)
-        if isinstance(mx, StochasticMix):  # mixer 2: (w0, theta, nu, rho, kappa, market sign, cos, sin of the turn)
-            mixer, mix = 2, (mx.w0, mx.theta, mx.nu, mx.rho, mx.kappa, sign, np.cos(turn), np.sin(turn))
-        else:
-            mixer, mix = 1, (mx.w0, mx.level, mx.down, mx.up, mx.sigma, sign, np.cos(turn), np.sin(turn))
-        low = np.ascontiguousarray(low, dtype=float)
+    parts = [_name_of(m, t0, n_x) for m in models]
+    tabulated = [q for q in parts if q.rows is not None]
+    step_row = None
+    if tabulated:
+        # a name's steps point at its own rows in the stacked table
+        step_row = np.zeros((len(parts), len(dt)), np.int32)
+        base = 0
+        for a, q in enumerate(parts):
+            if q.rows is not None:
+                step_row[a] = q.step_rows + base
+                base += len(q.rows[0])
+    fwd = np.zeros((len(parts), 0))
+    if len(times):
+        fwd = np.stack([np.asarray(mk.F(times), dtype=float) for mk in markets])
+    mixer, mix, low = _mixer_fields(model)
     return Plan(
-        names,
-        kind,
-        cvar,
-        np.ascontiguousarray(hes),
-        np.ascontiguousarray(root),
-        same,
-        dt,
-        t0,
-        obs,
-        np.ascontiguousarray(fwd),
-        rows_of if tab else None,
-        np.concatenate(lo) if lo else None,
-        np.concatenate(inv) if inv else None,
-        np.ascontiguousarray(np.concatenate(tab)) if tab else None,
-        mixer,
-        tuple(float(x) for x in mix),
-        low,
-        n_x,
+        names=names,
+        kind=np.array([q.kind for q in parts], np.int32),
+        cvar=np.array([q.cvar for q in parts], dtype=float),
+        heston=np.ascontiguousarray([q.factor for q in parts], dtype=float),
+        root=np.ascontiguousarray(root),
+        same=same,
+        dt=dt,
+        t0=t0,
+        obs_step=np.searchsorted(grid, times).astype(np.int32),
+        fwd=np.ascontiguousarray(fwd),
+        step_row=step_row,
+        lo=np.concatenate([q.rows[0] for q in tabulated]) if tabulated else None,
+        inv_dx=np.concatenate([q.rows[1] for q in tabulated]) if tabulated else None,
+        table=np.ascontiguousarray(np.concatenate([q.rows[2] for q in tabulated])) if tabulated else None,
+        mixer=mixer,
+        mix=mix,
+        low_root=low,
+        n_x=n_x,
     )
 
 
 def paths_request(p: Plan, n_pairs: int, seed: int, first_pair: int = 0, keep=None, windows=()):
     """(request, keep-alive arrays, n_keep) of a run: the optlab_paths_request of include/optlab_mi.h over the
     plan's arrays. `out` is left null; the caller points it at host memory or passes a device buffer."""
-    keep = np.arange(p.n_obs) if keep is None else np.asarray(keep, dtype=int)
-    if len(windows) > MAX_WINDOWS:
-        raise ValueError(f"at most {MAX_WINDOWS} windows")
-    col_of = np.full(p.n_obs, -1, np.int32)
-    col_of[keep] = np.arange(len(keep), dtype=np.int32)
-    win_lo = np.array([w[0] for w in windows], np.int32)
-    win_hi = np.array([w[1] for w in windows], np.int32)
-    n_cols = len(keep) + 2 * len(windows)
+    kept = np.arange(p.n_obs) if keep is None else np.asarray(keep, dtype=int)
+    spans = np.asarray(list(windows), dtype=np.int32).reshape(-1, 2)  # (first, last) observation of each window
+    if len(spans) > MAX_WINDOWS:
+        raise ValueError(f"{len(spans)} windows asked: a run follows at most {MAX_WINDOWS}")
+    col_of = -np.ones(p.n_obs, np.int32)  # output column of each observation, -1 where it is not kept
+    col_of[kept] = np.arange(kept.size, dtype=np.int32)
+    win_lo, win_hi = np.ascontiguousarray(spans[:, 0]), np.ascontiguousarray(spans[:, 1])
+    n_cols = kept.size + 2 * len(spans)
     arrays = {
         "dt": np.ascontiguousarray(p.dt, dtype=float),
         "t0": np.ascontiguousarray(p.t0, dtype=float),
@@ -364,9 +424,9 @@ def paths_request(p: Plan, n_pairs: int, seed: int, first_pair: int = 0, keep=No
     rq.same, rq.mixer = int(p.same), int(p.mixer)
     for i, v in enumerate(p.mix):
         rq.mix[i] = float(v)
-    rq.n_cols, rq.n_keep, rq.n_windows = n_cols, len(keep), len(windows)
+    rq.n_cols, rq.n_keep, rq.n_windows = n_cols, kept.size, len(spans)
     rq._keepalive = arrays  # the request points into these arrays: they live as long as it does
-    return rq, arrays, len(keep)
+    return rq, arrays, kept.size
 
 
 def run(p: Plan, n_pairs: int, seed: int, first_pair: int = 0, backend: str = "gpu", threads: int = 0, keep=None,
@@ -382,7 +442,8 @@ def run(p: Plan, n_pairs: int, seed: int, first_pair: int = 0, backend: str = "g
     _check(lib, lib.optlab_paths(C.byref(rq), prec, int(threads)))
     del keepalive
     spots = [out[a, :m].T for a in range(k)]
-    extremes = [[(out[a, m + 2 * w], out[a, m + 2 * w + 1]) for w in range(len(windows))] for a in range(k)]
+    ext = out[:, m:].reshape(k, -1, 2, 2 * n_pairs)  # (name, window, lowest / highest, row)
+    extremes = [[(ext[a, w, 0], ext[a, w, 1]) for w in range(ext.shape[1])] for a in range(k)]
     name = "gpu" if backend == "auto" else backend
     return spots, extremes, {"kernel_ms": rq.kernel_ms, "total_ms": rq.total_ms, "backend": name,
                              "precision": "fp64" if prec == 0 else "fp32"}
@@ -517,14 +578,15 @@ def leverage_request(params: HestonParams, surface, T_max: float, mix: float, n_
                      n_x, bandwidth_scale, rng, l_bounds, regression, positive_part, n_bins=400, n_fine=8192,
                      atm_vol=None, bandwidth=None):
     """(request, keep-alive arrays, t0) of the particle method, as native/__init__.py:460-495 fills it."""
-    if regression not in ("nadaraya_watson", "local_linear"):
-        raise ValueError(f"unknown regression {regression!r}, expected nadaraya_watson or local_linear")
-    grid = time_grid([T_max], dt_max, dt_min)
-    dt, t0 = np.diff(grid), grid[:-1].copy()
+    regressions = ("nadaraya_watson", "local_linear")
+    if regression not in regressions:
+        raise ValueError(f"regression must be one of {regressions}, not {regression!r}")
+    steps = time_grid([T_max], dt_max, dt_min)
+    dt, t0 = np.diff(steps), steps[:-1].copy()
     lv_lo, lv_inv, lv_rows = local_var_rows(surface, t0, N_X)
-    if atm_vol is None:
-        T1 = min(1.0, float(surface.maturities[-1]))
-        atm_vol = float(np.sqrt(surface.theta(T1) / T1))
+    if atm_vol is None:  # the ATM vol at one year, or at the last pillar when the surface ends sooner
+        horizon = min(1.0, float(surface.maturities[-1]))
+        atm_vol = float(np.sqrt(surface.theta(horizon) / horizon))
     arrays = {
         "dt": np.ascontiguousarray(dt),
         "t0": np.ascontiguousarray(t0),
