@@ -367,6 +367,18 @@ OPTLAB_API int optlab_leverage_ex(optlab_leverage_request* rq, const optlab_leve
    one maturity: params.hpp:27-40). */
 OPTLAB_API int optlab_strip(optlab_strip_request* rq, int backend);
 
+/* Addition with no counterpart in the reference: optlab_strip for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It calls no hipMalloc, hipFree or
+   other allocating routine: its device memory is one 16 MiB array per device, fixed at load time and held until the
+   process ends (the slot table, the totals, the chunk sums and the per-workgroup rows; the rows take what is left,
+   so a strip with many maturities runs fewer workgroups, and a request whose fixed parts do not fit is rejected).
+   Asynchronous on `stream` (a hipStream_t, NULL: the null stream): it neither waits nor copies to the host. The
+   estimates go to price_dev and se_dev, device arrays of n_mats x (n_strikes + 1) doubles laid out as rq->price;
+   rq->price, rq->se and the timings are not used. Every call on a device shares that device's array and leaves its
+   contents behind: calls that overlap on one device (two streams, two threads) race. */
+OPTLAB_API int optlab_strip_noalloc(const optlab_strip_request* rq, int backend, double* price_dev, double* se_dev,
+                                    void* stream);
+
 /* Addition: upload a surface's rows to the current device; *handle names it in later calls (the reference keeps
    the same rows per surface in a host cache, risk/surfaces.py:180-205). */
 OPTLAB_API int optlab_surface_upload(const optlab_surface_rows* rows, int* handle);

@@ -280,6 +280,21 @@ int optlab_strip(optlab_strip_request* rq, int backend) {
     });
 }
 
+int optlab_strip_noalloc(const optlab_strip_request* rq, int backend, double* price_dev, double* se_dev,
+                         void* stream) {
+    return guarded([&] {
+        if (!rq) invalid("null request");
+        if (!price_dev || !se_dev) invalid("null device output");
+        optlab_strip_request c = *rq;
+        c.price = price_dev;  // the checks ask for outputs; these are device arrays here
+        c.se = se_dev;
+        check_strip_request(c);
+        const int prec = precision_of(backend);
+        need_device();
+        run_strip_noalloc(c, prec, price_dev, se_dev, static_cast<hipStream_t>(stream));
+    });
+}
+
 int optlab_surface_upload(const optlab_surface_rows* rows, int* handle) {
     return guarded([&] {
         if (!rows || !handle) invalid("null argument");

@@ -22,6 +22,13 @@
 #include "device.hpp"
 #include "engine.hpp"
 
+// Device memory of the allocation-free strip (optlab_strip_noalloc): file scope, one copy on every device the
+// library runs on, held until the process ends. Laid out per call as [slot table][totals][chunk sums][per-block
+// rows]. Host code never passes this name to a kernel: it is a host-side stand-in, and the device address is what
+// hipGetSymbolAddress returns for the current device (noalloc_base below).
+constexpr size_t kStripNoAllocBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kStripNoAllocBytes];
+
 namespace optlab_mi {
 
 namespace {
@@ -532,10 +539,14 @@ __global__ void __launch_bounds__(256) reduce_totals_kernel(const double* chunks
     totals[slot] = a.value();
 }
 
-void reduce_rows(const double* partials, long long n_rows, int n_slots, double* totals, hipStream_t st) {
+// chunks: device memory for the chunk sums (kMaxChunks x n_slots x 2 doubles at most), or null for the cached
+// scratch slot.
+void reduce_rows(const double* partials, long long n_rows, int n_slots, double* totals, hipStream_t st,
+                 double* chunks = nullptr) {
     const long long chunk_rows = std::max<long long>(kMinChunkRows, (n_rows + kMaxChunks - 1) / kMaxChunks);
     const int n_chunks = static_cast<int>((n_rows + chunk_rows - 1) / chunk_rows);
-    double* chunks = static_cast<double*>(scratch(kSlotChunks, static_cast<size_t>(n_chunks) * n_slots * 2 * sizeof(double)));
+    if (!chunks)
+        chunks = static_cast<double*>(scratch(kSlotChunks, static_cast<size_t>(n_chunks) * n_slots * 2 * sizeof(double)));
     hipLaunchKernelGGL(reduce_chunks_kernel, dim3((n_slots + 63) / 64, n_chunks), dim3(kChunkThreads), 0, st, partials,
                        n_rows, n_slots, chunk_rows, chunks);
     hipLaunchKernelGGL(reduce_totals_kernel, dim3((n_slots + 255) / 256), dim3(256), 0, st, chunks, n_chunks, n_slots,
@@ -763,6 +774,127 @@ StripArgs<R> strip_args(const optlab_strip_request& rq) {
     return s;
 }
 
+// ---- the strip without allocation ------------------------------------------------------------------------------
+// What optlab_strip_noalloc needs beyond the strip's own kernels: the slot table built on the device from values
+// passed by value (no host-to-device copy), the estimates formed on the device into the caller's arrays, and a
+// per-device record of the file-scope array's address and of the strip kernels' occupancy (filled on the first call
+// on a device; no allocation after that, and none by the queries themselves).
+
+struct StripTableValues {  // 460 bytes: within a kernel's argument space
+    double K[kMaxStrikes];
+    double D[kMaxEvents];   // the discount to each maturity, exp(-r t_j), formed on the host as strip_args forms it
+    int mat_step[kMaxEvents];
+    int n_strikes, n_mats, put;
+};
+
+template <typename R>
+struct StripBlock {  // the layout strip_args copies to the device
+    StripSlot<R> slots[kMaxEvents * 32];
+    int mat_step[kMaxEvents];
+};
+static_assert(sizeof(StripBlock<double>) <= kStripTableBytes, "strip table layout");
+
+template <typename R>
+__global__ void __launch_bounds__(512) strip_table_kernel(StripTableValues v, StripBlock<R>* out) {
+    const int t = threadIdx.x;  // one thread per (maturity, slot): 16 x 32
+    const int e = t >> 5, k = t & 31;
+    if (e >= v.n_mats) return;
+    StripSlot<R> slot;
+    if (k < v.n_strikes)
+        slot = StripSlot<R>{static_cast<R>(v.K[k]), static_cast<R>(v.D[e]), v.put ? R(-1) : R(1)};
+    else if (k == v.n_strikes)
+        slot = StripSlot<R>{R(0), R(1), R(1)};  // the forward, undiscounted
+    else
+        slot = StripSlot<R>{R(0), R(0), R(1)};
+    out->slots[e * 32 + k] = slot;
+    if (k == 0) out->mat_step[e] = v.mat_step[e];
+}
+
+// price and SE of every (maturity, column) from the totals (cpu_mc.cpp:14-18), in the host estimate's operation
+// order and roundings (no contraction into fused multiply-adds)
+__global__ void __launch_bounds__(256) strip_estimate_kernel(const double* totals, int n_mats, int width,
+                                                             long long n_pairs, double* price, double* se) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_mats * width) return;
+    const int e = i / width, k = i - e * width;
+    const double sum = totals[e * 64 + k], sumsq = totals[e * 64 + 32 + k];
+    const double n = static_cast<double>(n_pairs), mean = sum / n;
+    const double var = sumsq / n - mean * mean;
+    price[i] = mean;
+    se[i] = n_pairs > 1 ? sqrt(fmax(var, 0.0) / (n - 1.0)) : 0.0;
+}
+
+constexpr int kNoAllocDevices = 64;
+
+struct NoAllocDevice {
+    unsigned char* base = nullptr;       // the device address of `scratch` on this device
+    int per_cu[2][2][kMaxEvents] = {};   // [precision][model][n_mats - 1]: resident strip workgroups a CU (0: unknown)
+};
+NoAllocDevice g_noalloc[kNoAllocDevices];
+
+NoAllocDevice& noalloc_device() {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kNoAllocDevices) invalid("device index beyond the allocation-free strip's table");
+    NoAllocDevice& d = g_noalloc[dev];
+    if (!d.base) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p) throw Error(OPTLAB_EDEVICE, "hipGetSymbolAddress(scratch) gave a null address");
+        d.base = static_cast<unsigned char*>(p);
+    }
+    return d;
+}
+
+constexpr size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+template <typename R>
+void strip_noalloc_on_device(NoAllocDevice& d, const StripArgs<R>& s_in, const StripTableValues& v, long long n_pairs,
+                             std::uint64_t seed, double* price, double* se, hipStream_t st) {
+    const std::uint32_t k0 = static_cast<std::uint32_t>(seed), k1 = static_cast<std::uint32_t>(seed >> 32);
+    const size_t lds = strip_lds_bytes<R>(s_in.n_mats);
+    const int n_slots = s_in.n_mats * 64;
+    int dev = 0, lds_max = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "attribute");
+    if (lds > static_cast<size_t>(lds_max))
+        invalid("too many maturities for one workgroup's LDS in this precision (" + std::to_string(lds) + " > " +
+                std::to_string(lds_max) + " bytes)");
+    int& per_cu = d.per_cu[sizeof(R) == 4][s_in.model][s_in.n_mats - 1];
+    if (!per_cu)
+        with_strip_kernel<R>(s_in.model, [&](auto kernel) {
+            hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, lds), "occupancy");
+        });
+    // the array's layout for this call; the per-block rows take what is left, which caps the grid
+    const size_t off_totals = align256(kStripTableBytes);
+    const size_t off_chunks = off_totals + align256(static_cast<size_t>(n_slots) * sizeof(double));
+    const size_t off_rows = off_chunks + align256(static_cast<size_t>(kMaxChunks) * n_slots * 2 * sizeof(double));
+    if (off_rows >= kStripNoAllocBytes) invalid("the strip does not fit the allocation-free strip's memory");
+    const long long max_rows = static_cast<long long>((kStripNoAllocBytes - off_rows) / (n_slots * sizeof(double)));
+    const long long want = (n_pairs + kBlock - 1) / kBlock;
+    const long long resident = static_cast<long long>(device_cu_count()) * std::max(per_cu, 1) * 8;
+    const long long grid = std::max<long long>(1, std::min({want, resident, max_rows}));
+    StripBlock<R>* table = reinterpret_cast<StripBlock<R>*>(d.base);
+    double* totals = reinterpret_cast<double*>(d.base + off_totals);
+    double* chunks = reinterpret_cast<double*>(d.base + off_chunks);
+    double* partials = reinterpret_cast<double*>(d.base + off_rows);
+    hipLaunchKernelGGL(strip_table_kernel<R>, dim3(1), dim3(kMaxEvents * 32), 0, st, v, table);
+    StripArgs<R> s = s_in;
+    s.slots = table->slots;
+    s.mat_step = table->mat_step;
+    with_strip_kernel<R>(s.model, [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(grid)), dim3(kBlock), lds, st, s, n_pairs, k0, k1,
+                           partials);
+    });
+    hip_check(hipGetLastError(), "strip kernel launch");
+    reduce_rows(partials, grid, n_slots, totals, st, chunks);
+    const int width = s.n_strikes + 1, n_out = s.n_mats * width;
+    hipLaunchKernelGGL(strip_estimate_kernel, dim3((n_out + 255) / 256), dim3(256), 0, st, totals, s.n_mats, width,
+                       n_pairs, price, se);
+    hip_check(hipGetLastError(), "strip estimate launch");
+}
+
 }  // namespace
 
 void check_price_request(const optlab_request& c) {  // requests.cpp:140-166, with product 3 added
@@ -843,6 +975,31 @@ void run_strip(optlab_strip_request& rq, int prec) {
         for (int k = 0; k < width; ++k)
             estimate(t[e * 64 + k], t[e * 64 + 32 + k], n_pairs, rq.price[e * width + k], rq.se[e * width + k]);
     rq.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+}
+
+void run_strip_noalloc(const optlab_strip_request& rq, int prec, double* price_dev, double* se_dev, hipStream_t st) {
+    NoAllocDevice& d = noalloc_device();
+    StripTableValues v{};
+    for (int k = 0; k < rq.n_strikes; ++k) v.K[k] = rq.strikes[k];
+    for (int e = 0; e < rq.n_mats; ++e) {
+        v.mat_step[e] = rq.mat_step[e];
+        v.D[e] = std::exp(-rq.r * (rq.T * rq.mat_step[e] / rq.n_steps));
+    }
+    v.n_strikes = rq.n_strikes, v.n_mats = rq.n_mats, v.put = rq.put;
+    const HostConsts hc = make_consts(rq.n_steps, rq.S0, rq.r, rq.q, rq.T, rq.sigma, rq.v0, rq.kappa, rq.theta,
+                                      rq.xi, rq.rho, 0.0, nullptr, nullptr, rq.psi_c);
+    const long long n_pairs = rq.n_paths / 2;
+    if (prec == 0) {
+        StripArgs<double> s{};
+        s.c = hc.as<double>();
+        s.model = rq.model, s.put = rq.put, s.n_strikes = rq.n_strikes, s.n_mats = rq.n_mats;
+        strip_noalloc_on_device(d, s, v, n_pairs, rq.seed, price_dev, se_dev, st);
+    } else {
+        StripArgs<float> s{};
+        s.c = hc.as<float>();
+        s.model = rq.model, s.put = rq.put, s.n_strikes = rq.n_strikes, s.n_mats = rq.n_mats;
+        strip_noalloc_on_device(d, s, v, n_pairs, rq.seed, price_dev, se_dev, st);
+    }
 }
 
 }  // namespace optlab_mi

@@ -368,6 +368,7 @@ _SIGNATURES = {
     "optlab_leverage": (_I, [C.POINTER(LeverageRequest), _I, _I]),
     "optlab_leverage_ex": (_I, [C.POINTER(LeverageRequest), C.POINTER(LeverageOptions), _I]),
     "optlab_strip": (_I, [C.POINTER(StripRequest), _I]),
+    "optlab_strip_noalloc": (_I, [C.POINTER(StripRequest), _I, C.c_void_p, C.c_void_p, C.c_void_p]),
     "optlab_surface_upload": (_I, [C.POINTER(SurfaceRows), _PI]),
     "optlab_surface_free": (_I, [_I]),
     "optlab_surface_table": (_I, [_I, _PD, _D, _D, _PD]),
