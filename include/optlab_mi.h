@@ -136,6 +136,21 @@ typedef struct {
     double bandwidth_coef;
 } optlab_leverage_options;
 
+/* ---- addition: the particle method's local variance as surface slices ------------------------------------- */
+/* Instead of the rows lv_rows of the request (the Dupire local variance tabulated at each step start, host data the
+   reference builds in NumPy, native/__init__.py:129-144), the surface at each step's maturity given as at most two
+   raw SVI slices w = c0 S(k; p0) + c1 S(k; p1), S(k; a, b, rho, m, s2) = a + b (rho (k - m) + sqrt((k - m)^2 + s2)),
+   with their derivatives in T; the engine evaluates the table entries it reads (the Dupire formula with Gatheral's g,
+   clipped to [floor, cap]) on the device, at k = j (hi - lo) / (n_lv - 1) + lo with lo = lv_lo[i] and the last point
+   hi[i], the entries of the table the rows would hold. Row i of records: c0, c1, dc0/dT, dc1/dT, p0, p1, dp0/dT,
+   dp1/dT (5 each: a, b, rho, m, s2). */
+typedef struct {
+    int n_rows;              /* = n_steps */
+    const double* hi;        /* host, n_rows */
+    const double* records;   /* host, n_rows x 24 */
+    double floor, cap;       /* the clip of the local variance (the reference: 1e-4, 4) */
+} optlab_lv_slices;
+
 /* ---- addition: a strip of vanillas on one simulation (Heston QE-M or Black-Scholes) ------------------------ */
 /* Calls (or puts) at n_strikes strikes and n_mats maturities t_j = T * mat_step[j] / n_steps, all on the same
    paths; price[j * (n_strikes + 1) + k] is strike k at maturity j, discounted by exp(-r t_j), and column
@@ -362,6 +377,12 @@ OPTLAB_API int optlab_leverage(optlab_leverage_request* rq, int backend, int n_t
 
 /* Addition: api::leverage with the options above (opt NULL: the reference's rule). */
 OPTLAB_API int optlab_leverage_ex(optlab_leverage_request* rq, const optlab_leverage_options* opt, int backend);
+
+/* Addition: optlab_leverage_ex with the local variance from surface slices (lv_rows may be NULL): the calibration
+   the reference runs through leverage() after building the rows on the host (native/__init__.py:460-495), with the
+   rows' host build and upload replaced by one record a step. opt may be NULL. */
+OPTLAB_API int optlab_leverage_slices(optlab_leverage_request* rq, const optlab_leverage_options* opt,
+                                      const optlab_lv_slices* lv, int backend);
 
 /* Addition: the strip of vanillas (no counterpart in the reference's fixed products, which price three legs at
    one maturity: params.hpp:27-40). */

@@ -270,6 +270,17 @@ int optlab_leverage_ex(optlab_leverage_request* rq, const optlab_leverage_option
     });
 }
 
+int optlab_leverage_slices(optlab_leverage_request* rq, const optlab_leverage_options* opt,
+                           const optlab_lv_slices* lv, int backend) {
+    return guarded([&] {
+        if (!rq || !lv) invalid("null request");
+        check_leverage_request(*rq, false);
+        const int prec = precision_of(backend);
+        need_device();
+        run_leverage(*rq, opt, prec, lv);
+    });
+}
+
 int optlab_strip(optlab_strip_request* rq, int backend) {
     return guarded([&] {
         if (!rq) invalid("null request");

@@ -82,8 +82,10 @@ void run_strip(optlab_strip_request& rq, int prec);
 void run_strip_noalloc(const optlab_strip_request& rq, int prec, double* price_dev, double* se_dev, hipStream_t st);
 
 // ---- SLV particles (leverage.hip) -----------------------------------------------------------------------------
-void check_leverage_request(const optlab_leverage_request& rq);
-void run_leverage(optlab_leverage_request& rq, const optlab_leverage_options* opt, int prec);
+void check_leverage_request(const optlab_leverage_request& rq, bool need_rows = true);
+// lvs: the local variance from slice records instead of rq.lv_rows (null: the rows)
+void run_leverage(optlab_leverage_request& rq, const optlab_leverage_options* opt, int prec,
+                  const optlab_lv_slices* lvs = nullptr);
 
 // ---- surfaces on the device (surface.hip) and the worst-of autocall (autocall.hip) -----------------------------
 int surface_upload(const optlab_surface_rows& rows);

@@ -50,11 +50,13 @@ def _slice_terms(p, dp, k):
     """S, S_k, S_kk and S_T of one slice (p, dp: (..., 5), broadcast against k)."""
     a, b, rho, m, s2 = (p[..., i] for i in range(5))
     da, db, drho, dm, ds2 = (dp[..., i] for i in range(5))
+    # The operations and their order are those of the device's evaluation (csrc/leverage.hip, slice_dupire), which
+    # reproduces these numbers bit for bit.
     x = k - m
     root = np.sqrt(x * x + s2)
     lin = rho * x + root
     s_k = b * (rho + x / root)
-    s_kk = b * s2 / root**3
+    s_kk = b * s2 / (root * root * root)
     # S_T from the partials dS/da = 1, dS/db = rho x + root, dS/drho = b x, dS/dm = -S_k, dS/ds2 = b / (2 root)
     s_t = da + db * lin + b * drho * x - s_k * dm + 0.5 * b * ds2 / root
     return a + b * lin, s_k, s_kk, s_t
@@ -168,8 +170,10 @@ class SliceSurface:
         return lo, (n_x - 1) / (hi - lo), self.local_var(k, T[:, None])
 
     def row_records(self, t0, n_x, sd_range=(-8.0, 6.0)):
-        """(T, lo, inv_dx, records): each row's grid and its slice record (`RowSlices.records`), for the device."""
+        """(lo, hi, inv_dx, records) of the engine's local-variance rows at step starts t0: each row's grid (the
+        rows of `local_var_rows` are the Dupire variance at k = j (hi - lo) / (n_x - 1) + lo, the last point hi) and
+        its slice record (`RowSlices.records`), from which the device evaluates any entry of the row."""
         T = np.maximum(np.asarray(t0, dtype=float), DAY)
         sd = np.sqrt(np.asarray(self.w(0.0, T), dtype=float))
         lo, hi = sd_range[0] * sd, sd_range[1] * sd
-        return T, lo, (n_x - 1) / (hi - lo), self.slices(T).records()
+        return lo, hi, (n_x - 1) / (hi - lo), self.slices(T).records()

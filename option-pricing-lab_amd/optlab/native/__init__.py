@@ -576,14 +576,27 @@ def regress(state, y, mean, scale, powers, knots=(), eval_state=None, backend: s
 
 def leverage_request(params: HestonParams, surface, T_max: float, mix: float, n_particles: int, dt_max, dt_min,
                      n_x, bandwidth_scale, rng, l_bounds, regression, positive_part, n_bins=400, n_fine=8192,
-                     atm_vol=None, bandwidth=None):
-    """(request, keep-alive arrays, t0) of the particle method, as native/__init__.py:460-495 fills it."""
+                     atm_vol=None, bandwidth=None, lv_source="rows"):
+    """(request, keep-alive arrays, t0) of the particle method, as native/__init__.py:460-495 fills it. lv_source
+    "rows": the Dupire rows built on the host (the reference's request); "slices": one slice record a step
+    (`row_records` of optlab.core's surfaces) for the device to evaluate, in rq._slices (lv_rows stays null);
+    "auto": slices where the surface has them."""
     regressions = ("nadaraya_watson", "local_linear")
     if regression not in regressions:
         raise ValueError(f"regression must be one of {regressions}, not {regression!r}")
+    if lv_source not in ("rows", "slices", "auto"):
+        raise ValueError(f"lv_source must be rows, slices or auto, not {lv_source!r}")
     steps = time_grid([T_max], dt_max, dt_min)
     dt, t0 = np.diff(steps), steps[:-1].copy()
-    lv_lo, lv_inv, lv_rows = local_var_rows(surface, t0, N_X)
+    use_slices = lv_source == "slices" or (lv_source == "auto" and hasattr(surface, "row_records"))
+    slices = None
+    if use_slices:
+        lv_lo, lv_hi, lv_inv, records = surface.row_records(t0, N_X)
+        lv_rows = None
+        slices = _abi.LvSlices()
+        slices.n_rows, slices.floor, slices.cap = len(t0), 1e-4, 4.0
+    else:
+        lv_lo, lv_inv, lv_rows = local_var_rows(surface, t0, N_X)
     if atm_vol is None:  # the ATM vol at one year, or at the last pillar when the surface ends sooner
         horizon = min(1.0, float(surface.maturities[-1]))
         atm_vol = float(np.sqrt(surface.theta(horizon) / horizon))
@@ -592,7 +605,7 @@ def leverage_request(params: HestonParams, surface, T_max: float, mix: float, n_
         "t0": np.ascontiguousarray(t0),
         "lv_lo": np.ascontiguousarray(lv_lo),
         "lv_inv_dx": np.ascontiguousarray(lv_inv),
-        "lv_rows": np.ascontiguousarray(lv_rows),
+        "lv_rows": None if lv_rows is None else np.ascontiguousarray(lv_rows),
         "x_grids": np.empty((len(t0), n_x)),
         "values": np.empty((len(t0), n_x)),
     }
@@ -606,8 +619,13 @@ def leverage_request(params: HestonParams, surface, T_max: float, mix: float, n_
     rq.n_lv = N_X
     for name, a in arrays.items():
         setattr(rq, name, _abi.ptr(a))
+    if slices is not None:
+        arrays["lv_hi"] = np.ascontiguousarray(lv_hi, dtype=float)
+        arrays["lv_records"] = np.ascontiguousarray(records, dtype=float)
+        slices.hi, slices.records = _abi.ptr(arrays["lv_hi"]), _abi.ptr(arrays["lv_records"])
     rq._keepalive = arrays
     rq._options = leverage_options(bandwidth)
+    rq._slices = slices
     return rq, arrays, t0
 
 
@@ -635,8 +653,10 @@ def calibrate_leverage(params, surface, market, T_max: float, mix: float = 1.0, 
     prec = precision_code(backend)
     rq, arrays, t0 = leverage_request(params, surface, T_max, mix, n_particles, dt_max, dt_min, n_x,
                                       bandwidth_scale, rng, l_bounds, regression, positive_part, n_bins=n_bins,
-                                      bandwidth=bandwidth)
-    if bandwidth is None:
+                                      bandwidth=bandwidth, lv_source="auto")
+    if rq._slices is not None:  # optlab.core's surfaces: the device evaluates the rows' entries from slice records
+        _check(lib, lib.optlab_leverage_slices(C.byref(rq), C.byref(rq._options), C.byref(rq._slices), prec))
+    elif bandwidth is None:
         _check(lib, lib.optlab_leverage(C.byref(rq), prec, int(threads)))
     else:
         _check(lib, lib.optlab_leverage_ex(C.byref(rq), C.byref(rq._options), prec))

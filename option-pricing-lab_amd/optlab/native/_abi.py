@@ -148,6 +148,12 @@ class LeverageOptions(C.Structure):
     _fields_ = [("bandwidth_rule", _I), ("bandwidth_coef", _D)]
 
 
+class LvSlices(C.Structure):
+    """optlab_lv_slices (addition: the particle method's local variance as surface slices)."""
+
+    _fields_ = [("n_rows", _I), ("hi", _PD), ("records", _PD), ("floor", _D), ("cap", _D)]
+
+
 class StripRequest(C.Structure):
     """optlab_strip_request (addition: a strip of vanillas on one simulation)."""
 
@@ -367,6 +373,7 @@ _SIGNATURES = {
     "optlab_paths_dev": (_I, [C.POINTER(PathsRequest), _I, C.c_void_p, C.c_void_p]),
     "optlab_leverage": (_I, [C.POINTER(LeverageRequest), _I, _I]),
     "optlab_leverage_ex": (_I, [C.POINTER(LeverageRequest), C.POINTER(LeverageOptions), _I]),
+    "optlab_leverage_slices": (_I, [C.POINTER(LeverageRequest), C.POINTER(LeverageOptions), C.POINTER(LvSlices), _I]),
     "optlab_strip": (_I, [C.POINTER(StripRequest), _I]),
     "optlab_strip_noalloc": (_I, [C.POINTER(StripRequest), _I, C.c_void_p, C.c_void_p, C.c_void_p]),
     "optlab_surface_upload": (_I, [C.POINTER(SurfaceRows), _PI]),

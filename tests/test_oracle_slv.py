@@ -70,3 +70,52 @@ def test_rule_one_equals_rule_zero_at_the_equivalent_scale_bit_for_bit():
     rq0.bandwidth_scale = COEF * sd1[1] * (1 + 1e-6)  # and a different scale moves the leverage
     _trace(rq0)
     assert not np.array_equal(got0["values"][1], got1["values"][1])
+
+
+def _record_entry(rec, k, floor=1e-4, cap=4.0):
+    """One table entry from a slice record, in the device's operation order (scalar IEEE doubles, no fused
+    multiply-add): what csrc/leverage.hip's slice_dupire computes."""
+    import math
+
+    acc = None
+    for j in range(2):
+        c, dc = rec[j], rec[2 + j]
+        a, b, rho, m, s2 = rec[4 + 5 * j: 9 + 5 * j]
+        da, db, drho, dm, ds2 = rec[14 + 5 * j: 19 + 5 * j]
+        x = k - m
+        root = math.sqrt(x * x + s2)
+        lin = rho * x + root
+        s_k = b * (rho + x / root)
+        s_kk = b * s2 / (root * root * root)
+        s_t = da + db * lin + b * drho * x - s_k * dm + 0.5 * b * ds2 / root
+        t = (c * (a + b * lin), c * s_k, c * s_kk, dc * (a + b * lin) + c * s_t)
+        acc = t if acc is None else tuple(u + v for u, v in zip(acc, t))
+    w, wk, wkk, wT = acc
+    u = 1.0 - 0.5 * k * wk / w
+    g = u * u - 0.25 * wk * wk * (1.0 / w + 0.25) + 0.5 * wkk
+    return min(max(wT / max(g, 1e-6), floor), cap)
+
+
+@pytest.mark.parametrize("which", ["ssvi", "svi"])
+def test_slice_records_give_the_host_table_bit_for_bit(which):
+    """The records the particle method hands the device (one a step) against the Dupire rows the host builds:
+    every sampled entry, evaluated in the device's order, is the table's number exactly -- the layout (weights,
+    slices, T-derivatives), the grid (j dx + lo, the last point hi) and the formula all pinned."""
+    from _support import simulated_context
+
+    if which == "ssvi":
+        _, surface, _ = bench.slv_inputs()
+        t0 = np.arange(0, 3 * 365) / 365.0
+    else:
+        _, surface, _ = simulated_context()
+        t0 = np.concatenate([np.arange(0, 400) / 200.0, [2.5, 3.1, 4.0]])  # before, between and past the pillars
+    lo, inv, rows = surface.local_var_rows(t0, native.N_X)
+    lo2, hi, inv2, rec = surface.row_records(t0, native.N_X)
+    np.testing.assert_array_equal(lo, lo2)
+    np.testing.assert_array_equal(inv, inv2)
+    rng = np.random.default_rng(3)
+    n = native.N_X
+    for i in rng.choice(len(t0), 40, replace=False).tolist() + [0, len(t0) - 1]:
+        for j in sorted(set(rng.choice(n, 25).tolist() + [0, 1, n // 2, n - 2, n - 1])):
+            k = hi[i] if j == n - 1 else j * ((hi[i] - lo[i]) / (n - 1)) + lo[i]
+            assert _record_entry(rec[i].tolist(), float(k)) == rows[i, j], (which, i, j)
