@@ -9,7 +9,9 @@
                   local-variance tables rebuilt on the device (risk/surfaces.py:141-165)
 
 `synthetic_book` draws the book of the benchmark: 512 trades from seed 7 (40 % European, 20 % Asian, 20 % daily
-barrier, 20 % worst-of autocall on 1 to 5 names; maturities uniform in 0.25..5 y; 2^14 to 2^17 paths by product)
+barrier, 20 % worst-of autocall on 1 to 5 names; maturities uniform in 0.25..5 y; 2^14 to 2^18 paths by product:
+a power of two drawn per trade in 2^14..2^16 for Europeans, 2^15..2^16 for Asians, 2^16..2^17 for barriers and
+2^17..2^18 for autocalls)
 on 5 names with SSVI surfaces, and 250 scenarios (spot N(0, 2 %) per name, parallel vol N(0, 1.5 pt), skew tilt
 N(0, 0.5 pt)) after the base scenario."""
 
@@ -85,12 +87,13 @@ def synthetic_book(seed: int = 7, n_trades: int = 512, n_names: int = 5, path_sc
     markets = [Market.flat(100.0, 0.03, float(q)) for q in np.linspace(0.0, 0.02, n_names)]
     corr = np.full((n_names, n_names), 0.6) + 0.4 * np.eye(n_names)
     kinds = rng.choice(4, size=n_trades, p=[0.4, 0.2, 0.2, 0.2])
-    paths = {0: 1 << 14, 1: 1 << 15, 2: 1 << 16, 3: 1 << 17}
+    log2_paths = {0: (14, 16), 1: (15, 16), 2: (16, 17), 3: (17, 18)}  # by product, both ends included
     trades = []
     for i, k in enumerate(kinds):
         T = float(rng.uniform(0.25, 5.0))
         qty = float(rng.choice([-1.0, 1.0]) * rng.integers(1, 11))
-        n_paths = max(2, paths[int(k)] // path_scale)
+        lo2, hi2 = log2_paths[int(k)]
+        n_paths = max(2, (1 << int(rng.integers(lo2, hi2 + 1))) // path_scale)
         if k == 3:
             every = STEPS_PER_YEAR // 4
             n_steps = max(1, round(T * 4)) * every

@@ -154,3 +154,65 @@ def test_benchmark_book_var_runs_end_to_end():
     q975 = -np.quantile(p, 0.025, method="lower")
     assert r["var"] > 0 and r["es"] >= q975 - 1e-12 and r["var"] >= q975 - 1e-12
     assert rev.name_steps > 0 and rev.kernel_ms > 0
+
+
+def test_benchmark_book_at_its_configuration():
+    """The benchmark's book at its configuration: 512 trades of seed 7 at 2^14-2^18 paths by product x 250 scenarios
+    and the base, FP32, one launch. Properties: every value finite and non-negative; the European calls at the base
+    scenario reprice their surface (local vol from the SSVI surface against Black-Scholes at its implied vol, within
+    4 SE and the daily step's bias); a trade's numbers do not depend on the rest of the batch (a sample revalued alone
+    with the same work split gives the same bits); the VaR is an actual scenario's loss."""
+    from optlab.core.blackscholes import black_price
+
+    book = synthetic_book()
+    paths = np.array([t.n_paths for t in book.trades])
+    assert paths.min() == 1 << 14 and paths.max() == 1 << 18
+    scen = synthetic_scenarios(len(book.surfaces), n=250)
+    print("benchmark book: 512 trades x 251 scenarios, FP32 batched launch", flush=True)
+    rev = revalue(book, scen)
+    assert rev.value.shape == (512, 251) and np.all(np.isfinite(rev.value)) and np.all(rev.value >= 0)
+    assert np.all(rev.se >= 0) and np.all(np.isfinite(rev.se))
+    gaps = []
+    for i, tr in enumerate(book.trades):
+        if tr.product != "european":
+            continue
+        a = tr.names[0]
+        T = tr.n_steps * book.dt
+        F, D = float(book.markets[a].F(T)), float(book.markets[a].D(T))
+        K = tr.strike * float(book.markets[a].F(0.0))
+        want = black_price(F, K, T, float(book.surfaces[a].iv(np.log(K / F), T)), D)
+        gaps.append((rev.value[i, 0] - want, rev.se[i, 0]))
+    gap, se = np.array(gaps).T
+    print(f"European calls: {len(gap)}, max |gap| / SE {np.max(np.abs(gap) / se):.2f}, max |gap| {np.max(np.abs(gap)):.4f}",
+          flush=True)
+    assert np.all(np.abs(gap) <= 4 * se + 0.03)
+    pick = [0, 1, 2, 3, 100, 257, 400, 511]
+    sub = Book(book.surfaces, book.markets, book.corr, [book.trades[i] for i in pick], book.n_steps, book.dt)
+    sub._device = book._device
+    alone = revalue(sub, scen, block=rev.block, pairs_per_lane=rev.pairs_per_lane)
+    np.testing.assert_array_equal(alone.value, rev.value[pick])
+    np.testing.assert_array_equal(alone.se, rev.se[pick])
+    r = var_es(pnl(book, rev))
+    assert r["var"] > 0 and r["var"] == -pnl(book, rev)[r["var_index"]]
+
+
+def test_benchmark_book_sampled_cells_are_the_oracle_in_fp64():
+    """FP64 parity with the oracle on sampled (trade, scenario) cells of the benchmark book at 256 pairs: a trade of
+    every product and an autocall on every name count present, under the base and two drawn scenarios."""
+    book = synthetic_book()
+    scen_all = synthetic_scenarios(len(book.surfaces), n=250)
+    rows = [0, 17, 203]
+    scen = Scenarios(scen_all.spot_shock[rows], scen_all.vol_shift[rows], scen_all.skew[rows])
+    pick, seen = [], set()
+    for i, tr in enumerate(book.trades):
+        key = (tr.product, len(tr.names))
+        if key not in seen:
+            seen.add(key)
+            pick.append(i)
+    assert {k[0] for k in seen} == {"european", "asian", "barrier", "autocall"}
+    from dataclasses import replace
+
+    trades = [replace(book.trades[i], n_paths=512) for i in pick]
+    sub = Book(book.surfaces, book.markets, book.corr, trades, book.n_steps, book.dt)
+    print(f"benchmark book: {len(trades)} sampled trades x 3 scenarios in FP64 against the oracle", flush=True)
+    _fp64_book_against_the_oracle("batched", 256, 1, sub, scen)
