@@ -114,6 +114,24 @@ def _request(backend_model, n_paths):
     return gpu.strip_request(model, market, STRIKES, steps, N_STEPS, T, n_paths, 42)
 
 
+def strip_vol_bp(p32, p64) -> float:
+    """The largest |implied vol(FP32 price) - implied vol(FP64 price)| over the strip's 21 x 8 calls, in basis
+    points (Black on the forward 100 e^{rT}, discount e^{-rT}); calls whose FP64 price has no implied vol are left
+    out."""
+    import numpy as np
+
+    from optlab.core.blackscholes import implied_vol
+
+    worst = 0.0
+    for j, t in enumerate(MATS):
+        F, D = 100.0 * math.exp(0.03 * t), math.exp(-0.03 * t)
+        for q, K in enumerate(STRIKES):
+            a, b = implied_vol(p32[j, q], F, K, t, D), implied_vol(p64[j, q], F, K, t, D)
+            if np.isfinite(a) and np.isfinite(b):
+                worst = max(worst, abs(a - b) * 1e4)
+    return worst
+
+
 def _time_gpu(lib, backend: int, steps: int, warmup: int):
     import torch
 
@@ -131,22 +149,54 @@ def _time_gpu(lib, backend: int, steps: int, warmup: int):
     return wall / steps * 1e3, sum(kernel_ms) / len(kernel_ms), out["price"].copy(), out["se"].copy()
 
 
-def _time_cpu(threads: int):
-    """The oracle (FP64, the reference CPU engine's arithmetic) on a bounded sample: the same strip on 2^18 paths."""
+def host_threads() -> dict:
+    """The host threads this process may use: the CPUs of its affinity mask (what nproc reports), capped by a cgroup
+    CPU quota where one is set; the CPU model beside it."""
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    threads = min(affinity, max(1, math.ceil(quota))) if quota else affinity
+    model = "unknown"
+    try:
+        for ln in Path("/proc/cpuinfo").read_text().splitlines():
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"threads": threads, "nproc": affinity, "cgroup_cpu_quota": quota, "cpu_model": model}
+
+
+def _time_cpu(threads: int, warmup_s: float = 2.0, min_rep_s: float = 0.3, reps: int = 5):
+    """The oracle (FP64, the reference CPU engine's arithmetic) on a bounded sample, the same strip on 2^18 paths, in
+    the protocol of the reference's benchmark (packages/native/cuda/bench/bench.cpp:117-123,181-204): warm-up for
+    at least 2 s, then 5 repetitions of at least 300 ms each; (median, min, max) path-steps/s."""
     sys.path.insert(0, str(ROOT / "tests"))
     from _support import oracle
 
     n = 1 << 18
     rq, out = _request(0, n)  # `out` owns the arrays rq points to
     lib = oracle()
-    assert lib.oracle_strip(C.byref(rq), threads) == 0  # warm-up (page faults, thread start)
-    reps = []
-    for _ in range(3):
-        t0 = time.perf_counter()
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < warmup_s:  # page faults, thread start, clocks
         assert lib.oracle_strip(C.byref(rq), threads) == 0
-        reps.append(time.perf_counter() - t0)
-    reps.sort()
-    return n * N_STEPS / reps[1], n
+    rates = []
+    for _ in range(reps):
+        calls, t0 = 0, time.perf_counter()
+        while True:
+            assert lib.oracle_strip(C.byref(rq), threads) == 0
+            calls += 1
+            dt = time.perf_counter() - t0
+            if dt >= min_rep_s:
+                break
+        rates.append(calls * n * N_STEPS / dt)
+    rates.sort()
+    return rates[len(rates) // 2], rates[0], rates[-1], n
 
 
 # ---- the worst-of autocall adjoint workload ----------------------------------------------------------------------
@@ -173,8 +223,9 @@ def autocall_note():
 
 def bench_adjoint(n_paths: int, seed: int = 42, reps: int = 3, bumps: bool = True) -> dict:
     """Adjoint Greeks (3 deltas + 240 node vegas) against bump-and-revalue of all 486 bumps on common random numbers,
-    both in FP32 on the same engine; the speedup at matched standard error is t_bump / t_adjoint times the median over
-    nodes of (SE_bump / SE_adjoint)^2 (the paths the bumps need for the adjoint's SE)."""
+    both in FP32 on the same engine; the speedup at matched standard error is, per node whose vega is beyond 3 SE,
+    t_bump / t_adjoint times (SE_bump / SE_adjoint)^2 (the paths the bumps need for the adjoint's SE): its median,
+    10th percentile and worst node."""
     import numpy as np
 
     note = autocall_note()
@@ -201,7 +252,8 @@ def bench_adjoint(n_paths: int, seed: int = 42, reps: int = 3, bumps: bool = Tru
     b = note.bump(n_paths, seed)
     t_bump = time.perf_counter() - t0
     live = np.abs(g.vega) > 3 * g.vega_se
-    ratio = float(np.median((b.vega_se[live] / g.vega_se[live]) ** 2)) if live.any() else float("nan")
+    node_ratio = (b.vega_se[live] / g.vega_se[live]) ** 2 if live.any() else np.array([np.nan])
+    ratio = float(np.median(node_ratio))
     moving = np.hypot(g.vega_se, b.vega_se) > 0  # nodes beyond the last date move nothing: both vegas are 0
     z = np.abs(g.vega - b.vega)[moving] / np.hypot(g.vega_se, b.vega_se)[moving]
     zd = np.abs(g.delta - b.delta) / np.hypot(g.delta_se, b.delta_se)
@@ -222,6 +274,10 @@ def bench_adjoint(n_paths: int, seed: int = 42, reps: int = 3, bumps: bool = Tru
         "speedup_time": t_bump / t_adj,
         "se_ratio_sq_median": ratio,
         "speedup_matched_se": t_bump / t_adj * ratio,
+        "speedup_matched_se_worst_node": t_bump / t_adj * float(np.min(node_ratio)),
+        "speedup_matched_se_p10_node": t_bump / t_adj * float(np.percentile(node_ratio, 10)),
+        "live_nodes": int(live.sum()),
+        "live_nodes_at_10x_matched_se": float(np.mean(t_bump / t_adj * node_ratio >= 10)),
         "moving_nodes": int(moving.sum()),
         "moving_nodes_within_4se": float(np.mean(z < 4)),
         "max_z_vega": float(z.max()),
@@ -567,6 +623,7 @@ def main(argv=None) -> int:
         "parity": {
             "max_abs_gap_fp32_vs_fp64": float(abs(p32[:, :k] - p64[:, :k]).max()),
             "max_gap_in_se": float(abs(gap).max()),
+            "max_abs_vol_bp_fp32_vs_fp64": strip_vol_bp(p32, p64),
             "forward_rel_err_fp32": float(max(abs(p32[j, k] / (100 * math.exp(0.03 * MATS[j])) - 1) for j in range(8))),
         },
         "roofline": {
@@ -582,15 +639,22 @@ def main(argv=None) -> int:
         },
     }
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        cpu_rate, n_cpu = _time_cpu(threads)
+        host = host_threads()
+        threads = host["threads"]
+        cpu_rate, cpu_min, cpu_max, n_cpu = _time_cpu(threads)
         line["cpu_baseline"] = {
             "value": cpu_rate,
             "unit": "path-steps/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"the same QE-M strip in FP64 on {n_cpu} paths x {N_STEPS} steps (median of 3 after a warm-up), "
-                      f"oracle/oracle.cpp on {threads} threads",
+            "sample": f"the same QE-M strip in FP64 on {n_cpu} paths x {N_STEPS} steps, oracle/oracle.cpp on every "
+                      f"host thread this process may use ({threads}); warm-up 2 s, median of 5 repetitions of at "
+                      "least 300 ms",
+            "min": cpu_min,
+            "max": cpu_max,
+            "nproc": host["nproc"],
+            "cgroup_cpu_quota": host["cgroup_cpu_quota"],
+            "cpu_model": host["cpu_model"],
         }
         line["gpu_over_cpu"] = v32 / cpu_rate
     if args.adjoint_paths:
