@@ -561,6 +561,9 @@ def main(argv=None) -> int:
     ap.add_argument("--only", choices=("adjoint", "book", "slv", "fixings", "lsm", "reference"), default=None,
                     help="run only this block and print its JSON (for profiling runs; the benchmark line needs all)")
     ap.add_argument("--no-bumps", action="store_true", help="with --only adjoint: skip the bump-and-revalue comparator")
+    ap.add_argument("--no-blocks", action="store_true",
+                    help="the strip only (the line's value, roofline and CPU baseline), none of the other workloads' "
+                         "blocks: for profiling runs of the strip kernel")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the timed region, one device-to-device copy of this many MiB: the known-size kernel "
                          "that scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -657,6 +660,9 @@ def main(argv=None) -> int:
             "cpu_model": host["cpu_model"],
         }
         line["gpu_over_cpu"] = v32 / cpu_rate
+    if args.no_blocks:
+        print(json.dumps(line), flush=True)
+        return 0
     if args.adjoint_paths:
         line["adjoint"] = bench_adjoint(args.adjoint_paths)
         print("bench: adjoint block done", file=sys.stderr, flush=True)
