@@ -50,6 +50,15 @@ STRIP_TRAFFIC_BYTES = int((480.2 * 2.0 + 8192.0) * 1024)
 # ---- the SLV particle workload (SSVI surface, Heston factor, uniform daily grid) --------------------------------
 
 SLV_PILLARS = [0.05, 0.1, 0.25, 0.5, 0.75, 1.0, 1.5, 2.0, 2.5, 3.0]
+# Algorithmic VALU lane-slots per particle-step of the SLV calibration (one particle, one step): half a Philox4x32-10
+# block (20 32-bit multiplies at 4 slots + 30 integer ops = 110), half of a pair's Box-Muller (2 specials at 4 + 3
+# = 11), the Euler step (the leverage lerp's division and 2 square roots at 4 slots + 26 FP32 ops = 38) and the
+# binning (2 divisions at 4 + 12 FP32/FP64 ops = 20): 179. The per-step regression (81 points x 256 bins) adds
+# under 2 % at 2^20 particles and is left out.
+SLV_SLOTS_PER_PARTICLE_STEP = 179
+# Algorithmic HBM bytes a step with the particles in registers: the bin totals (8192 fine counts x 4 B + 4 x 256
+# fixed-point sums x 8 B) written once and read once, and the leverage row (81 x 2 doubles) written.
+SLV_BYTES_PER_STEP = 2 * (8192 * 4 + 4 * 256 * 8) + 81 * 16
 SLV_STRIKES = [70.0 + 3.0 * k for k in range(21)]
 SLV_MATS = [0.375 * (j + 1) for j in range(8)]  # 8 maturities to 3 y
 
@@ -354,6 +363,15 @@ def bench_slv(n_particles: int = 1 << 20, T: float = 3.0, n_paths: int = 1 << 20
                     f"{n_paths} FP32 paths",
         "calibration_s": t_cal,
         "particle_steps_per_s": n_particles * n_steps / t_cal,
+        "roofline": {
+            "bound": "latency (two grid-wide waits and two exchanges of the bins a step)",
+            "valu_lane_slots_per_particle_step": SLV_SLOTS_PER_PARTICLE_STEP,
+            "valu_frac": SLV_SLOTS_PER_PARTICLE_STEP * n_particles * n_steps / t_cal / LANE_SLOTS_PER_S,
+            "hbm_bytes_per_step": SLV_BYTES_PER_STEP,
+            "hbm_GB_per_s": SLV_BYTES_PER_STEP * n_steps / t_cal / 1e9,
+            "hbm_frac": SLV_BYTES_PER_STEP * n_steps / t_cal / 8e12,
+            "us_per_step": t_cal / n_steps * 1e6,
+        },
         "reprice_s": t_rep,
         "max_abs_vol_bp_k79_121": float(np.nanmax(np.abs(r["vol_bp"][:, inner]))),
         "max_abs_vol_bp_all": float(np.nanmax(np.abs(r["vol_bp"]))),
