@@ -244,7 +244,7 @@ __device__ __forceinline__ double adj_rcp(double x) { return 1.0 / x; }
 
 template <typename R, int N, int SEG>
 __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s, long long n_chunks, R* xbar,
-                                                               long long* partial) {
+                                                               unsigned long long* batch) {
     constexpr int kBwdBlock = bwd_block<R>();
     extern __shared__ unsigned char lds_raw[];
     unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds_raw);  // SEG x N x n_x
@@ -354,35 +354,21 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
             }
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < len * row_len; e += kBwdBlock) {
-            const int k = e / row_len, rest = e - k * row_len, a = rest / s.n_x, xx = rest - a * s.n_x;
-            partial[((blk * N + a) * s.n_steps + s0 + k) * s.n_x + xx] = static_cast<long long>(acc[e]);
-        }
-        __syncthreads();
-    }
-}
-
-// batch[e][b] = sum over the backward workgroups of batch b (in order) of partial[w][e], e < entries: the 16
-// batches of an entry are contiguous (one 128-byte line), as the node contraction reads them. A workgroup takes 256
-// consecutive entries: reads are coalesced over e, the tile is transposed in LDS and written out contiguously.
-__global__ void __launch_bounds__(256) table_batches_kernel(const long long* partial, long long entries,
-                                                           long long n_blocks, long long* batch) {
-    __shared__ long long tile[256 * kBatches];
-    for (long long base = static_cast<long long>(blockIdx.x) * 256; base < entries;
-         base += static_cast<long long>(gridDim.x) * 256) {
-        const long long e = base + threadIdx.x;
-        if (e < entries) {
-            for (int b = 0; b < kBatches; ++b) {
-                const long long w0 = b * n_blocks / kBatches, w1 = (b + 1) * n_blocks / kBatches;
-                unsigned long long t = 0;  // integer sums: any order gives the same bits, so loads may overlap
-#pragma unroll 8
-                for (long long w = w0; w < w1; ++w) t += static_cast<unsigned long long>(partial[w * entries + e]);
-                tile[threadIdx.x * kBatches + b] = static_cast<long long>(t);
+        // the segment's rows into this workgroup's batch table (batch b = the workgroups [b G / 16, (b + 1) G / 16)):
+        // integer atomics, so the batch's table does not depend on the order of its workgroups' additions
+        {
+            long long b = blk * kBatches / G;
+            while (b > 0 && b * G / kBatches > blk) --b;
+            while (b + 1 < kBatches && (b + 1) * G / kBatches <= blk) ++b;
+            unsigned long long* tb = batch + b * static_cast<long long>(N) * s.n_steps * s.n_x;
+            for (int e = threadIdx.x; e < len * row_len; e += kBwdBlock) {
+                const unsigned long long v = acc[e];
+                if (!v) continue;
+                const int k = e / row_len, rest = e - k * row_len, a = rest / s.n_x, xx = rest - a * s.n_x;
+                __hip_atomic_fetch_add(tb + (static_cast<long long>(a) * s.n_steps + s0 + k) * s.n_x + xx, v,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
-        __syncthreads();
-        const long long n_here = std::min<long long>(256, entries - base) * kBatches;
-        for (long long k = threadIdx.x; k < n_here; k += 256) batch[base * kBatches + k] = tile[k];
         __syncthreads();
     }
 }
@@ -563,18 +549,17 @@ void run_typed(optlab_autocall_request& rq, bool adjoint) {
         bwd_blocks = std::min<long long>(n_chunks, static_cast<long long>(std::max(per_cu, 1)) * device_cu_count());
         bwd_blocks = std::max<long long>(bwd_blocks, 1);
         const long long entries = static_cast<long long>(n) * ns * n_x;
-        long long* partial = static_cast<long long*>(
-            scratch(kSlotAdjoint, static_cast<size_t>(bwd_blocks) * entries * sizeof(long long)));
+        // the 16 batch tables, one after the other ([batch][name][step][x], 64-bit fixed point), filled by the
+        // backward kernel's atomics: zeroed first
         long long* batch = static_cast<long long*>(
             scratch(kSlotBatches, static_cast<size_t>(kBatches) * entries * sizeof(long long) +
                                       static_cast<size_t>(n) * kBatches * n_nodes * sizeof(double)));
         double* vega_dev = reinterpret_cast<double*>(batch + kBatches * entries);
+        hip_check(hipMemsetAsync(batch, 0, static_cast<size_t>(kBatches) * entries * sizeof(long long), stream),
+                  "clear batch tables");
         hipLaunchKernelGGL((autocall_backward<R, N, SEG>), dim3(static_cast<unsigned>(bwd_blocks)), dim3(kBwdBlock), lds,
-                           stream, s, n_chunks, xbar, partial);
+                           stream, s, n_chunks, xbar, reinterpret_cast<unsigned long long*>(batch));
         hip_check(hipGetLastError(), "autocall backward launch");
-        const unsigned rb = static_cast<unsigned>(std::min<long long>((entries + 255) / 256, 8192));
-        hipLaunchKernelGGL(table_batches_kernel, dim3(rb), dim3(256), 0, stream, partial, entries, bwd_blocks, batch);
-        hip_check(hipGetLastError(), "table batches launch");
         SurfaceSet set{};
         DerivSet dset{};
         set.n = n;
@@ -582,7 +567,7 @@ void run_typed(optlab_autocall_request& rq, bool adjoint) {
             set.s[a] = *rows[a];
             dset.d[a] = surface_node_derivatives(rq.surface[a], stream);
         }
-        launch_node_contraction(set, dset, grid, batch, kBatches, 1.0 / kFixed, vega_dev, stream);
+        launch_node_contraction(set, dset, grid, batch, kBatches, 1.0 / kFixed, vega_dev, stream, true);
         node_out.resize(static_cast<size_t>(n) * kBatches * n_nodes);
         hip_check(hipEventRecord(ev.e[2], stream), "event");
         hip_check(hipMemcpyAsync(node_out.data(), vega_dev, node_out.size() * sizeof(double), hipMemcpyDeviceToHost,

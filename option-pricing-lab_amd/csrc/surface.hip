@@ -233,10 +233,13 @@ __global__ void __launch_bounds__(256) node_derivative_kernel(SurfaceRows s, Ten
 // each on half the row, measured slower (667 us: fewer workgroups resident).
 constexpr int kContractThreads = 320;
 
+// The adjoint of entry (a, row, x) of batch b is tbar[((a n_rows + row) n_x + x) x_stride + b batch_stride]: batches
+// interleaved (x_stride = n_batches, batch_stride = 1) or one table after the other (x_stride = 1, batch_stride =
+// the entries of a table).
 template <typename Adj>
 __global__ void __launch_bounds__(kContractThreads) node_contraction_kernel(SurfaceSet set, DerivSet dset, int n_active,
-                                                               const Adj* tbar, int n_batches, double unit,
-                                                               double* out) {
+                                                               const Adj* tbar, int n_batches, long long x_stride,
+                                                               long long batch_stride, double unit, double* out) {
     const long long row = blockIdx.x;
     const int a = blockIdx.y;
     const int n_x = set.s[a].n_x;
@@ -244,16 +247,16 @@ __global__ void __launch_bounds__(kContractThreads) node_contraction_kernel(Surf
     for (int t = threadIdx.x; t < n_batches * n_active; t += blockDim.x) {
         const int b = t / n_active, na = t - b * n_active;
         const double* d = dset.d[a] + row * n_x * n_active + na;
-        const Adj* tb = tbar + (static_cast<long long>(a) * n_rows + row) * n_x * n_batches + b;
+        const Adj* tb = tbar + (static_cast<long long>(a) * n_rows + row) * n_x * x_stride + b * batch_stride;
         // four partial sums over x % 4 (loads in flight), added in a fixed order
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         int x = 0;
         for (; x + 4 <= n_x; x += 4) {
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                acc[u] += static_cast<double>(tb[static_cast<long long>(x + u) * n_batches]) * d[(x + u) * n_active];
+                acc[u] += static_cast<double>(tb[static_cast<long long>(x + u) * x_stride]) * d[(x + u) * n_active];
         }
-        for (; x < n_x; ++x) acc[x & 3] += static_cast<double>(tb[static_cast<long long>(x) * n_batches]) * d[x * n_active];
+        for (; x < n_x; ++x) acc[x & 3] += static_cast<double>(tb[static_cast<long long>(x) * x_stride]) * d[x * n_active];
         out[((static_cast<long long>(a) * n_batches + b) * n_rows + row) * n_active + na] =
             ((acc[0] + acc[1]) + (acc[2] + acc[3])) * unit;
     }
@@ -327,7 +330,7 @@ void launch_bumped_table(const SurfaceRows& s, const TentGrid& g, const double* 
 
 template <typename A>
 void contract(const SurfaceSet& set, const DerivSet& dset, const TentGrid& g, const A* tbar, int n_batches,
-              double unit, double* out, hipStream_t stream) {
+              double unit, double* out, hipStream_t stream, bool batch_major = false) {
     if (n_batches < 1 || n_batches > kMaxBatches) invalid("n_batches must be 1 to 16");
     if (set.n < 1 || set.n > kMaxSurfaces) invalid("1 to 5 surfaces");
     for (int a = 1; a < set.n; ++a)
@@ -338,8 +341,10 @@ void contract(const SurfaceSet& set, const DerivSet& dset, const TentGrid& g, co
         scratch(kSlotNodes, static_cast<size_t>(set.n) * n_batches * n_rows * n_active * sizeof(double)));
     const int threads = (n_batches * n_active + 63) / 64 * 64;  // a thread per (batch, node), whole waves
     if (threads > kContractThreads) invalid("at most 320 (batch, node) pairs a row");
+    const long long table_entries = static_cast<long long>(set.n) * n_rows * set.s[0].n_x;
+    const long long x_stride = batch_major ? 1 : n_batches, batch_stride = batch_major ? table_entries : 1;
     hipLaunchKernelGGL(node_contraction_kernel<A>, dim3(static_cast<unsigned>(n_rows), set.n), dim3(threads), 0,
-                       stream, set, dset, n_active, tbar, n_batches, unit, per_row);
+                       stream, set, dset, n_active, tbar, n_batches, x_stride, batch_stride, unit, per_row);
     hip_check(hipGetLastError(), "node contraction kernel launch");
     hipLaunchKernelGGL(node_sum_kernel, dim3(g.n_strikes * g.n_tenors, n_batches, set.n), dim3(64), 0, stream, set, g,
                        per_row, n_batches, out);
@@ -347,8 +352,8 @@ void contract(const SurfaceSet& set, const DerivSet& dset, const TentGrid& g, co
 }
 
 void launch_node_contraction(const SurfaceSet& set, const DerivSet& dset, const TentGrid& g, const long long* tbar,
-                             int n_batches, double unit, double* out, hipStream_t stream) {
-    contract(set, dset, g, tbar, n_batches, unit, out, stream);
+                             int n_batches, double unit, double* out, hipStream_t stream, bool batch_major) {
+    contract(set, dset, g, tbar, n_batches, unit, out, stream, batch_major);
 }
 
 // ---- the registry of uploaded surfaces ----------------------------------------------------------------------------

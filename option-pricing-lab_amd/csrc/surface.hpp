@@ -59,10 +59,12 @@ struct DerivSet {  // each surface's node derivatives (surface_node_derivatives)
 };
 
 // out[(a * n_batches + b) * n_nodes + node] = sum over (row, x) of tbar[a][row][x][b] * unit * dLV_a[row][x] /
-// dH_node at H = 0: the adjoint of surface a's table (n_rows x n_x entries, each with its n_batches batches
-// contiguous, fixed-point integers of weight `unit`) contracted with the table's node derivatives.
+// dH_node at H = 0: the adjoint of surface a's table (n_rows x n_x entries, fixed-point integers of weight `unit`)
+// contracted with the table's node derivatives. The batches of an entry are contiguous, or with batch_major the
+// batches' tables follow one another (tbar[b][a][row][x]).
 void launch_node_contraction(const SurfaceSet& set, const DerivSet& dset, const TentGrid& g, const long long* tbar,
-                             int n_batches, double unit, double* out, hipStream_t stream);
+                             int n_batches, double unit, double* out, hipStream_t stream,
+                             bool batch_major = false);
 
 // The rows of an uploaded surface (handle from surface_upload) and its unmoved local-variance table in precision
 // prec (0 double, 1 float), built on first use and kept with the surface.
