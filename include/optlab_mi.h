@@ -388,6 +388,15 @@ OPTLAB_API int optlab_leverage_slices(optlab_leverage_request* rq, const optlab_
    one maturity: params.hpp:27-40). */
 OPTLAB_API int optlab_strip(optlab_strip_request* rq, int backend);
 
+/* Addition: pathwise sensitivities of every price of the strip, on the strip's paths (no counterpart in the
+   reference, whose Greeks are central bumps on common random numbers: structured/greeks.py:164-313, a spot bump
+   scaling the paths). delta[j * (n_strikes + 1) + k] is d price / d S0 and vega[...] the derivative in the variance
+   parameter: v0 under Heston QE-M (through every step of the scheme), sigma under Black-Scholes; column n_strikes
+   holds those of E[S_{t_j}]. Each with its standard error (over pairs). rq->price and rq->se are not written;
+   kernel_ms and total_ms are. */
+OPTLAB_API int optlab_strip_greeks(optlab_strip_request* rq, int backend, double* delta, double* delta_se,
+                                   double* vega, double* vega_se);
+
 /* Addition with no counterpart in the reference: optlab_strip for callers that cannot allocate device memory (code
    under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It calls no hipMalloc, hipFree or
    other allocating routine: its device memory is one 16 MiB array per device, fixed at load time and held until the

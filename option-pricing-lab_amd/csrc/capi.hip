@@ -291,6 +291,21 @@ int optlab_strip(optlab_strip_request* rq, int backend) {
     });
 }
 
+int optlab_strip_greeks(optlab_strip_request* rq, int backend, double* delta, double* delta_se, double* vega,
+                        double* vega_se) {
+    return guarded([&] {
+        if (!rq) invalid("null request");
+        if (!delta || !delta_se || !vega || !vega_se) invalid("null output");
+        optlab_strip_request c = *rq;
+        c.price = delta;  // the checks ask for the price's outputs, which this call does not write
+        c.se = delta_se;
+        check_strip_request(c);
+        const int prec = precision_of(backend);
+        need_device();
+        run_strip_greeks(*rq, prec, delta, delta_se, vega, vega_se);
+    });
+}
+
 int optlab_strip_noalloc(const optlab_strip_request* rq, int backend, double* price_dev, double* se_dev,
                          void* stream) {
     return guarded([&] {

@@ -77,6 +77,10 @@ void check_price_request(const optlab_request& rq);
 void run_price(const optlab_request& rq, int prec, optlab_result& out);
 void check_strip_request(const optlab_strip_request& rq);
 void run_strip(optlab_strip_request& rq, int prec);
+// The strip's pathwise sensitivities: delta and d/dv0 (Heston) or d/dsigma (Black-Scholes), n_mats x (n_strikes + 1)
+// each with its standard error.
+void run_strip_greeks(optlab_strip_request& rq, int prec, double* delta, double* delta_se, double* vega,
+                      double* vega_se);
 // The strip in the file-scope device array of fixed.hip (no allocation), asynchronous on `st`: the estimates go to
 // price_dev and se_dev (device, n_mats x (n_strikes + 1) doubles each).
 void run_strip_noalloc(const optlab_strip_request& rq, int prec, double* price_dev, double* se_dev, hipStream_t st);

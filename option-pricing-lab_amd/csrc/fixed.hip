@@ -50,12 +50,16 @@ struct Consts {  // schemes.hpp:159-207, computed in double on the host and cast
     // theta (1 - E), 2 A and K1 + K3 / 2 for the FP32 step: the scalar unit has no float arithmetic, so a float
     // expression of kernel arguments formed in the step would cost a vector instruction every step
     R th1E, twoA, k1h;
+    // the derivatives in sigma of the Black-Scholes step's drift and volatility (-sigma dt, sqrt dt): the tangent of
+    // the path along sigma in the strip's sensitivities
+    R g_drift, g_vol;
 };
 
 struct HostConsts {
     double S0, D, bs_drift, bs_vol, ln_h, K[3];
     int leg[3], n_steps;
     double inv_n, v0, theta, mu_dt, E, c1, c2, K1, K2, K3, K4, A, K0p, psi_c;
+    double g_drift, g_vol;
     template <typename R>
     Consts<R> as() const {
         Consts<R> c{};
@@ -65,6 +69,7 @@ struct HostConsts {
         c.c1 = R(c1), c.c2 = R(c2), c.K1 = R(K1), c.K2 = R(K2), c.K3 = R(K3), c.K4 = R(K4), c.A = R(A);
         c.K0p = R(K0p), c.psi_c = R(psi_c);
         c.th1E = R(theta * (1 - E)), c.twoA = R(2 * A), c.k1h = R(K1 + 0.5 * K3);
+        c.g_drift = R(g_drift), c.g_vol = R(g_vol);
         return c;
     }
 };
@@ -78,6 +83,8 @@ HostConsts make_consts(int n_steps, double S0, double r, double q, double T, dou
     c.D = std::exp(-r * T);
     c.bs_drift = (r - q - 0.5 * sigma * sigma) * dt;
     c.bs_vol = sigma * std::sqrt(dt);
+    c.g_drift = -sigma * dt;
+    c.g_vol = std::sqrt(dt);
     c.ln_h = H > 0 ? std::log(H / S0) : 0.0;
     for (int k = 0; k < 3; ++k) c.K[k] = strikes ? strikes[k] : 0.0, c.leg[k] = legs ? legs[k] : 0;
     c.n_steps = n_steps;
@@ -476,6 +483,180 @@ __device__ __forceinline__ void strip_body(const StripArgs<R>& s, long long n_pa
     (void)wave;
 }
 
+// ---- pathwise sensitivities of the strip ------------------------------------------------------------------------
+// dPrice/dS0 and the derivative in the model's variance parameter (Heston: v0; Black-Scholes: sigma) of every price
+// of the strip, by forward (tangent) differentiation of each path: S_t = S0 exp(y_t) gives dS_t/dS0 = S_t / S0 (the
+// path in y does not depend on S0), and the tangent (dy, dv) of the state along v0 (sigma) is carried through every
+// step beside the state. A payoff D max(s (S - K), 0) differentiates to D 1{s (S - K) > 0} s dS. Many outputs and
+// two inputs make the tangent the cheap direction here: two more registers a leg and no stored path.
+
+// One QE-M step of (y, v) in qe_step's operations, with the tangent (ty, tv) of (y, v) along v0 (the step's
+// constants do not depend on v0). The exponential branch's v' = 0 region (1 - U >= 1 - p) has a zero tangent. The
+// switch between the branches at psi_c and the martingale correction's switch to K0p are steps of the scheme that a
+// pathwise derivative does not see: on paths that cross them under a bump the tangent and a finite difference part.
+template <typename R>
+__device__ __forceinline__ void qe_step_tangent(const Consts<R>& c, R& y, R& v, R& ty, R& tv, R zv, R zx) {
+    const R m = c.theta + (v - c.theta) * c.E;
+    const R psi = (v * c.c1 + c.c2) / (m * m);
+    const R dm = c.E * tv;
+    const R dpsi = (c.c1 * tv - R(2) * psi * m * dm) / (m * m);  // psi = s2 / m^2 with s2 = v c1 + c2
+    R vn, lin, w, arg, dvn, dlin, darg;
+    bool finite;
+    if (psi <= c.psi_c) {
+        const R inv = R(2) / psi;
+        const R root = r_sqrt(inv) * r_sqrt(r_max(inv - R(1), R(0)));
+        const R b2 = inv - R(1) + root;
+        const R a = m / (R(1) + b2), b = r_sqrt(b2);
+        vn = a * (b + zv) * (b + zv);
+        arg = R(1) - R(2) * c.A * a;
+        finite = c.A < R(1) / (R(2) * a);
+        lin = -c.A * b2 * a / arg;
+        w = R(0.5);
+        const R dinv = -inv * dpsi / psi;
+        // root = sqrt(inv^2 - inv): d root = (2 inv - 1) d inv / (2 root)
+        const R db2 = dinv + (root > R(0) ? (R(2) * inv - R(1)) * dinv / (R(2) * root) : R(0));
+        const R da = (dm - a * db2) / (R(1) + b2), db = db2 / (R(2) * b);
+        dvn = da * (b + zv) * (b + zv) + R(2) * a * (b + zv) * db;
+        darg = R(-2) * c.A * da;
+        dlin = (-c.A * (db2 * a + b2 * da) - lin * darg) / arg;
+    } else {
+        const R p = (psi - R(1)) / (psi + R(1)), beta = (R(1) - p) / m;
+        const R one_minus_u = R(0.5) * r_erfc(zv * R(0.70710678118654752));
+        const bool zero = one_minus_u >= R(1) - p;
+        vn = zero ? R(0) : r_log((R(1) - p) / one_minus_u) / beta;
+        arg = p + beta * (R(1) - p) / (beta - c.A);
+        finite = c.A < beta;
+        lin = R(0);
+        w = R(-1);
+        const R dp = R(2) * dpsi / ((psi + R(1)) * (psi + R(1)));
+        const R dbeta = (-dp - beta * dm) / m;
+        dvn = zero ? R(0) : (-dp / (R(1) - p) - vn * dbeta) / beta;
+        const R gap = beta - c.A;
+        darg = dp + (dbeta * (R(1) - p) - beta * dp) / gap - beta * (R(1) - p) * dbeta / (gap * gap);
+        dlin = R(0);
+    }
+    const R K0 = finite ? lin + w * r_log(arg) - (c.K1 + R(0.5) * c.K3) * v : c.K0p;
+    const R dK0 = finite ? dlin + w * darg / arg - (c.K1 + R(0.5) * c.K3) * tv : R(0);
+    const R s = c.K3 * v + c.K4 * vn;
+    const R sd = r_sqrt(r_max(s, R(0)));
+    y += c.mu_dt + K0 + c.K1 * v + c.K2 * vn + sd * zx;
+    ty += dK0 + c.K1 * tv + c.K2 * dvn + (s > R(0) ? (c.K3 * tv + c.K4 * dvn) * zx / (R(2) * sd) : R(0));
+    v = vn;
+    tv = dvn;
+}
+
+// The sums of one maturity's sensitivities (g 0: delta, 1: the variance parameter) over the wave's pairs, added to
+// the wave's accumulators acc + g * stride + lane in the layout of the price's (lanes 0-31: value sums of slots
+// 0-31, lanes 32-63: sums of squares). Called by the whole wave at once (the maturity steps are the same on every
+// lane); inactive lanes add zeros.
+template <typename R>
+__device__ __forceinline__ void greek_event_sums(const StripSlot<R>* row, R S0, R S1, R dS0, R dS1, R inv_S0,
+                                                 int n_values, bool active, double* acc_s, double* acc_c, int stride,
+                                                 int lane) {
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+        const R t0 = g == 0 ? S0 * inv_S0 : dS0, t1 = g == 0 ? S1 * inv_S0 : dS1;
+#pragma unroll
+        for (int chunk = 0; chunk < 4; ++chunk) {
+            const int first = (chunk & 1) * 16;
+            if (first >= n_values) continue;
+            double vals[16];
+#pragma unroll
+            for (int i2 = 0; i2 < 16; ++i2) {
+                const StripSlot<R>& t = row[first + i2];
+                const R o = (t.sign * (S0 - t.K) > R(0) ? t.D * t.sign * t0 : R(0)) +
+                            (t.sign * (S1 - t.K) > R(0) ? t.D * t.sign * t1 : R(0));
+                const double yk = active ? static_cast<double>(R(0.5) * o) : 0.0;
+                vals[i2] = chunk < 2 ? yk : yk * yk;
+            }
+            reduce_scatter16(vals, lane);
+            if ((lane >> 4) == chunk) {
+                Comp a{acc_s[g * stride + lane], acc_c[g * stride + lane]};
+                a.add(vals[0]);
+                acc_s[g * stride + lane] = a.s;
+                acc_c[g * stride + lane] = a.c;
+            }
+        }
+    }
+}
+
+// The strip's sensitivities: the strip's paths (one pair a lane, stream 0 of the pair) with the tangent carried
+// along; at each maturity the wave adds the pairs' sensitivities to its LDS accumulators. One row of 2 n_mats x 64
+// partial sums a block: [greek][maturity][slot sums, then sums of squares].
+template <typename R, int MODEL>
+__device__ __forceinline__ void strip_greeks_body(const StripArgs<R>& s, long long n_pairs, std::uint32_t k0,
+                                                  std::uint32_t k1, double* partials) {
+    extern __shared__ double strip_lds[];
+    __shared__ StripSlot<R> table[kMaxEvents * 32];
+    for (int t = threadIdx.x; t < s.n_mats * 32; t += blockDim.x) table[t] = s.slots[t];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int n_values = s.n_strikes + 1, n_mats = s.n_mats, stride = 64 * n_mats;
+    const int per_wave = 4 * stride;
+    double* acc_s = strip_lds + (threadIdx.x >> 6) * per_wave;  // [greek][maturity][64], then the compensations
+    double* acc_c = acc_s + 2 * stride;
+    for (int i = lane; i < 2 * stride; i += 64) acc_s[i] = 0.0, acc_c[i] = 0.0;
+    const Consts<R>& c = s.c;
+    const R inv_S0 = R(1) / c.S0;
+    const int n = c.n_steps;
+    constexpr int per_block = MODEL == 1 ? 2 : 4;  // steps a Philox block serves
+    const long long grid_stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long base = static_cast<long long>(blockIdx.x) * blockDim.x; base < n_pairs; base += grid_stride) {
+        const long long p = base + threadIdx.x;
+        const bool active = p < n_pairs;
+        const std::uint64_t pair = static_cast<std::uint64_t>(active ? p : 0);
+        const std::uint32_t p0 = static_cast<std::uint32_t>(pair), p1 = static_cast<std::uint32_t>(pair >> 32);
+        R y0 = R(0), y1 = R(0), v0 = c.v0, v1 = c.v0;
+        R ty0 = R(0), ty1 = R(0), tv0 = R(1), tv1 = R(1);  // d/dv0 (Heston) or d/dsigma (Black-Scholes)
+        int e = 0, next_event = s.mat_step[0];
+        R z[4] = {R(0), R(0), R(0), R(0)};
+        for (int i = 1; i <= n; ++i) {
+            const int j = (i - 1) & (per_block - 1);
+            if (j == 0) {
+                const U4 r = philox10(U4{p0, p1, static_cast<std::uint32_t>((i - 1) / per_block), 0u}, k0, k1);
+                gauss2(r.x, r.y, z[0], z[1]);
+                gauss2(r.z, r.w, z[2], z[3]);
+            }
+            if (MODEL == 1) {
+                const R zv = j == 0 ? z[0] : z[2], zx = j == 0 ? z[1] : z[3];
+                qe_step_tangent(c, y0, v0, ty0, tv0, zv, zx);
+                qe_step_tangent(c, y1, v1, ty1, tv1, -zv, -zx);
+            } else {
+                const R zb = j == 0 ? z[0] : j == 1 ? z[1] : j == 2 ? z[2] : z[3];
+                y0 += c.bs_drift + c.bs_vol * zb;
+                y1 += c.bs_drift + c.bs_vol * -zb;
+                ty0 += c.g_drift + c.g_vol * zb;
+                ty1 += c.g_drift + c.g_vol * -zb;
+            }
+            if (i == next_event) {
+                const R S0 = c.S0 * r_exp(y0), S1 = c.S0 * r_exp(y1);
+                greek_event_sums(table + e * 32, S0, S1, S0 * ty0, S1 * ty1, inv_S0, n_values, active,
+                                 acc_s + e * 64, acc_c + e * 64, stride, lane);
+                ++e;
+                next_event = e < n_mats ? s.mat_step[e] : -1;
+            }
+        }
+    }
+    // the block's waves in a fixed order
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        for (int r = 0; r < 2 * n_mats; ++r) {
+            Comp a{acc_s[r * 64 + lane], acc_c[r * 64 + lane]};
+            for (int w = 1; w < kBlock / 64; ++w) {
+                a.add(strip_lds[w * per_wave + r * 64 + lane]);
+                a.c += strip_lds[w * per_wave + 2 * stride + r * 64 + lane];
+            }
+            partials[(static_cast<long long>(blockIdx.x) * 2 * n_mats + r) * 64 + lane] = a.value();
+        }
+    }
+}
+
+template <typename R, int MODEL>
+__global__ void __launch_bounds__(kBlock) strip_greeks_kernel(StripArgs<R> s, long long n_pairs, std::uint32_t k0,
+                                                              std::uint32_t k1, double* partials) {
+    strip_greeks_body<R, MODEL>(s, n_pairs, k0, k1, partials);
+}
+
 // Separate FP32 and FP64 kernels, so an occupancy hint can be given to one without the other (OPTLAB_STRIP_WAVES:
 // minimum waves per SIMD asked of the FP32 kernel; 1 = the compiler's choice, measured fastest).
 template <int MODEL>
@@ -731,6 +912,36 @@ double strip_on_device(const StripArgs<R>& s, long long n_pairs, std::uint64_t s
     return timer.stop();
 }
 
+// The sensitivities' pass: as strip_on_device with the tangent kernel, whose rows hold 2 n_mats x 64 slots.
+template <typename R>
+double strip_greeks_on_device(const StripArgs<R>& s, long long n_pairs, std::uint64_t seed, double* totals,
+                              double* host_totals) {
+    const std::uint32_t k0 = static_cast<std::uint32_t>(seed), k1 = static_cast<std::uint32_t>(seed >> 32);
+    const size_t lds = static_cast<size_t>(kBlock / 64) * 4 * 64 * s.n_mats * sizeof(double);
+    const int n_slots = 2 * s.n_mats * 64;
+    int dev = 0, lds_max = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "attribute");
+    if (lds > static_cast<size_t>(lds_max))
+        invalid("too many maturities for one workgroup's LDS (" + std::to_string(lds) + " > " +
+                std::to_string(lds_max) + " bytes)");
+    auto run = [&](auto kernel) {
+        const unsigned grid = resident_grid(kernel, lds, n_pairs);
+        double* partials =
+            static_cast<double*>(scratch(kSlotPartials, static_cast<size_t>(grid) * n_slots * sizeof(double)));
+        Timer timer(nullptr);
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), lds, nullptr, s, n_pairs, k0, k1, partials);
+        hip_check(hipGetLastError(), "strip sensitivities kernel launch");
+        timer.mark();
+        reduce_rows(partials, static_cast<long long>(grid), n_slots, totals, nullptr);
+        hip_check(hipMemcpyAsync(host_totals, totals, static_cast<size_t>(n_slots) * sizeof(double),
+                                 hipMemcpyDeviceToHost, nullptr),
+                  "copy totals");
+        return timer.stop();
+    };
+    return s.model == 1 ? run(strip_greeks_kernel<R, 1>) : run(strip_greeks_kernel<R, 0>);
+}
+
 // The strip's constants on the host, and its (event, slot) table and event steps copied to device memory.
 template <typename R>
 StripArgs<R> strip_args(const optlab_strip_request& rq) {
@@ -974,6 +1185,29 @@ void run_strip(optlab_strip_request& rq, int prec) {
     for (int e = 0; e < rq.n_mats; ++e)
         for (int k = 0; k < width; ++k)
             estimate(t[e * 64 + k], t[e * 64 + 32 + k], n_pairs, rq.price[e * width + k], rq.se[e * width + k]);
+    rq.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+}
+
+void run_strip_greeks(optlab_strip_request& rq, int prec, double* delta, double* delta_se, double* vega,
+                      double* vega_se) {
+    const auto t_start = std::chrono::steady_clock::now();
+    const long long n_pairs = rq.n_paths / 2;
+    const int n_slots = 2 * rq.n_mats * 64;
+    double* totals = static_cast<double*>(scratch(kSlotTotals, static_cast<size_t>(n_slots) * sizeof(double)));
+    double* host = pinned_totals(static_cast<size_t>(n_slots));
+    rq.kernel_ms = prec == 0 ? strip_greeks_on_device(strip_args<double>(rq), n_pairs, rq.seed, totals, host)
+                             : strip_greeks_on_device(strip_args<float>(rq), n_pairs, rq.seed, totals, host);
+    const std::vector<double> t(host, host + n_slots);
+    const int width = rq.n_strikes + 1, n_mats = rq.n_mats;
+    for (int g = 0; g < 2; ++g) {
+        double* value = g == 0 ? delta : vega;
+        double* err = g == 0 ? delta_se : vega_se;
+        for (int e = 0; e < n_mats; ++e) {
+            const int row = (g * n_mats + e) * 64;
+            for (int k = 0; k < width; ++k)
+                estimate(t[row + k], t[row + 32 + k], n_pairs, value[e * width + k], err[e * width + k]);
+        }
+    }
     rq.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
 }
 

@@ -375,6 +375,7 @@ _SIGNATURES = {
     "optlab_leverage_ex": (_I, [C.POINTER(LeverageRequest), C.POINTER(LeverageOptions), _I]),
     "optlab_leverage_slices": (_I, [C.POINTER(LeverageRequest), C.POINTER(LeverageOptions), C.POINTER(LvSlices), _I]),
     "optlab_strip": (_I, [C.POINTER(StripRequest), _I]),
+    "optlab_strip_greeks": (_I, [C.POINTER(StripRequest), _I, _PD, _PD, _PD, _PD]),
     "optlab_strip_noalloc": (_I, [C.POINTER(StripRequest), _I, C.c_void_p, C.c_void_p, C.c_void_p]),
     "optlab_surface_upload": (_I, [C.POINTER(SurfaceRows), _PI]),
     "optlab_surface_free": (_I, [_I]),

@@ -205,10 +205,8 @@ def strip_request(model, market: Market, strikes, mat_steps, n_steps: int, T: fl
     return rq, out
 
 
-def strip(model, market: Market, strikes, maturities, T: float | None = None, dt_max=1 / 252, n_paths=1_000_000,
-          seed: int = 0, cp="call", psi_c=1.5, backend="auto") -> Strip:
-    """Vanillas at every (maturity, strike) on one simulation of uniform steps T / n_steps; every maturity must fall
-    on a step."""
+def _strip_steps(maturities, T, dt_max):
+    """(maturities, T, n_steps, the step of each maturity) of a strip of uniform steps T / n_steps."""
     maturities = np.asarray(maturities, dtype=float)
     T = float(maturities.max() if T is None else T)
     n_steps = max(1, math.ceil(T / dt_max - 1e-9))
@@ -216,6 +214,14 @@ def strip(model, market: Market, strikes, maturities, T: float | None = None, dt
     mat_steps = np.rint(steps).astype(np.int32)
     if np.any(np.abs(steps - mat_steps) > 1e-6):
         raise ValueError("every maturity must fall on a step of T / n_steps")
+    return maturities, T, n_steps, mat_steps
+
+
+def strip(model, market: Market, strikes, maturities, T: float | None = None, dt_max=1 / 252, n_paths=1_000_000,
+          seed: int = 0, cp="call", psi_c=1.5, backend="auto") -> Strip:
+    """Vanillas at every (maturity, strike) on one simulation of uniform steps T / n_steps; every maturity must fall
+    on a step."""
+    maturities, T, n_steps, mat_steps = _strip_steps(maturities, T, dt_max)
     name, prec = _choose(backend, _unsupported(model, market, T, cp, None))
     rq, out = strip_request(model, market, strikes, mat_steps, n_steps, T, n_paths, seed, cp, psi_c)
     ext = _ext()
@@ -223,3 +229,39 @@ def strip(model, market: Market, strikes, maturities, T: float | None = None, dt
     k = len(out["strikes"])
     return Strip(out["strikes"], maturities, out["price"][:, :k], out["se"][:, :k], out["price"][:, k],
                  out["se"][:, k], rq.kernel_ms, rq.total_ms, name)
+
+
+@dataclass
+class StripGreeks:
+    """Pathwise sensitivities of a strip's prices (index [maturity, strike]) on its own paths: delta = d price / d S0
+    and vega = d price / d `parameter` ("v0" under Heston, "sigma" under Black-Scholes), each with its standard error;
+    forward_* are those of E[S_{t_j}]."""
+
+    strikes: np.ndarray
+    maturities: np.ndarray
+    parameter: str
+    delta: np.ndarray
+    delta_se: np.ndarray
+    vega: np.ndarray
+    vega_se: np.ndarray
+    forward_delta: np.ndarray
+    forward_vega: np.ndarray
+    kernel_ms: float
+    total_ms: float
+    backend: str
+
+
+def strip_greeks(model, market: Market, strikes, maturities, T: float | None = None, dt_max=1 / 252,
+                 n_paths=1_000_000, seed: int = 0, cp="call", psi_c=1.5, backend="auto") -> StripGreeks:
+    """`strip`'s sensitivities on the same paths (same seed, steps and draws): the spot's by scaling the paths, the
+    variance parameter's by a tangent carried through every step (optlab_strip_greeks)."""
+    maturities, T, n_steps, mat_steps = _strip_steps(maturities, T, dt_max)
+    name, prec = _choose(backend, _unsupported(model, market, T, cp, None))
+    rq, out = strip_request(model, market, strikes, mat_steps, n_steps, T, n_paths, seed, cp, psi_c)
+    shape = out["price"].shape
+    d, dse, g, gse = (np.empty(shape) for _ in range(4))
+    ext = _ext()
+    _check(ext, ext.optlab_strip_greeks(C.byref(rq), prec, _abi.ptr(d), _abi.ptr(dse), _abi.ptr(g), _abi.ptr(gse)))
+    k = len(out["strikes"])
+    return StripGreeks(out["strikes"], maturities, "v0" if isinstance(model, HestonModel) else "sigma", d[:, :k],
+                       dse[:, :k], g[:, :k], gse[:, :k], d[:, k], g[:, k], rq.kernel_ms, rq.total_ms, name)
