@@ -240,6 +240,23 @@ typedef struct {
     int checkpoint_used;     /* out: the interval the adjoint ran with (0 for a price) */
 } optlab_autocall_request;
 
+/* ---- addition: the names' models of a worst-of autocall ----------------------------------------------------- */
+/* Per name: kind 0 constant variance cvar, 1 the surface's local variance (what optlab_autocall assumes for every
+   name), 2 Heston, 3 SLV: spot variance lev2(x, t) max(v, 0) with v a Heston variance (Euler, full truncation) --
+   the steps of paths.hpp:196-361 for these kinds, with the variance normals (n_names a step) drawn from the pair's
+   aux stream when any name is SLV, else from stream 0 after the step's spot normals, and correlated by root when
+   `same`. lev2 holds each SLV name's squared leverage on its surface's table grid (row s read at step s, entry j
+   at x = lo[s] + j / inv_dx[s]). Greeks: deltas for every kind (sticky moneyness); node vegas for kinds 1 and 3
+   (0 for the others), an SLV name's with its conditional expectation E[v | x] = LV / L^2 held at every entry, so
+   that a surface bump moves lev2 to lev2 LV_bumped / LV (the move a bumped price, nodes != NULL, makes too). */
+typedef struct {
+    const int* kind;         /* host, n_names */
+    const double* cvar;      /* host or NULL: n_names, kind 0's variance */
+    const double* heston;    /* host or NULL: n_names x 5 (v0, kappa, theta, xi, rho), kinds 2 and 3 */
+    const double* lev2;      /* host or NULL: n_names x n_steps x n_x, read for kind 3 names */
+    int same;
+} optlab_autocall_models;
+
 /* ---- addition: a book of trades revalued under scenarios, in one persistent launch --------------------------- */
 /* Trades on the names of one book (each name: a surface handle on the book's grid of uniform steps, and its
    forwards); paths by the local-vol step of paths.hpp:274-361 on stream 0 of each trade's own seed, so every
@@ -428,6 +445,11 @@ OPTLAB_API int optlab_autocall(optlab_autocall_request* rq, int backend);
 
 /* Addition: price, deltas and node vegas by the pathwise adjoint (nodes must be NULL). */
 OPTLAB_API int optlab_adjoint(optlab_autocall_request* rq, int backend);
+
+/* Addition: optlab_autocall (adjoint 0) or optlab_adjoint (adjoint 1) with each name's model; models NULL: every
+   name local vol. */
+OPTLAB_API int optlab_autocall_ex(optlab_autocall_request* rq, const optlab_autocall_models* models, int adjoint,
+                                  int backend);
 
 /* Addition: the book above (backend 0 double paths, 1 float paths; sums in double either way). */
 OPTLAB_API int optlab_book(optlab_book_request* rq, int backend);

@@ -72,9 +72,57 @@ struct AcArgs {
     R* lseed;             // n_legs: dV / dx of the lowest close's name at that close
     int* lat;             // n_legs: 8 step + name of the lowest close
     double* partials;     // forward workgroups x 16
+    // Variance state (the kernels' VAR form): name a's spot variance is its table entry times max(v_a, 0) with v_a a
+    // Heston variance stepped by Euler with full truncation (SLV: the table holds the squared leverage; Heston: a
+    // table of ones; constant variance: a table of sigma^2 with v_a frozen at 1). A local-vol name has v_a frozen at
+    // 1 (kappa = xi = 0, v0 = 1), so its steps are the local-vol steps exactly.
+    R hv0[kMaxNames], hkappa[kMaxNames], htheta[kMaxNames], hxi[kMaxNames], hrho[kMaxNames], hrhoc[kMaxNames];
+    int var_aux;                  // variance normals from the pair's aux stream (any SLV name), else from stream 0
+    int same;                     // variance normals correlated by root, as the spot normals
+    const R* ratio[kMaxNames];    // n_steps x n_x: d table / d local variance (SLV: L^2 / LV), or null: 1
+    int grad_mask;                // bit a: name a's table moves with its surface (local vol and SLV)
+    R* ckv;                       // checkpoints of v, laid out as ck
 };
 
-template <typename R, int N, int SEG>
+// The normals of one step: n spot normals from stream 0, then (VAR) n variance normals from the aux stream or
+// stream 0 -- the order of the path engine's steps -- correlated into z and w for the leg of sign `sign`.
+template <typename R, int N, bool VAR>
+__device__ __forceinline__ void step_normals(const AcArgs<R>& s, NormalStream<R>& rng, NormalStream<R>& aux, R sign,
+                                             R (&z)[N], R (&w)[N]) {
+    R zs[N], zr[N];
+#pragma unroll
+    for (int a = 0; a < N; ++a) zs[a] = rng.next();
+    if (VAR) {
+#pragma unroll
+        for (int a = 0; a < N; ++a) zr[a] = s.var_aux ? aux.next() : rng.next();
+    }
+    correlate<R, N>(s.root, sign, zs, z);
+    if (VAR) {
+        if (s.same) {
+            correlate<R, N>(s.root, sign, zr, w);
+        } else {
+#pragma unroll
+            for (int a = 0; a < N; ++a) w[a] = sign * zr[a];
+        }
+    }
+}
+
+// The spot variance of name a at this step from its interpolated table entry t (and v: max(v, 0)), and v's Euler step
+// (paths.hip: the variance is read before v moves).
+template <typename R, bool VAR>
+__device__ __forceinline__ R step_variance(const AcArgs<R>& s, int a, R t, R& v, R za, R wa, R& vp) {
+    if (!VAR) {
+        vp = R(1);
+        return t;
+    }
+    vp = r_max(v, R(0));
+    const R var = t * vp;
+    const R zv = s.hrho[a] * za + s.hrhoc[a] * wa;
+    v += s.hkappa[a] * (s.htheta[a] - vp) * s.dt + s.hxi[a] * r_sqrt(vp * s.dt) * zv;
+    return var;
+}
+
+template <typename R, int N, int SEG, bool VAR>
 __global__ void __launch_bounds__(kFwdBlock) autocall_forward(AcArgs<R> s) {
     __shared__ double wave_part[kFwdBlock / 64][16];
     const long long i = static_cast<long long>(blockIdx.x) * kFwdBlock + threadIdx.x;
@@ -82,12 +130,13 @@ __global__ void __launch_bounds__(kFwdBlock) autocall_forward(AcArgs<R> s) {
     const long long leg = active ? i : 0;
     const R sign = (i & 1) ? R(-1) : R(1);
     const int stride = s.n_steps + 1;
-    NormalStream<R> rng;
+    NormalStream<R> rng, aux;
     rng.init(static_cast<std::uint64_t>(s.first_pair + (leg >> 1)), s.key0, s.key1, 0u);
+    if (VAR) aux.init(static_cast<std::uint64_t>(s.first_pair + (leg >> 1)), s.key0, s.key1, kAuxTag);
 
-    R x[N];
+    R x[N], vs[N];
 #pragma unroll
-    for (int a = 0; a < N; ++a) x[a] = R(0);
+    for (int a = 0; a < N; ++a) x[a] = R(0), vs[a] = VAR ? s.hv0[a] : R(1);
     R ymin = ldc(s.lf, 0);
     int at = 0;
 #pragma unroll
@@ -100,18 +149,21 @@ __global__ void __launch_bounds__(kFwdBlock) autocall_forward(AcArgs<R> s) {
         if (s.greeks && step % SEG == 0 && active) {
             const long long c = step / SEG;
 #pragma unroll
-            for (int a = 0; a < N; ++a) s.ck[(c * N + a) * s.n_legs + i] = x[a];
+            for (int a = 0; a < N; ++a) {
+                s.ck[(c * N + a) * s.n_legs + i] = x[a];
+                if (VAR) s.ckv[(c * N + a) * s.n_legs + i] = vs[a];
+            }
         }
-        R zs[N], z[N];
-#pragma unroll
-        for (int a = 0; a < N; ++a) zs[a] = rng.next();
-        correlate<R, N>(s.root, sign, zs, z);
+        R z[N], wz[N];
+        step_normals<R, N, VAR>(s, rng, aux, sign, z, wz);
 #pragma unroll
         for (int a = 0; a < N; ++a) {
             const R* y = s.table[a] + static_cast<long long>(step) * s.n_x;
             const Lerp<R> l =
                 lerp_at(y, static_cast<R>(ldc(s.lo[a], step)), static_cast<R>(ldc(s.inv[a], step)), s.n_x, x[a]);
-            x[a] += R(-0.5) * l.var * s.dt + r_sqrt(l.var * s.dt) * z[a];
+            R vp;
+            const R var = step_variance<R, VAR>(s, a, l.var, vs[a], z[a], VAR ? wz[a] : R(0), vp);
+            x[a] += R(-0.5) * var * s.dt + r_sqrt(var * s.dt) * z[a];
         }
         const int t1 = step + 1;
         R yw = ldc(s.lf, t1) + x[0];
@@ -242,7 +294,7 @@ __device__ __forceinline__ long long to_fixed(R v) {
 __device__ __forceinline__ float adj_rcp(float x) { return hw_rcp(x); }
 __device__ __forceinline__ double adj_rcp(double x) { return 1.0 / x; }
 
-template <typename R, int N, int SEG>
+template <typename R, int N, int SEG, bool VAR>
 __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s, long long n_chunks, R* xbar,
                                                                unsigned long long* batch) {
     constexpr int kBwdBlock = bwd_block<R>();
@@ -261,46 +313,59 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
             const bool active = i < s.n_legs;
             const long long leg = active ? i : 0;
             const R sign = (i & 1) ? R(-1) : R(1);
-            R x[N], xb[N];
+            R x[N], xb[N], v[N];
 #pragma unroll
             for (int a = 0; a < N; ++a) {
                 x[a] = s.ck[(static_cast<long long>(seg) * N + a) * s.n_legs + leg];
+                v[a] = VAR ? s.ckv[(static_cast<long long>(seg) * N + a) * s.n_legs + leg] : R(1);
                 xb[a] = seg == n_seg - 1 ? R(0) : xbar[a * s.n_legs + leg];
             }
-            NormalStream<R> rng;
+            NormalStream<R> rng, aux;
             rng.init(static_cast<std::uint64_t>(s.first_pair + (leg >> 1)), s.key0, s.key1, 0u);
-            rng.block = static_cast<std::uint32_t>(seg * (SEG * N / 4));
+            // a segment's normals start at a Philox block: SEG n of stream 0 a segment (2 SEG n when the variance
+            // normals come from stream 0 too), SEG n of the aux stream
+            const int per_seg = (VAR && !s.var_aux ? 2 : 1) * (SEG * N / 4);
+            rng.block = static_cast<std::uint32_t>(seg * per_seg);
+            if (VAR) {
+                aux.init(static_cast<std::uint64_t>(s.first_pair + (leg >> 1)), s.key0, s.key1, kAuxTag);
+                aux.block = static_cast<std::uint32_t>(seg * (SEG * N / 4));
+            }
             // What the walk back needs of each (step, name). Up to 12 (step, name) pairs a segment it is kept from
-            // the recompute: the lerp's grid coordinate (its entry and weight), its slope and dx'/dvar =
-            // dt (-1/2 + z / (2 sqrt(var dt))), which spares the walk back its table reads. Beyond that the three
-            // values would not fit the 128 VGPRs of a 1024-lane workgroup: the state x and the normal z are kept
-            // instead and the walk back reads the table again.
+            // the recompute: the lerp's grid coordinate (its entry and weight), its slope and dx'/dt =
+            // v+ dt (-1/2 + z / (2 sqrt(var dt))) for the table entry t (var = t v+; v+ = 1 without a variance
+            // state), which spares the walk back its table reads. Beyond that the three values would not fit the
+            // 128 VGPRs of a 1024-lane workgroup: the state x, the normal z (and v+) are kept instead and the walk
+            // back reads the table again.
             constexpr bool kKeep = SEG * N <= 12;
-            constexpr int KS = kKeep ? SEG : 1, XS = kKeep ? 1 : SEG;
-            R lu[KS][N], ls[KS][N], lw[KS][N], xs[XS][N], zz[XS][N];
+            constexpr int KS = kKeep ? SEG : 1, XS = kKeep ? 1 : SEG, VS = (!kKeep && VAR) ? SEG : 1;
+            R lu[KS][N], ls[KS][N], lw[KS][N], xs[XS][N], zz[XS][N], vv[VS][N];
 #pragma unroll
             for (int k = 0; k < SEG; ++k) {
                 if (k >= len) break;
                 const int step = s0 + k;
-                R zs[N], z[N];
-#pragma unroll
-                for (int a = 0; a < N; ++a) zs[a] = rng.next();
-                correlate<R, N>(s.root, sign, zs, z);
+                R z[N], w[N];
+                step_normals<R, N, VAR>(s, rng, aux, sign, z, w);
 #pragma unroll
                 for (int a = 0; a < N; ++a) {
                     const R* y = s.table[a] + static_cast<long long>(step) * s.n_x;
                     const Lerp<R> l = lerp_at(y, static_cast<R>(ldc(s.lo[a], step)), static_cast<R>(ldc(s.inv[a], step)),
                                               s.n_x, x[a]);
-                    const R sd = r_sqrt(l.var * s.dt);
-                    if constexpr (kKeep) {
-                        lu[k][a] = l.u;
-                        ls[k][a] = l.slope;
-                        lw[k][a] = s.dt * (R(-0.5) + R(0.5) * z[a] * adj_rcp(sd));
-                    } else {
+                    if constexpr (!kKeep) {
                         xs[k][a] = x[a];
                         zz[k][a] = z[a];
                     }
-                    x[a] += R(-0.5) * l.var * s.dt + sd * z[a];
+                    R vp;
+                    const R var = step_variance<R, VAR>(s, a, l.var, v[a], z[a], VAR ? w[a] : R(0), vp);
+                    const R sd = r_sqrt(var * s.dt);
+                    if constexpr (kKeep) {
+                        lu[k][a] = l.u;
+                        ls[k][a] = l.slope;
+                        // a step with no variance (v+ = 0) does not move with its table entry: weight 0, not 0 inf
+                        lw[k][a] = sd > R(0) ? vp * (s.dt * (R(-0.5) + R(0.5) * z[a] * adj_rcp(sd))) : R(0);
+                    } else if constexpr (VAR) {
+                        vv[k][a] = vp;
+                    }
+                    x[a] += R(-0.5) * var * s.dt + sd * z[a];
                 }
             }
             const int lat = s.lat[leg];
@@ -333,15 +398,24 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
                         const R* y = s.table[a] + static_cast<long long>(step) * s.n_x;
                         const Lerp<R> l = lerp_at(y, static_cast<R>(ldc(s.lo[a], step)),
                                                   static_cast<R>(ldc(s.inv[a], step)), s.n_x, xs[k][a]);
+                        const R vp = VAR ? vv[VAR ? k : 0][a] : R(1);
+                        const R sd = r_sqrt(l.var * vp * s.dt);
                         u = l.u, slope = l.slope;
-                        wgt = s.dt * (R(-0.5) + R(0.5) * zz[k][a] * adj_rcp(r_sqrt(l.var * s.dt)));
+                        wgt = sd > R(0) ? vp * (s.dt * (R(-0.5) + R(0.5) * zz[k][a] * adj_rcp(sd))) : R(0);
                     }
-                    const R vbar = xb[a] * wgt;
+                    const R vbar = xb[a] * wgt;  // d V / d (the table entry interpolated at x)
                     const int i0 = static_cast<int>(u);
                     const R f = u - static_cast<R>(i0);
-                    if (active) {
+                    if (active && (!VAR || ((s.grad_mask >> a) & 1))) {
+                        // a table of the squared leverage moves with the local variance as L^2 / LV of each entry
+                        R r0 = R(1), r1 = R(1);
+                        if (VAR && s.ratio[a]) {
+                            const R* rr = s.ratio[a] + static_cast<long long>(s0 + k) * s.n_x + i0;
+                            r0 = rr[0];
+                            r1 = rr[1];  // i0 + 1 <= n_x - 1: the lerp keeps u below n_x - 1
+                        }
                         unsigned long long* r = acc + (k * N + a) * s.n_x + i0;
-                        const long long q0 = to_fixed(vbar * (R(1) - f)), q1 = to_fixed(vbar * f);
+                        const long long q0 = to_fixed(vbar * (R(1) - f) * r0), q1 = to_fixed(vbar * f * r1);
                         if (q0) atomicAdd(r, static_cast<unsigned long long>(q0));
                         if (q1) atomicAdd(r + 1, static_cast<unsigned long long>(q1));
                     }
@@ -371,6 +445,25 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
         }
         __syncthreads();
     }
+}
+
+// An SLV name's table from its squared leverage rows (lev2) and the surface's local variance: ratio = L^2 / LV of
+// every entry (the adjoint's d table / d LV), and the table itself -- lev2 at the unbumped surface, ratio times the
+// bumped local variance under a bump (the conditional expectation E[v | x] = LV / L^2 held).
+template <typename R>
+__global__ void slv_tables_kernel(const double* lev2, const double* lv_base, const R* lv_moved, R* ratio, R* table,
+                                  long long n) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double r = lev2[i] / lv_base[i];
+    if (ratio) ratio[i] = static_cast<R>(r);
+    table[i] = lv_moved ? static_cast<R>(r * static_cast<double>(lv_moved[i])) : static_cast<R>(lev2[i]);
+}
+
+template <typename R>
+__global__ void fill_table_kernel(R* table, R value, long long n) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) table[i] = value;
 }
 
 // ---- host -------------------------------------------------------------------------------------------------------
@@ -411,8 +504,10 @@ void mean_se(double sum, double sum2, double n, double& mean, double& se) {  // 
     se = std::sqrt(var / n);
 }
 
-template <typename R, int N, int SEG>
-void run_typed(optlab_autocall_request& rq, bool adjoint) {
+int model_kind(const optlab_autocall_models* m, int a) { return m ? m->kind[a] : 1; }
+
+template <typename R, int N, int SEG, bool VAR>
+void run_typed(optlab_autocall_request& rq, const optlab_autocall_models* models, bool adjoint) {
     static_assert(SEG % 4 == 0, "a segment's normals must fill whole Philox blocks");
     const auto wall0 = std::chrono::steady_clock::now();
     const int n = rq.n_names, ns = rq.n_steps;
@@ -440,10 +535,60 @@ void run_typed(optlab_autocall_request& rq, bool adjoint) {
     unsigned char* bump_mem = n_bumped ? static_cast<unsigned char*>(scratch(
                                              kSlotTables, n_bumped * (((table_bytes + 255) & ~size_t(255)) + node_bytes)))
                                        : nullptr;
-    for (int a = 0, b = 0; a < n; ++a) {
+    // the other kinds' tables: constants (kinds 0 and 2), and SLV's from its leverage rows with their ratios
+    const long long entries_a = static_cast<long long>(ns) * n_x;
+    const size_t real_b = (static_cast<size_t>(entries_a) * sizeof(R) + 255) & ~size_t(255);
+    const size_t dbl_b = (static_cast<size_t>(entries_a) * sizeof(double) + 255) & ~size_t(255);
+    int n_other = 0, n_slv = 0;
+    for (int a = 0; a < n; ++a) {
+        const int k = model_kind(models, a);
+        n_other += k != 1;
+        n_slv += k == 3;
+    }
+    unsigned char* model_mem =
+        n_other ? static_cast<unsigned char*>(scratch(kSlotModels, n_other * real_b + n_slv * (real_b + dbl_b)))
+                : nullptr;
+    const R* ratios[kMaxNames] = {};
+    const unsigned blocks_a = static_cast<unsigned>((entries_a + 255) / 256);
+    for (int a = 0, b = 0, m = 0, o = 0; a < n; ++a) {
         bool moved = false;
         if (rq.nodes)
             for (int k = 0; k < n_nodes; ++k) moved |= rq.nodes[a * n_nodes + k] != 0.0;
+        const int kind = model_kind(models, a);
+        if (kind == 0 || kind == 2) {
+            R* t = reinterpret_cast<R*>(model_mem + o++ * real_b);
+            const R value = kind == 0 ? static_cast<R>(models->cvar[a]) : R(1);
+            hipLaunchKernelGGL(fill_table_kernel<R>, dim3(blocks_a), dim3(256), 0, stream, t, value, entries_a);
+            hip_check(hipGetLastError(), "table fill launch");
+            tables[a] = t;
+            continue;
+        }
+        if (kind == 3) {
+            R* t = reinterpret_cast<R*>(model_mem + o++ * real_b);
+            unsigned char* at = model_mem + n_other * real_b + m++ * (real_b + dbl_b);
+            R* ratio = reinterpret_cast<R*>(at);
+            double* lev2 = reinterpret_cast<double*>(at + real_b);
+            hip_check(hipMemcpyAsync(lev2, models->lev2 + a * entries_a, entries_a * sizeof(double),
+                                     hipMemcpyHostToDevice, stream),
+                      "copy leverage rows");
+            const double* lv_base = static_cast<const double*>(surface_base_table(rq.surface[a], 0, stream));
+            const R* lv_moved = nullptr;
+            if (moved) {
+                unsigned char* bt = bump_mem + b++ * (((table_bytes + 255) & ~size_t(255)) + node_bytes);
+                double* d_nodes = reinterpret_cast<double*>(bt + ((table_bytes + 255) & ~size_t(255)));
+                hip_check(hipMemcpyAsync(d_nodes, rq.nodes + a * n_nodes, node_bytes, hipMemcpyHostToDevice, stream),
+                          "copy nodes");
+                launch_bumped_table(*rows[a], grid, d_nodes, SurfaceMove{0.0, 0.0},
+                                    std::is_same<R, double>::value ? 0 : 1, bt, stream);
+                lv_moved = reinterpret_cast<const R*>(bt);
+            }
+            hipLaunchKernelGGL(slv_tables_kernel<R>, dim3(blocks_a), dim3(256), 0, stream, lev2, lv_base, lv_moved,
+                               adjoint ? ratio : nullptr, t, entries_a);
+            hip_check(hipGetLastError(), "SLV table launch");
+            tables[a] = t;
+            ratios[a] = ratio;
+            continue;
+        }
         if (!moved) {
             tables[a] = static_cast<const R*>(surface_base_table(rq.surface[a], std::is_same<R, double>::value ? 0 : 1,
                                                                  stream));
@@ -496,6 +641,24 @@ void run_typed(optlab_autocall_request& rq, bool adjoint) {
                     static_cast<R>(rq.ki_barrier), static_cast<R>(rq.put_strike), static_cast<R>(rq.smoothing),
                     rq.memory};
     s.greeks = adjoint ? 1 : 0;
+    s.grad_mask = 0;
+    s.var_aux = 0;
+    s.same = models ? models->same : 0;
+    for (int a = 0; a < n; ++a) {
+        const int kind = model_kind(models, a);
+        if (kind == 1 || kind == 3) s.grad_mask |= 1 << a;
+        s.var_aux |= kind == 3;
+        s.ratio[a] = ratios[a];
+        if (kind == 2 || kind == 3) {
+            const double* h = models->heston + 5 * a;
+            s.hv0[a] = static_cast<R>(h[0]), s.hkappa[a] = static_cast<R>(h[1]), s.htheta[a] = static_cast<R>(h[2]);
+            s.hxi[a] = static_cast<R>(h[3]), s.hrho[a] = static_cast<R>(h[4]);
+            s.hrhoc[a] = static_cast<R>(std::sqrt(1.0 - h[4] * h[4]));
+        } else {  // a variance frozen at 1: the name's own table is its spot variance
+            s.hv0[a] = R(1), s.hkappa[a] = R(0), s.htheta[a] = R(0), s.hxi[a] = R(0), s.hrho[a] = R(0);
+            s.hrhoc[a] = R(1);
+        }
+    }
 
     const long long fwd_blocks = (n_legs + kFwdBlock - 1) / kFwdBlock;
     const int n_seg = (ns + SEG - 1) / SEG;
@@ -507,7 +670,9 @@ void run_typed(optlab_autocall_request& rq, bool adjoint) {
     s.partials = partials;
     R* xbar = nullptr;
     if (adjoint) {
-        s.ck = static_cast<R*>(scratch(kSlotCheckpoints, static_cast<size_t>(n_seg) * n * n_legs * sizeof(R)));
+        const size_t ck_b = up(static_cast<size_t>(n_seg) * n * n_legs * sizeof(R));
+        s.ck = static_cast<R*>(scratch(kSlotCheckpoints, (VAR ? 2 : 1) * ck_b));
+        s.ckv = VAR ? reinterpret_cast<R*>(reinterpret_cast<unsigned char*>(s.ck) + ck_b) : nullptr;
         const size_t perf_b = up(static_cast<size_t>(rq.n_obs) * n_legs * sizeof(R));
         const size_t worst_b = up(static_cast<size_t>(rq.n_obs) * n_legs);
         const size_t leg_b = up(static_cast<size_t>(n_legs) * sizeof(R));
@@ -523,7 +688,7 @@ void run_typed(optlab_autocall_request& rq, bool adjoint) {
 
     Events ev;
     hip_check(hipEventRecord(ev.e[0], stream), "event");
-    hipLaunchKernelGGL((autocall_forward<R, N, SEG>), dim3(static_cast<unsigned>(fwd_blocks)), dim3(kFwdBlock), 0, stream,
+    hipLaunchKernelGGL((autocall_forward<R, N, SEG, VAR>), dim3(static_cast<unsigned>(fwd_blocks)), dim3(kFwdBlock), 0, stream,
                        s);
     hip_check(hipGetLastError(), "autocall forward launch");
     hipLaunchKernelGGL(batch_sums_kernel, dim3(1), dim3(kBatches * 16), 0, stream, partials, fwd_blocks, batches);
@@ -539,11 +704,12 @@ void run_typed(optlab_autocall_request& rq, bool adjoint) {
         hip_check(hipGetDevice(&dev_id), "hipGetDevice");
         hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_id), "attribute");
         if (lds > static_cast<size_t>(max_lds)) invalid("n_x too large for the adjoint's LDS rows");
-        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(autocall_backward<R, N, SEG>),
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(autocall_backward<R, N, SEG, VAR>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
                   "LDS attribute");
         int per_cu = 0;
-        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, autocall_backward<R, N, SEG>, kBwdBlock, lds),
+        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, autocall_backward<R, N, SEG, VAR>, kBwdBlock,
+                                                               lds),
                   "occupancy");
         n_chunks = (n_legs + kBwdBlock - 1) / kBwdBlock;
         bwd_blocks = std::min<long long>(n_chunks, static_cast<long long>(std::max(per_cu, 1)) * device_cu_count());
@@ -557,7 +723,7 @@ void run_typed(optlab_autocall_request& rq, bool adjoint) {
         double* vega_dev = reinterpret_cast<double*>(batch + kBatches * entries);
         hip_check(hipMemsetAsync(batch, 0, static_cast<size_t>(kBatches) * entries * sizeof(long long), stream),
                   "clear batch tables");
-        hipLaunchKernelGGL((autocall_backward<R, N, SEG>), dim3(static_cast<unsigned>(bwd_blocks)), dim3(kBwdBlock), lds,
+        hipLaunchKernelGGL((autocall_backward<R, N, SEG, VAR>), dim3(static_cast<unsigned>(bwd_blocks)), dim3(kBwdBlock), lds,
                            stream, s, n_chunks, xbar, reinterpret_cast<unsigned long long*>(batch));
         hip_check(hipGetLastError(), "autocall backward launch");
         SurfaceSet set{};
@@ -634,11 +800,11 @@ constexpr bool seg_compiled(int seg) {
     return seg == 4 || (seg == 8 && N <= 3) || (seg == 16 && N == 1);
 }
 
-template <typename R, int N>
-void run_seg(optlab_autocall_request& rq, bool adjoint, int seg) {
-    if (seg == 8 && N <= 3) return run_typed<R, N, (N <= 3 ? 8 : 4)>(rq, adjoint);
-    if (seg == 16 && N == 1) return run_typed<R, N, (N == 1 ? 16 : 4)>(rq, adjoint);
-    return run_typed<R, N, 4>(rq, adjoint);
+template <typename R, int N, bool VAR>
+void run_seg(optlab_autocall_request& rq, const optlab_autocall_models* m, bool adjoint, int seg) {
+    if (seg == 8 && N <= 3) return run_typed<R, N, (N <= 3 ? 8 : 4), VAR>(rq, m, adjoint);
+    if (seg == 16 && N == 1) return run_typed<R, N, (N == 1 ? 16 : 4), VAR>(rq, m, adjoint);
+    return run_typed<R, N, 4, VAR>(rq, m, adjoint);
 }
 
 // Autotuned checkpoint interval per shape (precision, names, steps, legs, n_x): more steps a segment means fewer
@@ -649,7 +815,9 @@ void run_seg(optlab_autocall_request& rq, bool adjoint, int seg) {
 struct SegKey {
     int prec, n, n_steps, n_x;
     long long n_legs;
+    bool var;
     bool operator<(const SegKey& o) const {
+        if (var != o.var) return var < o.var;
         if (prec != o.prec) return prec < o.prec;
         if (n != o.n) return n < o.n;
         if (n_steps != o.n_steps) return n_steps < o.n_steps;
@@ -659,10 +827,10 @@ struct SegKey {
 };
 std::map<SegKey, int> g_seg;
 
-template <typename R, int N>
-int pick_seg(optlab_autocall_request& rq) {
+template <typename R, int N, bool VAR>
+int pick_seg(optlab_autocall_request& rq, const optlab_autocall_models* m) {
     const int n_x = static_cast<int>(surface_rows(rq.surface[0]).n_x);
-    const SegKey key{sizeof(R) == 4 ? 1 : 0, N, rq.n_steps, n_x, 2 * rq.n_pairs};
+    const SegKey key{sizeof(R) == 4 ? 1 : 0, N, rq.n_steps, n_x, 2 * rq.n_pairs, VAR};
     const auto it = g_seg.find(key);
     if (it != g_seg.end()) return it->second;
     int dev_id = 0, max_lds = 0;
@@ -672,14 +840,14 @@ int pick_seg(optlab_autocall_request& rq) {
     double best_ms = 0.0;
     {
         optlab_autocall_request t = rq;
-        run_seg<R, N>(t, true, 4);
+        run_seg<R, N, VAR>(t, m, true, 4);
     }
     for (const int seg : {4, 8, 16}) {
         if (!seg_compiled<N>(seg) || static_cast<size_t>(seg) * N * n_x * 8 > static_cast<size_t>(max_lds)) continue;
         double ms = 0.0;
         for (int rep = 0; rep < 3; ++rep) {
             optlab_autocall_request t = rq;
-            run_seg<R, N>(t, true, seg);
+            run_seg<R, N, VAR>(t, m, true, seg);
             const double m = t.forward_ms + t.backward_ms;
             ms = rep == 0 ? m : std::min(ms, m);
         }
@@ -689,23 +857,23 @@ int pick_seg(optlab_autocall_request& rq) {
     return best;
 }
 
-template <typename R, int N>
-void run_names(optlab_autocall_request& rq, bool adjoint) {
-    if (!adjoint) return run_typed<R, N, 4>(rq, false);  // a price takes no checkpoints
+template <typename R, int N, bool VAR>
+void run_names(optlab_autocall_request& rq, const optlab_autocall_models* m, bool adjoint) {
+    if (!adjoint) return run_typed<R, N, 4, VAR>(rq, m, false);  // a price takes no checkpoints
     int seg = rq.checkpoint_every;
-    if (seg == 0) seg = pick_seg<R, N>(rq);
+    if (seg == 0) seg = pick_seg<R, N, VAR>(rq, m);
     if (!seg_compiled<N>(seg)) invalid("checkpoint_every must be 0, 4, 8 (up to 3 names) or 16 (one name)");
-    run_seg<R, N>(rq, true, seg);
+    run_seg<R, N, VAR>(rq, m, true, seg);
 }
 
-template <typename R>
-void run_prec(optlab_autocall_request& rq, bool adjoint) {
+template <typename R, bool VAR>
+void run_prec(optlab_autocall_request& rq, const optlab_autocall_models* m, bool adjoint) {
     switch (rq.n_names) {
-        case 1: return run_names<R, 1>(rq, adjoint);
-        case 2: return run_names<R, 2>(rq, adjoint);
-        case 3: return run_names<R, 3>(rq, adjoint);
-        case 4: return run_names<R, 4>(rq, adjoint);
-        default: return run_names<R, 5>(rq, adjoint);
+        case 1: return run_names<R, 1, VAR>(rq, m, adjoint);
+        case 2: return run_names<R, 2, VAR>(rq, m, adjoint);
+        case 3: return run_names<R, 3, VAR>(rq, m, adjoint);
+        case 4: return run_names<R, 4, VAR>(rq, m, adjoint);
+        default: return run_names<R, 5, VAR>(rq, m, adjoint);
     }
 }
 
@@ -744,11 +912,35 @@ void check_autocall_request(const optlab_autocall_request& c, bool adjoint) {
     }
 }
 
-void run_autocall(optlab_autocall_request& rq, int prec, bool adjoint) {
-    if (prec == 0)
-        run_prec<double>(rq, adjoint);
-    else
-        run_prec<float>(rq, adjoint);
+void check_autocall_models(const optlab_autocall_request& c, const optlab_autocall_models& m) {
+    if (!m.kind) invalid("null array");
+    bool slv = false;
+    for (int a = 0; a < c.n_names; ++a) {
+        const int k = m.kind[a];
+        if (k < 0 || k > 3) invalid("kind must be 0 (constant), 1 (local vol), 2 (Heston) or 3 (SLV)");
+        if (k == 0 && !(m.cvar && m.cvar[a] > 0)) invalid("constant variance must be > 0");
+        if (k == 2 || k == 3) {
+            if (!m.heston) invalid("null array");
+            const double* h = m.heston + 5 * a;
+            if (!(h[0] >= 0 && h[1] >= 0 && h[2] >= 0 && h[3] >= 0 && std::fabs(h[4]) <= 1))
+                invalid("Heston parameters out of range");
+        }
+        slv |= k == 3;
+    }
+    if (slv && !m.lev2) invalid("null array");
+}
+
+void run_autocall(optlab_autocall_request& rq, int prec, bool adjoint, const optlab_autocall_models* models) {
+    bool var = false;
+    if (models)
+        for (int a = 0; a < rq.n_names; ++a) var |= models->kind[a] == 2 || models->kind[a] == 3;
+    if (prec == 0) {
+        if (var) run_prec<double, true>(rq, models, adjoint);
+        else run_prec<double, false>(rq, models, adjoint);
+    } else {
+        if (var) run_prec<float, true>(rq, models, adjoint);
+        else run_prec<float, false>(rq, models, adjoint);
+    }
 }
 
 }  // namespace optlab_mi

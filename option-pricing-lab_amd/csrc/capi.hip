@@ -372,6 +372,17 @@ int optlab_adjoint(optlab_autocall_request* rq, int backend) {
     });
 }
 
+int optlab_autocall_ex(optlab_autocall_request* rq, const optlab_autocall_models* models, int adjoint, int backend) {
+    return guarded([&] {
+        if (!rq) invalid("null request");
+        const int prec = precision_of(backend);
+        need_device();
+        check_autocall_request(*rq, adjoint != 0);
+        if (models) check_autocall_models(*rq, *models);
+        run_autocall(*rq, prec, adjoint != 0, models);
+    });
+}
+
 int optlab_book(optlab_book_request* rq, int backend) {
     return guarded([&] {
         if (!rq) invalid("null request");

@@ -41,6 +41,7 @@ enum Slot {
     kSlotLsm,          // Longstaff-Schwartz: spots and variances at the dates, values, partial sums, coefficients
     kSlotChunks,       // fixed products and the strip: compensated sums of chunks of the per-block rows
     kSlotRegress,      // the issuer's call regression: terms, states, sums, coefficients
+    kSlotModels,       // autocall: the tables of names that are not local vol, SLV leverage rows and ratios
     kSlotCount
 };
 
@@ -97,7 +98,9 @@ void surface_free(int handle);
 void surface_table(int handle, const double* nodes, double parallel, double skew, double* out_host);
 void surface_node_grad(int handle, const double* tbar_host, double* out_host);
 void check_autocall_request(const optlab_autocall_request& rq, bool adjoint);
-void run_autocall(optlab_autocall_request& rq, int prec, bool adjoint);
+void check_autocall_models(const optlab_autocall_request& rq, const optlab_autocall_models& m);
+// models: NULL for every name local vol
+void run_autocall(optlab_autocall_request& rq, int prec, bool adjoint, const optlab_autocall_models* models = nullptr);
 void check_book_request(const optlab_book_request& rq);
 void run_book(optlab_book_request& rq, int prec);
 

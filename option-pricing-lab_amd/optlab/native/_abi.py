@@ -148,6 +148,12 @@ class LeverageOptions(C.Structure):
     _fields_ = [("bandwidth_rule", _I), ("bandwidth_coef", _D)]
 
 
+class AutocallModels(C.Structure):
+    """optlab_autocall_models (addition: each name's model in the worst-of autocall)."""
+
+    _fields_ = [("kind", _PI), ("cvar", _PD), ("heston", _PD), ("lev2", _PD), ("same", _I)]
+
+
 class LvSlices(C.Structure):
     """optlab_lv_slices (addition: the particle method's local variance as surface slices)."""
 
@@ -383,6 +389,7 @@ _SIGNATURES = {
     "optlab_surface_node_grad": (_I, [_I, _PD, _PD]),
     "optlab_autocall": (_I, [C.POINTER(AutocallRequest), _I]),
     "optlab_adjoint": (_I, [C.POINTER(AutocallRequest), _I]),
+    "optlab_autocall_ex": (_I, [C.POINTER(AutocallRequest), C.POINTER(AutocallModels), _I, _I]),
     "optlab_book": (_I, [C.POINTER(BookRequest), _I]),
     "optlab_lsm": (_I, [C.POINTER(LsmRequest), _I]),
     "optlab_regress": (_I, [C.POINTER(RegressRequest), _I]),

@@ -1,6 +1,6 @@
-"""Worst-of autocall (Phoenix) under Dupire local vol on the MI355X engine: price, pathwise adjoint Greeks (delta per
-name, vega per node of the risk grid per surface) carried by a torch.autograd.Function, and the bump-and-revalue
-comparator on common random numbers.
+"""Worst-of autocall (Phoenix) on the MI355X engine, each name under Dupire local vol (the default), SLV, Heston or
+Black-Scholes: price, pathwise adjoint Greeks (delta per name, vega per node of the risk grid per surface) carried by
+a torch.autograd.Function, and the bump-and-revalue comparator on common random numbers.
 
     product      packages/structured/src/optlab/structured/compiler.py:114-161, 270-295 and presets.py:24-31
                  (phoenix: quarterly autocall at 100 %, coupon barrier 70 % with memory, put knocked in at 60 %),
@@ -11,7 +11,13 @@ comparator on common random numbers.
                  packages/risk/src/optlab/risk/greeks.py:199-244 with risk/surfaces.py:95-170 (node vegas: the
                  surface bumped by one tent of the grid, its local-variance table rebuilt, the paths simulated
                  again on the same seed)
-    paths        packages/native/cuda/include/optlab_cuda/paths.hpp:274-361 (local vol, n names, stream 0)
+    paths        packages/native/cuda/include/optlab_cuda/paths.hpp:196-361 (local vol, n names, stream 0; Heston
+                 and SLV variances by Euler with full truncation, their normals on the aux stream when a name is
+                 SLV)
+    SLV vegas    an SLV name's node vegas hold the conditional expectation E[v | x] = LV / L^2 of its calibration
+                 (packages/native/cuda/src/leverage.cpp:42-97 divides the local variance by it): a surface bump moves
+                 the squared leverage to L^2 LV_bumped / LV at every entry of the table, with no new calibration
+                 (the reference recalibrates: structured/greeks.py:164-313, `slv_vega`)
 
 Everything runs on the current HIP device through liboptlab_mi.so: the surfaces' rows are uploaded once
 (`optlab.risk.surfaces.DeviceSurface`), every bumped table is built on the device, and no step of the simulation or of
@@ -25,6 +31,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from optlab.core.engines import EngineError
+from optlab.core.models import BlackScholesModel, HestonModel, LocalVolModel, SLVModel
 from optlab.native import _abi, precision_code
 from optlab.risk import grid
 from optlab.risk.surfaces import DeviceSurface
@@ -80,10 +87,14 @@ def _batch_se(values: np.ndarray, legs: np.ndarray) -> tuple[float, float]:
 
 class WorstOfAutocall:
     """A worst-of Phoenix on len(surfaces) names (1 to 5): local vol from each name's implied surface, constant
-    correlation, uniform steps (`steps_per_year`), performances against `fixings` (default: the spots)."""
+    correlation, uniform steps (`steps_per_year`), performances against `fixings` (default: the spots).
+
+    `models` (optional, one a name; None keeps local vol): an SLVModel (its leverage resampled on the surface's
+    table grid, row of step i the leverage's last time row at or before t_i), a HestonModel or a
+    BlackScholesModel. `same`: the variance normals correlated by `corr` as the spot normals."""
 
     def __init__(self, surfaces, markets, corr, product: Phoenix = Phoenix(), steps_per_year: int = 252,
-                 n_x: int = 801, fixings=None):
+                 n_x: int = 801, fixings=None, models=None, same: bool = False):
         n = len(surfaces)
         if not 1 <= n <= 5 or len(markets) != n:
             raise ValueError("1 to 5 names, one market each")
@@ -104,6 +115,53 @@ class WorstOfAutocall:
         self.coupon = product.coupon_rate * product.frequency_months / 12
         self.n_names = n
         self._lib = _abi.load()
+        self.models = self._models(models, same) if models is not None else None
+
+    # ---- the names' models ---------------------------------------------------------------------------------------
+    def _lev2_rows(self, leverage, a: int) -> np.ndarray:
+        """The squared leverage on name a's table grid, (n_steps, n_x)."""
+        from optlab.native import _leverage_curves
+
+        curves = list(_leverage_curves(leverage))
+        times = np.asarray(leverage.times, dtype=float)
+        which = np.clip(np.searchsorted(times, self.times[:-1] + 1e-12, side="right") - 1, 0, len(times) - 1)
+        k = self.surfaces[a].k()
+        out = np.empty_like(k)
+        for i, w in enumerate(which):
+            _, _, x, y = curves[w]
+            out[i] = np.interp(k[i], x, y) ** 2
+        return out
+
+    def _models(self, models, same: bool) -> _abi.AutocallModels:
+        n = self.n_names
+        if len(models) != n:
+            raise ValueError("one model a name")
+        kind, cvar, heston = np.ones(n, np.int32), np.zeros(n), np.zeros((n, 5))
+        lev2 = None
+        for a, m in enumerate(models):
+            if m is None or isinstance(m, LocalVolModel):
+                continue
+            if isinstance(m, BlackScholesModel):
+                kind[a], cvar[a] = 0, m.vol**2
+            elif isinstance(m, (HestonModel, SLVModel)):
+                h = m.heston  # SLV: vol-of-vol scaled by the mix
+                heston[a] = (h.v0, h.kappa, h.theta, h.xi, h.rho)
+                kind[a] = 2
+                if isinstance(m, SLVModel):
+                    kind[a] = 3
+                    if lev2 is None:
+                        lev2 = np.zeros((n, self.n_steps, self.surfaces[a].n_x))
+                    lev2[a] = self._lev2_rows(m.leverage, a)
+            else:
+                raise EngineError(f"{type(m).__name__} is not a model of the autocall engine")
+        out = _abi.AutocallModels()
+        keep = [kind, cvar, heston] + ([lev2] if lev2 is not None else [])
+        out.kind, out.cvar, out.heston = _abi.ptr(kind, _abi._PI), _abi.ptr(cvar), _abi.ptr(heston)
+        out.lev2 = _abi.ptr(lev2) if lev2 is not None else None
+        out.same = int(same)
+        out._keepalive = keep
+        self.model_arrays = {"kind": kind, "cvar": cvar, "heston": heston, "lev2": lev2, "same": bool(same)}
+        return out
 
     # ---- requests ------------------------------------------------------------------------------------------------
     def request(self, n_paths: int, seed: int, first_pair: int = 0, nodes=None, spot=None) -> _abi.AutocallRequest:
@@ -131,8 +189,9 @@ class WorstOfAutocall:
         rq._keepalive = keep
         return rq
 
-    def _call(self, fn, rq) -> None:
-        code = fn(C.byref(rq), rq._prec)
+    def _call(self, adjoint: bool, rq) -> None:
+        models = C.byref(self.models) if self.models is not None else None
+        code = self._lib.optlab_autocall_ex(C.byref(rq), models, int(adjoint), rq._prec)
         if code:
             raise EngineError(self._lib.optlab_last_error().decode())
 
@@ -140,7 +199,7 @@ class WorstOfAutocall:
         """(price, se, the 16 batch prices, timings): one forward simulation."""
         rq = self.request(n_paths, seed, first_pair, nodes, spot)
         rq._prec = precision_code(backend)
-        self._call(self._lib.optlab_autocall, rq)
+        self._call(False, rq)
         return rq.price, rq.se, np.array(rq.batch[:]), {"forward_ms": rq.forward_ms, "total_ms": rq.total_ms}
 
     def adjoint(self, n_paths: int, seed: int, backend: str = "gpu", first_pair: int = 0,
@@ -154,7 +213,7 @@ class WorstOfAutocall:
         shape = (self.n_names, _abi.GRID_STRIKES, _abi.GRID_TENORS)
         vega, vega_se = np.empty(shape), np.empty(shape)
         rq.vega, rq.vega_se = _abi.ptr(vega), _abi.ptr(vega_se)
-        self._call(self._lib.optlab_adjoint, rq)
+        self._call(True, rq)
         n = self.n_names
         return Greeks(rq.price, rq.se, np.array(rq.delta[:n]), np.array(rq.delta_se[:n]), vega, vega_se, n_paths,
                       {"forward_ms": rq.forward_ms, "backward_ms": rq.backward_ms, "total_ms": rq.total_ms,
@@ -215,7 +274,7 @@ def autocall_function():
             shape = (note.n_names, _abi.GRID_STRIKES, _abi.GRID_TENORS)
             vega = np.empty(shape)
             rq.vega = _abi.ptr(vega)
-            note._call(note._lib.optlab_adjoint, rq)
+            note._call(True, rq)
             delta = torch.tensor(np.array(rq.delta[: note.n_names]), dtype=spots.dtype, device=spots.device)
             dvdh = torch.tensor(vega / 0.01, dtype=nodes.dtype, device=nodes.device)  # per unit of vol
             ctx.save_for_backward(delta, dvdh)

@@ -6,6 +6,7 @@ path engine (liboracle.so, oracle_paths), which the HIP autocall kernels are che
     smoothed comparisons ............................. packages/structured/src/optlab/structured/script/evaluator.py:93-108
     an `if` on a mask pays mask * then + (1 - mask) * else; `terminate` multiplies the live weight by 1 - mask
     local-vol paths of n names, one table row a step . packages/native/cuda/include/optlab_cuda/paths.hpp:274-361
+    other kinds a name (constant, Heston, SLV) ......... paths.hpp:196-361 (the oracle path engine's own kinds)
 """
 
 from __future__ import annotations
@@ -53,9 +54,12 @@ def phoenix(closes, fixing, obs_step, disc, coupon, coupon_barrier, autocall, me
     return value
 
 
-def closes(lib, paths_request_type, n_pairs, seed, root, tables, lo, inv, fwd, dt, first_pair=0, threads=0):
-    """The spots at every step, (n_names, n_steps + 1, 2 n_pairs), from the oracle's path engine: local vol for every
-    name, table row s of name a read at step s, forwards fwd (n_names, n_steps + 1)."""
+def closes(lib, paths_request_type, n_pairs, seed, root, tables, lo, inv, fwd, dt, first_pair=0, threads=0,
+           kinds=None, cvar=None, heston=None, same=0):
+    """The spots at every step, (n_names, n_steps + 1, 2 n_pairs), from the oracle's path engine: table row s of name
+    a read at step s, forwards fwd (n_names, n_steps + 1); every name local vol, or name a of kind kinds[a] (0
+    constant variance cvar[a], 1 local vol, 2 Heston, 3 SLV with tables[a] its squared leverage) with Heston
+    parameters heston[a] = (v0, kappa, theta, xi, rho) and the variance normals correlated by root when `same`."""
     n, n_steps, n_x = tables.shape
     rq = paths_request_type()
     keep = {}
@@ -71,10 +75,10 @@ def closes(lib, paths_request_type, n_pairs, seed, root, tables, lo, inv, fwd, d
     rq.t0 = put("t0", np.arange(n_steps) * dt)
     rq.obs_step = put("obs", np.arange(n_steps + 1), C.c_int)
     rq.root = put("root", root)
-    rq.kind = put("kind", np.ones(n), C.c_int)
-    rq.cvar = put("cvar", np.zeros(n))
-    rq.heston = put("heston", np.zeros(5 * n))
-    rq.same = 0
+    rq.kind = put("kind", np.ones(n) if kinds is None else kinds, C.c_int)
+    rq.cvar = put("cvar", np.zeros(n) if cvar is None else cvar)
+    rq.heston = put("heston", np.zeros(5 * n) if heston is None else np.asarray(heston).reshape(-1))
+    rq.same = int(same)
     rq.step_row = put("step_row", np.arange(n * n_steps), C.c_int)
     rq.row_lo = put("lo", np.asarray(lo).reshape(-1))
     rq.row_inv_dx = put("inv", np.asarray(inv).reshape(-1))
