@@ -257,6 +257,15 @@ typedef struct {
     int same;
 } optlab_autocall_models;
 
+/* ---- addition: a payoff script fused onto the device ----------------------------------------------------------- */
+/* What optlab_script returns: the price (the mean over pairs of the pair-averaged discounted payoff) and its standard
+   error, the path kernel's and the payoff kernel's times, the hipRTC compile time of the script (0 when the process
+   had compiled it already on this device) and the call's wall time, in ms. */
+typedef struct {
+    double price, se;
+    double paths_ms, payoff_ms, compile_ms, total_ms;
+} optlab_script_result;
+
 /* ---- addition: a book of trades revalued under scenarios, in one persistent launch --------------------------- */
 /* Trades on the names of one book (each name: a surface handle on the book's grid of uniform steps, and its
    forwards); paths by the local-vol step of paths.hpp:274-361 on stream 0 of each trade's own seed, so every
@@ -450,6 +459,15 @@ OPTLAB_API int optlab_adjoint(optlab_autocall_request* rq, int backend);
    name local vol. */
 OPTLAB_API int optlab_autocall_ex(optlab_autocall_request* rq, const optlab_autocall_models* models, int adjoint,
                                   int backend);
+
+/* Addition: a product in the payoff language priced on the engine's paths without bringing them to the host (the
+   reference evaluates the script in NumPy on paths copied back: structured/script/evaluator.py, structured/
+   engine.py:199-330). `source` is the HIP source the host lowers the script to (optlab.native.script), compiled at
+   run time by hipRTC for the current device and cached per process and device by its text; it defines
+   extern "C" __global__ void script_payoff(const double* out, long long n_pairs, double* rows), run with 256 lanes
+   a workgroup over the paths of rq (rq->out is not used: the paths stay in device memory). The payoff is evaluated
+   in double whatever the backend of the paths. */
+OPTLAB_API int optlab_script(optlab_paths_request* rq, int backend, const char* source, optlab_script_result* out);
 
 /* Addition: the book above (backend 0 double paths, 1 float paths; sums in double either way). */
 OPTLAB_API int optlab_book(optlab_book_request* rq, int backend);

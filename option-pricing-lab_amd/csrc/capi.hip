@@ -249,6 +249,18 @@ int optlab_paths_dev(optlab_paths_request* rq, int backend, double* d_out, void*
     });
 }
 
+int optlab_script(optlab_paths_request* rq, int backend, const char* source, optlab_script_result* out) {
+    return guarded([&] {
+        if (!rq || !source || !out) invalid("null argument");
+        check_paths_request(*rq);
+        if (rq->n_pairs < 2) invalid("n_pairs must be >= 2");
+        const int prec = precision_of(backend);
+        need_device();
+        std::memset(out, 0, sizeof *out);
+        run_script(*rq, prec, source, *out);
+    });
+}
+
 int optlab_leverage(optlab_leverage_request* rq, int backend, int n_threads) {
     (void)n_threads;
     return guarded([&] {

@@ -42,6 +42,8 @@ enum Slot {
     kSlotChunks,       // fixed products and the strip: compensated sums of chunks of the per-block rows
     kSlotRegress,      // the issuer's call regression: terms, states, sums, coefficients
     kSlotModels,       // autocall: the tables of names that are not local vol, SLV leverage rows and ratios
+    kSlotScript,       // script payoffs: the path engine's output the payoff kernel reads
+    kSlotScriptRows,   // script payoffs: each workgroup's (sum, sum of squares)
     kSlotCount
 };
 
@@ -58,6 +60,10 @@ hipStream_t engine_stream();
 double launch_paths(const optlab_paths_request& rq, int prec, double* d_out, hipStream_t stream,
                     double* d_var = nullptr);
 void check_paths_request(const optlab_paths_request& rq);
+
+// ---- script payoffs (script.hip) ----------------------------------------------------------------------------------
+// The paths of rq into device memory, then the payoff kernel compiled from `source` by hipRTC over them.
+void run_script(optlab_paths_request& rq, int prec, const char* source, optlab_script_result& res);
 
 // An exercise rule for the path kernel of one name (the pricing pass of Longstaff-Schwartz): at observation j a leg
 // exercises when its exercise value max(phi (S / K - 1), 0) is positive and beats coefs[j] . basis(S / K, v), the

@@ -154,6 +154,12 @@ class AutocallModels(C.Structure):
     _fields_ = [("kind", _PI), ("cvar", _PD), ("heston", _PD), ("lev2", _PD), ("same", _I)]
 
 
+class ScriptResult(C.Structure):
+    """optlab_script_result (addition: a payoff script fused onto the device)."""
+
+    _fields_ = [("price", _D), ("se", _D), ("paths_ms", _D), ("payoff_ms", _D), ("compile_ms", _D), ("total_ms", _D)]
+
+
 class LvSlices(C.Structure):
     """optlab_lv_slices (addition: the particle method's local variance as surface slices)."""
 
@@ -389,6 +395,7 @@ _SIGNATURES = {
     "optlab_surface_node_grad": (_I, [_I, _PD, _PD]),
     "optlab_autocall": (_I, [C.POINTER(AutocallRequest), _I]),
     "optlab_adjoint": (_I, [C.POINTER(AutocallRequest), _I]),
+    "optlab_script": (_I, [C.POINTER(PathsRequest), _I, C.c_char_p, C.POINTER(ScriptResult)]),
     "optlab_autocall_ex": (_I, [C.POINTER(AutocallRequest), C.POINTER(AutocallModels), _I, _I]),
     "optlab_book": (_I, [C.POINTER(BookRequest), _I]),
     "optlab_lsm": (_I, [C.POINTER(LsmRequest), _I]),

@@ -53,6 +53,7 @@ def test_ctypes_layouts_match_the_c_header(tmp_path):
         "optlab_leverage_options": _abi.LeverageOptions,
         "optlab_lv_slices": _abi.LvSlices,
         "optlab_autocall_models": _abi.AutocallModels,
+        "optlab_script_result": _abi.ScriptResult,
         "optlab_surface_rows": _abi.SurfaceRows,
         "optlab_autocall_request": _abi.AutocallRequest,
         "optlab_trade": _abi.Trade,
