@@ -2,8 +2,10 @@
 // Semantics: packages/native/cuda/src/leverage.cpp:17-186 (the host algorithm and the CPU particles) and
 // include/optlab_cuda/leverage.hpp:13-53.
 //
-// Per step i, five kernels on one stream, no host round trip (the reference copies the bins to the host and the
-// leverage row back, four synchronous copies per step):
+// Two forms, the same bits: by default one persistent launch for the whole calibration (below, "the calibration as
+// one persistent launch"); with OPTLAB_SLV_WORKGROUPS=0, or where the persistent launch cannot be resident, per step
+// i five kernels on one stream, no host round trip (the reference copies the bins to the host and the leverage row
+// back, four synchronous copies per step):
 //   bin   -- every block first reduces the previous step's per-block ranges (min x, max x, max |v|) in a fixed
 //            order, then bins its particles in LDS: a fine histogram (u32) for the quantiles and, on the regression
 //            bins, count and four fixed-point sums (v, d, d^2, d v with d = (x - centre) / width in [-1/2, 1/2]),
@@ -11,10 +13,9 @@
 //   reduce-- adds the per-block bins in a fixed order;
 //   prep  -- one workgroup: quantiles by a block scan of the fine histogram, the 81-point grid, the bandwidth
 //            and each regression bin's moments;
-//   fit   -- one wave per grid point, across the chip: the Gaussian Nadaraya-Watson (or local-linear) regression
-//            of E[v | x] on the bin centres in FP64 (fixed-order wave reductions), the FP64 division
-//            L = clip(sqrt(sigma_loc^2 / E[v | x])), and the leverage row for the step kernel (the regression is
-//            most of a step's FP64 work: on one CU it was the longest kernel of the step);
+//   fit   -- one lane per grid point: the Gaussian Nadaraya-Watson (or local-linear) regression of E[v | x] on
+//            the bin centres in FP64, in the reference's order over the bins within 40 bandwidths of the point,
+//            the FP64 division L = clip(sqrt(sigma_loc^2 / E[v | x])), and the leverage row for the step kernel;
 //   step  -- one pair per lane, the leverage row staged in LDS, one Euler step of both legs (Philox counter
 //            (pair, i + 1, 0), leverage.cpp:197-208), block min / max / max |v| for the next bin.
 // Particles are FP32 (backend 1) or FP64 (backend 0); everything after the bins is FP64.
@@ -573,8 +574,8 @@ __global__ void __launch_bounds__(kBlock) step_kernel(LevArgs s, Particles<R> P,
 // lane + k x lanes), and a step is three phases between two grid-wide waits:
 //   A. from the totals of step i's bins: the quantiles, the grid, the bins' moments and the bandwidth, then the
 //      regression at every grid point and the leverage row in LDS. Every workgroup computes the same row, in the
-//      five-kernel form's arithmetic (one wave per grid point, lanes over the bins): a few hundred FP64 operations
-//      a lane, cheaper than a third wait to broadcast it;
+//      five-kernel form's arithmetic (fit_point, a lane a grid point): a few hundred FP64 operations a lane,
+//      cheaper than a third wait to broadcast it;
 //   B. one Euler step of the workgroup's particles (leverage.cpp:189-214), the workgroup's range published;
 //   C. after the wait for every range: the particles binned in LDS and the workgroup's nonzero bins added into the
 //      step's totals with integer atomics (order-independent, so the totals are the five-kernel form's bits).
