@@ -297,6 +297,137 @@ def bench_adjoint(n_paths: int, seed: int = 42, reps: int = 3, bumps: bool = Tru
     }
 
 
+def bench_adjoint_models(n_paths: int = 1 << 19, seed: int = 42, reps: int = 3) -> dict:
+    """The autocall's adjoint with other models a name: the note of `autocall_note` with name 0 under SLV (its
+    leverage calibrated by the particle method on name 0's surface, 2^18 particles), name 1 under local vol and name
+    2 under Heston; device time of the forward and backward stages, against the local-vol note's."""
+    import numpy as np
+
+    from optlab import native
+    from optlab.core.market import Market
+    from optlab.core.models import HestonModel, HestonParams
+    from optlab.native.adjoint import WorstOfAutocall
+
+    base = autocall_note()
+    surfaces = [s.surface for s in base.surfaces]
+    market = Market.flat(100.0, 0.03, 0.0)
+    heston = HestonParams(0.04, 2.0, 0.04, 0.5, -0.7)
+    slv = native.calibrate_leverage(heston, surfaces[0], market, 3.0, 1.0, 1 << 18, rng=17, backend="gpu",
+                                    n_bins=256, bandwidth=("std", 0.25))
+    models = [slv, None, HestonModel(HestonParams(0.05, 2.0, 0.05, 0.5, -0.6))]
+    note = WorstOfAutocall(surfaces, [market] * 3, base.root @ base.root.T, models=models, same=True)
+    out = {}
+    for name, nt in (("slv_lv_heston", note), ("local_vol", base)):
+        nt.adjoint(n_paths, seed)  # warm-up, the autotuner's choice
+        dev, walls = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            g = nt.adjoint(n_paths, seed)
+            walls.append(time.perf_counter() - t0)
+            dev.append(g.seconds["forward_ms"] + g.seconds["backward_ms"])
+        out[name] = {"device_ms": float(np.median(dev)), "wall_ms": float(np.median(walls)) * 1e3,
+                     "checkpoint_every": g.seconds["checkpoint_every"], "price": g.price,
+                     "live_vegas": int((np.abs(g.vega) > 3 * g.vega_se).sum())}
+        nt.close()
+    out["workload"] = ("the worst-of phoenix of the adjoint block with name 0 under SLV (leverage calibrated on its "
+                       f"surface), name 1 local vol, name 2 Heston; {n_paths} FP32 paths; 3 deltas + node vegas of "
+                       "the SLV and local-vol names")
+    return out
+
+
+def bench_strip_greeks(reps: int = 5) -> dict:
+    """The strip's sensitivities (delta and d/dv0 of all 21 x 8 calls, optlab_strip_greeks) on the benchmark
+    strip's paths, against the strip itself and against central bumps of S0 and v0 on common random numbers (four
+    more strips)."""
+    import numpy as np
+    import torch
+
+    from optlab.native import _abi
+
+    lib = _abi.load()
+    rq, out = _request(1, N_PATHS)
+    shape = out["price"].shape
+    d, dse, g, gse = (np.empty(shape) for _ in range(4))
+
+    def greeks(backend):
+        assert lib.optlab_strip_greeks(C.byref(rq), backend, _abi.ptr(d), _abi.ptr(dse), _abi.ptr(g),
+                                       _abi.ptr(gse)) == 0, lib.optlab_last_error()
+        return rq.kernel_ms
+
+    def strip():
+        assert lib.optlab_strip(C.byref(rq), 1) == 0, lib.optlab_last_error()
+        return rq.kernel_ms
+
+    greeks(1), strip()
+    torch.cuda.synchronize()
+    t_g, k_g, t_s = [], [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        k_g.append(greeks(1))
+        t_g.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        strip()
+        t_s.append(time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    greeks(0)
+    t64 = time.perf_counter() - t0
+    wall, price = float(np.median(t_g)) * 1e3, float(np.median(t_s)) * 1e3
+    return {"workload": "delta and d/dv0 of the strip's 21 x 8 calls on its 2^20 FP32 paths x 504 steps (a tangent "
+                        "through every QE-M step)",
+            "greeks_ms": wall, "greeks_kernel_ms": float(np.median(k_g)), "greeks_fp64_ms": t64 * 1e3,
+            "strip_ms": price, "greeks_over_strip": wall / price,
+            "speedup_vs_central_bumps": 4 * price / wall,
+            "path_steps_per_s": N_PATHS * N_STEPS / (wall * 1e-3),
+            "atm_1y": {"delta": float(d[3, 10]), "delta_se": float(dse[3, 10]), "dv0": float(g[3, 10]),
+                       "dv0_se": float(gse[3, 10])}}
+
+
+def bench_script(n_paths: int = 1 << 20, reps: int = 5, cpu: bool = True) -> dict:
+    """The phoenix of the preset list written in the payoff language (3 local-vol names, 3 y quarterly, knock-in on
+    756 daily closes), priced by optlab_script: the paths into HBM, the payoff compiled by hipRTC and run over them,
+    pair sums back. The CPU baseline is the reference's way, the NumPy evaluator (restated in oracle/script_oracle.py)
+    on the same paths copied to the host, on a bounded sample."""
+    import numpy as np
+
+    from optlab.core.market import Market
+    from optlab.core.models import LocalVolModel
+    from optlab.native.script import ScriptPricer
+    from optlab.structured import script as sc
+    from optlab.structured.multiasset import Basket
+
+    base = autocall_note()
+    names = ("A", "B", "C")
+    market = Market.flat(100.0, 0.03, 0.0)
+    basket = Basket(names, {n: LocalVolModel(s.surface) for n, s in zip(names, base.surfaces)},
+                    {n: market for n in names}, base.root @ base.root.T)
+    base.close()
+    pricer = ScriptPricer(sc.phoenix_script(names, 3.0, 0.25, daily_ki=756), basket, smoothing=0.005)
+    first = pricer.price(n_paths, 7, "fp32")
+    runs = [pricer.price(n_paths, 7, "fp32") for _ in range(reps)]
+    med = {k: float(np.median([r[k] for r in runs])) for k in ("paths_ms", "payoff_ms", "total_ms")}
+    out = {"workload": f"phoenix script, 3 local-vol names, 3 y quarterly, daily knock-in window; {n_paths} FP32 "
+                       "paths, payoff in FP64 on the device",
+           "price": runs[0]["price"], "se": runs[0]["se"], "compile_ms_first_call": first["compile_ms"], **med,
+           "name_steps_per_s": n_paths * 756 * 3 / (med["total_ms"] * 1e-3)}
+    if cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import script_oracle
+
+        sample = 1 << 16
+        t0 = time.perf_counter()
+        spots, wins = pricer.paths(sample, 7, "fp32")
+        t_paths = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        values = script_oracle.leg_values(pricer.program, None, 0.005, spots, wins, lambda t: float(market.D(t)))
+        t_eval = time.perf_counter() - t0
+        price, _ = script_oracle.pair_mean(values)
+        out["cpu_baseline"] = {"value": t_eval * 1e3 * n_paths / sample, "unit": "ms", "cores": 1, "kind": "port",
+                               "sample": f"the evaluator in NumPy on {sample} of the same paths brought to the host "
+                                         f"({t_paths * 1e3:.1f} ms to simulate and copy them), scaled to {n_paths}",
+                               "price_on_sample": price}
+    return out
+
+
 def bench_book(path_scale: int = 1, n_scen: int = 250) -> dict:
     """The batched book: 512 trades (seed 7) x 250 scenarios plus the base, 2^14-2^18 paths by product (divided by
     `path_scale`); one persistent launch captured in a HIP graph against the per-trade loop, same numbers. The first
@@ -576,7 +707,8 @@ def main(argv=None) -> int:
     ap.add_argument("--book-scale", type=int, default=1,
                     help="divide every trade's paths of the batched book by this (0: skip the book)")
     ap.add_argument("--slv-particles", type=int, default=1 << 20, help="SLV particles (0: skip the SLV block)")
-    ap.add_argument("--only", choices=("adjoint", "book", "slv", "fixings", "lsm", "reference"), default=None,
+    ap.add_argument("--only", choices=("adjoint", "adjoint_models", "book", "slv", "fixings", "lsm", "reference",
+                                       "strip_greeks", "script"), default=None,
                     help="run only this block and print its JSON (for profiling runs; the benchmark line needs all)")
     ap.add_argument("--no-bumps", action="store_true", help="with --only adjoint: skip the bump-and-revalue comparator")
     ap.add_argument("--no-blocks", action="store_true",
@@ -606,7 +738,10 @@ def main(argv=None) -> int:
                  "slv": lambda: bench_slv(args.slv_particles),
                  "fixings": lambda: bench_fixings(),
                  "lsm": lambda: bench_lsm(cpu=not args.no_cpu),
-                 "reference": lambda: bench_reference_workloads()}[args.only]()
+                 "reference": lambda: bench_reference_workloads(),
+                 "adjoint_models": lambda: bench_adjoint_models(),
+                 "strip_greeks": lambda: bench_strip_greeks(),
+                 "script": lambda: bench_script(cpu=not args.no_cpu)}[args.only]()
         print(json.dumps({args.only: block}), flush=True)
         return 0
     ms32, kern32, p32, se32 = _time_gpu(lib, 1, args.steps, args.warmup)
@@ -696,6 +831,13 @@ def main(argv=None) -> int:
     print("bench: lsm block done", file=sys.stderr, flush=True)
     line["reference_workloads"] = bench_reference_workloads()
     print("bench: reference workloads done", file=sys.stderr, flush=True)
+    line["strip_greeks"] = bench_strip_greeks()
+    print("bench: strip sensitivities done", file=sys.stderr, flush=True)
+    line["script"] = bench_script(cpu=not args.no_cpu)
+    print("bench: script payoff done", file=sys.stderr, flush=True)
+    if args.adjoint_paths:
+        line["adjoint_models"] = bench_adjoint_models(args.adjoint_paths)
+        print("bench: adjoint with models done", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
     if args.copy_mib:
         src = torch.ones(args.copy_mib << 20, dtype=torch.uint8, device="cuda")

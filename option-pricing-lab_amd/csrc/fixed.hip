@@ -490,59 +490,113 @@ __device__ __forceinline__ void strip_body(const StripArgs<R>& s, long long n_pa
 // step beside the state. A payoff D max(s (S - K), 0) differentiates to D 1{s (S - K) > 0} s dS. Many outputs and
 // two inputs make the tangent the cheap direction here: two more registers a leg and no stored path.
 
-// One QE-M step of (y, v) in qe_step's operations, with the tangent (ty, tv) of (y, v) along v0 (the step's
+// The special functions of the tangent step: in FP32 the price kernels' (hardware log, the Chebyshev erfc), in
+// FP64 the library's, as qe_step takes them.
+__device__ __forceinline__ float tan_log(float x) { return hw_log(x); }
+__device__ __forceinline__ double tan_log(double x) { return r_log(x); }
+__device__ __forceinline__ float tan_erfc(float x) { return erfc_fast(x); }
+__device__ __forceinline__ double tan_erfc(double x) { return r_erfc(x); }
+
+// A QE-M step of (y, v) in qe_step's operations, with the tangent (ty, tv) of (y, v) along v0 (the step's
 // constants do not depend on v0). The exponential branch's v' = 0 region (1 - U >= 1 - p) has a zero tangent. The
 // switch between the branches at psi_c and the martingale correction's switch to K0p are steps of the scheme that a
 // pathwise derivative does not see: on paths that cross them under a bump the tangent and a finite difference part.
 template <typename R>
-__device__ __forceinline__ void qe_step_tangent(const Consts<R>& c, R& y, R& v, R& ty, R& tv, R zv, R zx) {
-    const R m = c.theta + (v - c.theta) * c.E;
-    const R psi = (v * c.c1 + c.c2) / (m * m);
-    const R dm = c.E * tv;
-    const R dpsi = (c.c1 * tv - R(2) * psi * m * dm) / (m * m);  // psi = s2 / m^2 with s2 = v c1 + c2
+struct TanBranch {
     R vn, lin, w, arg, dvn, dlin, darg;
     bool finite;
-    if (psi <= c.psi_c) {
-        const R inv = R(2) / psi;
-        const R root = r_sqrt(inv) * r_sqrt(r_max(inv - R(1), R(0)));
-        const R b2 = inv - R(1) + root;
-        const R a = m / (R(1) + b2), b = r_sqrt(b2);
-        vn = a * (b + zv) * (b + zv);
-        arg = R(1) - R(2) * c.A * a;
-        finite = c.A < R(1) / (R(2) * a);
-        lin = -c.A * b2 * a / arg;
-        w = R(0.5);
-        const R dinv = -inv * dpsi / psi;
-        // root = sqrt(inv^2 - inv): d root = (2 inv - 1) d inv / (2 root)
-        const R db2 = dinv + (root > R(0) ? (R(2) * inv - R(1)) * dinv / (R(2) * root) : R(0));
-        const R da = (dm - a * db2) / (R(1) + b2), db = db2 / (R(2) * b);
-        dvn = da * (b + zv) * (b + zv) + R(2) * a * (b + zv) * db;
-        darg = R(-2) * c.A * da;
-        dlin = (-c.A * (db2 * a + b2 * da) - lin * darg) / arg;
-    } else {
-        const R p = (psi - R(1)) / (psi + R(1)), beta = (R(1) - p) / m;
-        const R one_minus_u = R(0.5) * r_erfc(zv * R(0.70710678118654752));
-        const bool zero = one_minus_u >= R(1) - p;
-        vn = zero ? R(0) : r_log((R(1) - p) / one_minus_u) / beta;
-        arg = p + beta * (R(1) - p) / (beta - c.A);
-        finite = c.A < beta;
-        lin = R(0);
-        w = R(-1);
-        const R dp = R(2) * dpsi / ((psi + R(1)) * (psi + R(1)));
-        const R dbeta = (-dp - beta * dm) / m;
-        dvn = zero ? R(0) : (-dp / (R(1) - p) - vn * dbeta) / beta;
-        const R gap = beta - c.A;
-        darg = dp + (dbeta * (R(1) - p) - beta * dp) / gap - beta * (R(1) - p) * dbeta / (gap * gap);
-        dlin = R(0);
-    }
-    const R K0 = finite ? lin + w * r_log(arg) - (c.K1 + R(0.5) * c.K3) * v : c.K0p;
-    const R dK0 = finite ? dlin + w * darg / arg - (c.K1 + R(0.5) * c.K3) * tv : R(0);
-    const R s = c.K3 * v + c.K4 * vn;
+};
+
+// psi = s2 / m^2 with s2 = v c1 + c2, m = theta + (v - theta) E; their tangents along v0 (dv = tv)
+template <typename R>
+struct TanMoments {
+    R m, psi, dm, dpsi;
+};
+
+template <typename R>
+__device__ __forceinline__ TanMoments<R> tan_moments(const Consts<R>& c, R v, R tv) {
+    TanMoments<R> o;
+    o.m = c.theta + (v - c.theta) * c.E;
+    o.psi = (v * c.c1 + c.c2) / (o.m * o.m);
+    o.dm = c.E * tv;
+    o.dpsi = (c.c1 * tv - R(2) * o.psi * o.m * o.dm) / (o.m * o.m);
+    return o;
+}
+
+template <typename R>
+__device__ __forceinline__ TanBranch<R> tan_quadratic(const Consts<R>& c, const TanMoments<R>& q, R zv) {
+    TanBranch<R> o;
+    const R inv = R(2) / q.psi;
+    const R root = r_sqrt(inv) * r_sqrt(r_max(inv - R(1), R(0)));
+    const R b2 = inv - R(1) + root;
+    const R a = q.m / (R(1) + b2), b = r_sqrt(b2);
+    o.vn = a * (b + zv) * (b + zv);
+    o.arg = R(1) - R(2) * c.A * a;
+    o.finite = c.A < R(1) / (R(2) * a);
+    o.lin = -c.A * b2 * a / o.arg;
+    o.w = R(0.5);
+    const R dinv = -inv * q.dpsi / q.psi;
+    // root = sqrt(inv^2 - inv): d root = (2 inv - 1) d inv / (2 root)
+    const R db2 = dinv + (root > R(0) ? (R(2) * inv - R(1)) * dinv / (R(2) * root) : R(0));
+    const R da = (q.dm - a * db2) / (R(1) + b2), db = db2 / (R(2) * b);
+    o.dvn = da * (b + zv) * (b + zv) + R(2) * a * (b + zv) * db;
+    o.darg = R(-2) * c.A * da;
+    o.dlin = (-c.A * (db2 * a + b2 * da) - o.lin * o.darg) / o.arg;
+    return o;
+}
+
+template <typename R>
+__device__ __forceinline__ TanBranch<R> tan_exponential(const Consts<R>& c, const TanMoments<R>& q, R zv) {
+    TanBranch<R> o;
+    const R p = (q.psi - R(1)) / (q.psi + R(1)), beta = (R(1) - p) / q.m;
+    const R one_minus_u = R(0.5) * tan_erfc(zv * R(0.70710678118654752));
+    const bool zero = one_minus_u >= R(1) - p;
+    o.vn = zero ? R(0) : tan_log((R(1) - p) / one_minus_u) / beta;
+    o.arg = p + beta * (R(1) - p) / (beta - c.A);
+    o.finite = c.A < beta;
+    o.lin = R(0);
+    o.w = R(-1);
+    const R dp = R(2) * q.dpsi / ((q.psi + R(1)) * (q.psi + R(1)));
+    const R dbeta = (-dp - beta * q.dm) / q.m;
+    o.dvn = zero ? R(0) : (-dp / (R(1) - p) - o.vn * dbeta) / beta;
+    const R gap = beta - c.A;
+    o.darg = dp + (dbeta * (R(1) - p) - beta * dp) / gap - beta * (R(1) - p) * dbeta / (gap * gap);
+    o.dlin = R(0);
+    return o;
+}
+
+template <typename R>
+__device__ __forceinline__ void tan_tail(const Consts<R>& c, R& y, R& v, R& ty, R& tv, const TanBranch<R>& o, R zx) {
+    const R K0 = o.finite ? o.lin + o.w * tan_log(o.arg) - (c.K1 + R(0.5) * c.K3) * v : c.K0p;
+    const R dK0 = o.finite ? o.dlin + o.w * o.darg / o.arg - (c.K1 + R(0.5) * c.K3) * tv : R(0);
+    const R s = c.K3 * v + c.K4 * o.vn;
     const R sd = r_sqrt(r_max(s, R(0)));
-    y += c.mu_dt + K0 + c.K1 * v + c.K2 * vn + sd * zx;
-    ty += dK0 + c.K1 * tv + c.K2 * dvn + (s > R(0) ? (c.K3 * tv + c.K4 * dvn) * zx / (R(2) * sd) : R(0));
-    v = vn;
-    tv = dvn;
+    y += c.mu_dt + K0 + c.K1 * v + c.K2 * o.vn + sd * zx;
+    ty += dK0 + c.K1 * tv + c.K2 * o.dvn + (s > R(0) ? (c.K3 * tv + c.K4 * o.dvn) * zx / (R(2) * sd) : R(0));
+    v = o.vn;
+    tv = o.dvn;
+}
+
+// Both legs of a pair (normals (zv, zx) and (-zv, -zx)) as qe_pair takes them: the quadratic branch for both legs
+// in every lane, the exponential branch once, on the leg of each lane that needs it (leg 0 when both do), and again
+// only for lanes whose two legs both need it. Each leg gets the numbers of the branch its own psi selects.
+template <typename R>
+__device__ __forceinline__ void qe_pair_tangent(const Consts<R>& c, R (&y)[2], R (&v)[2], R (&ty)[2], R (&tv)[2],
+                                                R zv, R zx) {
+    const TanMoments<R> q[2] = {tan_moments(c, v[0], tv[0]), tan_moments(c, v[1], tv[1])};
+    const bool ex[2] = {!(q[0].psi <= c.psi_c), !(q[1].psi <= c.psi_c)};
+    TanBranch<R> o[2] = {tan_quadratic(c, q[0], zv), tan_quadratic(c, q[1], -zv)};
+    if (ex[0] || ex[1]) {
+        const bool first = ex[0];
+        const TanBranch<R> r = tan_exponential(c, first ? q[0] : q[1], first ? zv : -zv);
+        if (first)
+            o[0] = r;
+        else
+            o[1] = r;
+    }
+    if (ex[0] && ex[1]) o[1] = tan_exponential(c, q[1], -zv);
+    tan_tail(c, y[0], v[0], ty[0], tv[0], o[0], zx);
+    tan_tail(c, y[1], v[1], ty[1], tv[1], o[1], -zx);
 }
 
 // The sums of one maturity's sensitivities (g 0: delta, 1: the variance parameter) over the wave's pairs, added to
@@ -619,8 +673,9 @@ __device__ __forceinline__ void strip_greeks_body(const StripArgs<R>& s, long lo
             }
             if (MODEL == 1) {
                 const R zv = j == 0 ? z[0] : z[2], zx = j == 0 ? z[1] : z[3];
-                qe_step_tangent(c, y0, v0, ty0, tv0, zv, zx);
-                qe_step_tangent(c, y1, v1, ty1, tv1, -zv, -zx);
+                R y[2] = {y0, y1}, v[2] = {v0, v1}, ty[2] = {ty0, ty1}, tv[2] = {tv0, tv1};
+                qe_pair_tangent(c, y, v, ty, tv, zv, zx);
+                y0 = y[0], y1 = y[1], v0 = v[0], v1 = v[1], ty0 = ty[0], ty1 = ty[1], tv0 = tv[0], tv1 = tv[1];
             } else {
                 const R zb = j == 0 ? z[0] : j == 1 ? z[1] : j == 2 ? z[2] : z[3];
                 y0 += c.bs_drift + c.bs_vol * zb;
