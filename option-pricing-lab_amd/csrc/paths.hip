@@ -97,9 +97,8 @@ struct Extremes {
 
 // paths.hpp:241-272, for the two legs of a lane's pair (output rows row[0], row[1]).
 template <typename R, int N>
-__device__ __forceinline__ void record(const PathArgs<R>& s, const long long (&row)[2], bool active, int j,
-                                       const R (&x)[2][N], const R (&v)[2][N], Extremes<R, N>& e) {
-    const int c = ldc(s.col_of, j);
+__device__ __forceinline__ void record(const PathArgs<R>& s, const long long (&row)[2], bool active, int j, int c,
+                                       const R (&fw)[N], const R (&x)[2][N], const R (&v)[2][N], Extremes<R, N>& e) {
     if (s.out_var && c >= 0 && active) {
 #pragma unroll
         for (int l = 0; l < 2; ++l) s.out_var[c * s.rows + row[l]] = static_cast<double>(v[l][0]);
@@ -107,7 +106,7 @@ __device__ __forceinline__ void record(const PathArgs<R>& s, const long long (&r
 #pragma unroll
     for (int a = 0; a < N; ++a) {
         if (a >= s.n_names) break;
-        const R f = ldc(s.fwd, a * s.n_obs + j);
+        const R f = fw[a];
         R S[2];
 #pragma unroll
         for (int l = 0; l < 2; ++l) S[l] = f * r_exp(x[l][a]);
@@ -245,14 +244,36 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
     constexpr bool mixer = FORM == kFormMixed;
     Extremes<R, N> ext;
     Exercise exs{{0.0, 0.0}, {s.n_obs - 1, s.n_obs - 1}, {false, false}};
+    // The next observation's step, output column and forwards are loaded one observation ahead, and the next
+    // step's dt one step ahead: a scalar load's wait then falls after a step of vector work instead of at its
+    // start (every step waited for its constants when each was loaded where it was used).
+    int next_step = -1, next_col = -1;
+    R next_fw[N];
+    auto load_obs = [&](int jj) {
+        const bool in = jj < s.n_obs;
+        next_step = in ? ldc(s.obs_step, jj) : -1;
+        if constexpr (!EXER) {
+            next_col = in ? ldc(s.col_of, jj) : -1;
+#pragma unroll
+            for (int a = 0; a < N; ++a) next_fw[a] = (in && a < n) ? ldc(s.fwd, a * s.n_obs + jj) : R(0);
+        }
+    };
     auto observe = [&](int jj) {
-        if constexpr (EXER)
+        if constexpr (EXER) {
             exercise<R, N>(s, jj, x, v, exs);
-        else
-            record<R, N>(s, row, active, jj, x, v, ext);
+        } else {
+            R fw[N];
+#pragma unroll
+            for (int a = 0; a < N; ++a) fw[a] = next_fw[a];
+            record<R, N>(s, row, active, jj, next_col, fw, x, v, ext);
+        }
     };
     int j = 0;
-    while (j < s.n_obs && ldc(s.obs_step, j) == 0) observe(j++);
+    load_obs(0);
+    while (next_step == 0) {
+        observe(j++);
+        load_obs(j);
+    }
 
     NormalStream<R> rng, aux, drv;
     rng.init(pair, s.key0, s.key1, 0u);
@@ -263,13 +284,15 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
     const int stage_len = s.n_tab * s.n_x;
     if (staged && s.n_steps > 0) stage_rows(s, 0, lds);
 
+    R dt_next = s.n_steps > 0 ? ldc(s.dt, 0) : R(0);
     for (int i = 1; i <= s.n_steps; ++i) {
         const R* cur = lds + ((i - 1) & 1) * stage_len;
         if (staged) {
             __syncthreads();
             if (i < s.n_steps) stage_rows(s, i, lds + (i & 1) * stage_len);
         }
-        const R dt = ldc(s.dt, i - 1), t0 = ldc(s.t0, i - 1);
+        const R dt = dt_next, t0 = mixer ? ldc(s.t0, i - 1) : R(0);
+        if (i < s.n_steps) dt_next = ldc(s.dt, i);
         R zs[N], zr[N], fresh = R(0);
 #pragma unroll
         for (int a = 0; a < N; ++a)
@@ -391,7 +414,10 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
                 x[leg][a] += R(-0.5) * var * dt + r_sqrt(var * dt) * z[a];
             }
         }
-        while (j < s.n_obs && ldc(s.obs_step, j) == i) observe(j++);
+        while (next_step == i) {
+            observe(j++);
+            load_obs(j);
+        }
     }
     if constexpr (EXER) {  // products.py:150 and :30-35: K cash D(t_tau) of each leg, the pair's average
         const ExerciseConst r = (ExerciseConst)s.ex;
