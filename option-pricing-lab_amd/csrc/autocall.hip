@@ -296,7 +296,7 @@ __device__ __forceinline__ double adj_rcp(double x) { return 1.0 / x; }
 
 template <typename R, int N, int SEG, bool VAR>
 __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s, long long n_chunks, R* xbar,
-                                                               unsigned long long* batch) {
+                                                               unsigned long long* batch, int xl_legs) {
     constexpr int kBwdBlock = bwd_block<R>();
     extern __shared__ unsigned char lds_raw[];
     unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds_raw);  // SEG x N x n_x
@@ -304,6 +304,9 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
     const long long c0 = blk * n_chunks / G, c1 = (blk + 1) * n_chunks / G;
     const int n_seg = (s.n_steps + SEG - 1) / SEG;
     const int row_len = N * s.n_x;
+    // x-bar of the workgroup's legs from one segment to the one before: in LDS after the rows when xl_legs > 0
+    // ([name][xl_legs], a lane's own entries only), else in device memory
+    R* xl = reinterpret_cast<R*>(acc + SEG * row_len);
     for (int seg = n_seg - 1; seg >= 0; --seg) {
         const int s0 = seg * SEG, len = min(SEG, s.n_steps - s0);
         for (int e = threadIdx.x; e < SEG * row_len; e += kBwdBlock) acc[e] = 0ull;
@@ -312,13 +315,14 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
             const long long i = chunk * kBwdBlock + threadIdx.x;
             const bool active = i < s.n_legs;
             const long long leg = active ? i : 0;
+            const int local = static_cast<int>(chunk - c0) * kBwdBlock + static_cast<int>(threadIdx.x);
             const R sign = (i & 1) ? R(-1) : R(1);
             R x[N], xb[N], v[N];
 #pragma unroll
             for (int a = 0; a < N; ++a) {
                 x[a] = s.ck[(static_cast<long long>(seg) * N + a) * s.n_legs + leg];
                 v[a] = VAR ? s.ckv[(static_cast<long long>(seg) * N + a) * s.n_legs + leg] : R(1);
-                xb[a] = seg == n_seg - 1 ? R(0) : xbar[a * s.n_legs + leg];
+                xb[a] = seg == n_seg - 1 ? R(0) : (xl_legs ? xl[a * xl_legs + local] : xbar[a * s.n_legs + leg]);
             }
             NormalStream<R> rng, aux;
             rng.init(static_cast<std::uint64_t>(s.first_pair + (leg >> 1)), s.key0, s.key1, 0u);
@@ -424,7 +428,12 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
             }
             if (seg > 0 && active) {
 #pragma unroll
-                for (int a = 0; a < N; ++a) xbar[a * s.n_legs + i] = xb[a];
+                for (int a = 0; a < N; ++a) {
+                    if (xl_legs)
+                        xl[a * xl_legs + local] = xb[a];
+                    else
+                        xbar[a * s.n_legs + i] = xb[a];
+                }
             }
         }
         __syncthreads();
@@ -697,23 +706,38 @@ void run_typed(optlab_autocall_request& rq, const optlab_autocall_models* models
 
     std::vector<double> node_out;
     long long bwd_blocks = 0, n_chunks = 0;
+    int xl_legs = 0;
     if (adjoint) {
-        const size_t lds = static_cast<size_t>(SEG) * N * n_x * sizeof(unsigned long long);
+        size_t lds = static_cast<size_t>(SEG) * N * n_x * sizeof(unsigned long long);
         int max_lds = 0;
         int dev_id = 0;
         hip_check(hipGetDevice(&dev_id), "hipGetDevice");
         hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_id), "attribute");
         if (lds > static_cast<size_t>(max_lds)) invalid("n_x too large for the adjoint's LDS rows");
-        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(autocall_backward<R, N, SEG, VAR>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
-                  "LDS attribute");
-        int per_cu = 0;
-        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, autocall_backward<R, N, SEG, VAR>, kBwdBlock,
-                                                               lds),
-                  "occupancy");
+        const void* bwd_fn = reinterpret_cast<const void*>(autocall_backward<R, N, SEG, VAR>);
+        auto resident = [&](size_t bytes) {
+            hip_check(hipFuncSetAttribute(bwd_fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)),
+                      "LDS attribute");
+            int per = 0;
+            hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, autocall_backward<R, N, SEG, VAR>, kBwdBlock,
+                                                                   bytes),
+                      "occupancy");
+            return per;
+        };
+        const int per_cu = resident(lds);
         n_chunks = (n_legs + kBwdBlock - 1) / kBwdBlock;
         bwd_blocks = std::min<long long>(n_chunks, static_cast<long long>(std::max(per_cu, 1)) * device_cu_count());
         bwd_blocks = std::max<long long>(bwd_blocks, 1);
+        // x-bar in LDS beside the rows where it fits without costing a resident workgroup: no HBM round trip of
+        // n_names x n_legs values a segment
+        const long long xl = (n_chunks + bwd_blocks - 1) / bwd_blocks * kBwdBlock;
+        const size_t xl_bytes = static_cast<size_t>(N) * xl * sizeof(R);
+        if (lds + xl_bytes <= static_cast<size_t>(max_lds) && resident(lds + xl_bytes) >= per_cu) {
+            lds += xl_bytes;
+            xl_legs = static_cast<int>(xl);
+        } else {
+            resident(lds);
+        }
         const long long entries = static_cast<long long>(n) * ns * n_x;
         // the 16 batch tables, one after the other ([batch][name][step][x], 64-bit fixed point), filled by the
         // backward kernel's atomics: zeroed first
@@ -724,7 +748,7 @@ void run_typed(optlab_autocall_request& rq, const optlab_autocall_models* models
         hip_check(hipMemsetAsync(batch, 0, static_cast<size_t>(kBatches) * entries * sizeof(long long), stream),
                   "clear batch tables");
         hipLaunchKernelGGL((autocall_backward<R, N, SEG, VAR>), dim3(static_cast<unsigned>(bwd_blocks)), dim3(kBwdBlock), lds,
-                           stream, s, n_chunks, xbar, reinterpret_cast<unsigned long long*>(batch));
+                           stream, s, n_chunks, xbar, reinterpret_cast<unsigned long long*>(batch), xl_legs);
         hip_check(hipGetLastError(), "autocall backward launch");
         SurfaceSet set{};
         DerivSet dset{};
