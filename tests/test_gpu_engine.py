@@ -282,17 +282,29 @@ def test_fp64_leverage_matches_the_oracle(lib, local_linear):
     np.testing.assert_allclose(got["values"], want["values"], rtol=1e-9, atol=1e-12)
 
 
-def test_fp32_leverage_is_as_close_as_the_reference_gpu(lib):
-    """The reference's own float GPU leverage sits within 3.2e-3 of its CPU one on the golden case
-    (native_gpu.json against native_cpu.json); ours must be as close to the oracle."""
+def test_fp32_leverage_is_within_a_fraction_of_its_own_sampling_noise(lib):
+    """FP32 particles against FP64 on the same draws (the golden case: 20 000 particles, 1 y): the largest gap of
+    the leverage, over the whole grid, is measured against the estimator's own sampling noise, the largest gap
+    between the FP64 leverages of two seeds; the x grids (quantiles) against the golden CPU digest within the
+    reference GPU's own gap to it (7e-4, native_gpu.json against native_cpu.json)."""
     market, surface, heston = simulated_context()
-    rq, got, _ = native.leverage_request(heston, surface, 1.0, 0.5, 20_000, 1 / 100, 1 / 2000, 81, 1.0, 5,
-                                         (0.05, 8.0), "nadaraya_watson", False)
-    assert lib.optlab_leverage(C.byref(rq), 1, 0) == 0
+
+    def lev(backend, seed):
+        rq, got, _ = native.leverage_request(heston, surface, 1.0, 0.5, 20_000, 1 / 100, 1 / 2000, 81, 1.0, seed,
+                                             (0.05, 8.0), "nadaraya_watson", False)
+        assert lib.optlab_leverage(C.byref(rq), backend, 0) == 0, lib.optlab_last_error()
+        return np.asarray(got["x_grids"]).copy(), np.asarray(got["values"]).copy()
+
+    x32, v32 = lev(1, 5)
+    x64, v64 = lev(0, 5)
+    _, v64b = lev(0, 6)
+    rounding = np.max(np.abs(v32 - v64))
+    noise = np.max(np.abs(v64b - v64))
+    print("FP32 - FP64 leverage", rounding, "FP64 seed 5 - seed 6", noise, flush=True)
+    assert rounding < 0.25 * noise
     g = fixture()["native_cpu"]["leverage"]
-    for key, tol in (("x_grids", 7e-4 * 3), ("values", 3.2e-3 * 3)):
-        samples, _ = digest_samples(got[key])
-        assert np.max(np.abs(samples - np.array(g[key]["samples"]))) < tol
+    samples, _ = digest_samples(x32)
+    assert np.max(np.abs(samples - np.array(g["x_grids"]["samples"]))) < 7e-4
 
 
 def test_bad_requests_raise_without_touching_the_device(lib):
