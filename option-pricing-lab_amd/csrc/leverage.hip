@@ -14,7 +14,7 @@
 //   prep  -- one workgroup: quantiles by a block scan of the fine histogram, the 81-point grid, the bandwidth
 //            and each regression bin's moments;
 //   fit   -- one lane per grid point: the Gaussian Nadaraya-Watson (or local-linear) regression of E[v | x] on
-//            the bin centres in FP64, in the reference's order over the bins within 40 bandwidths of the point,
+//            the bin centres in FP64, in the reference's order over the bins within 12 bandwidths of the point,
 //            the FP64 division L = clip(sqrt(sigma_loc^2 / E[v | x])), and the leverage row for the step kernel;
 //   step  -- one pair per lane, the leverage row staged in LDS, one Euler step of both legs (Philox counter
 //            (pair, i + 1, 0), leverage.cpp:197-208), block min / max / max |v| for the next bin.
@@ -320,15 +320,17 @@ __device__ double local_var_join(const LevArgs& s, int i, double x0, double y0, 
 }
 
 // E[v | x0] of step i > 0 from the bins (leverage.cpp:67-84): the Gaussian Nadaraya-Watson ratio, or the local-linear
-// estimate, over the bins in increasing order as the reference adds them, skipping only the bins more than 40
-// bandwidths from x0, whose weight exp(-d^2 / 2) underflows to exactly 0 (one bin of margin for the rounding of
-// the window's ends). B: centre, count, sum v, sum x, sum x^2, sum x v of each bin (6 x nb); lo, hi the bins' range.
+// estimate, over the bins in increasing order as the reference adds them, skipping the bins more than 12 bandwidths
+// from x0 (one bin of margin for the rounding of the window's ends): their weights exp(-d^2 / 2) < exp(-72) carry,
+// over at most 2^22 particles, a total below 3e-25, under 1e-16 of any S0 the estimate uses (S0 > 1e-8, else the
+// fallback; local-linear: S0 >= 50), so the result is the full sum's to the last digits. d = (x0 - c) / bw as the
+// reference forms it. B: centre, count, sum v, sum x, sum x^2, sum x v of each bin (6 x nb); lo, hi the bins' range.
 __device__ double fit_point(const LevArgs& s, const double* B, double x0, double bw, double fallback, double lo,
                             double hi) {
 #pragma clang fp contract(off)
     const int nb = s.n_bins;
     int ka = 0, kb = nb - 1;
-    const double reach = 40.0 * bw, we = (hi + 1e-12 - lo) / nb;
+    const double reach = 12.0 * bw, we = (hi + 1e-12 - lo) / nb;
     if (bw > 0 && reach < INFINITY && we > 0) {
         const double fa = floor((x0 - reach - lo) / we - 0.5) - 1.0, fb = ceil((x0 + reach - lo) / we - 0.5) + 1.0;
         ka = fa <= 0.0 ? 0 : (fa >= nb ? nb : static_cast<int>(fa));
@@ -687,8 +689,8 @@ __global__ void __launch_bounds__(kPersistThreads) slv_persistent_kernel(LevArgs
         }
         // ---- A: the grid and the leverage row of step i (leverage.cpp:42-97). Latency-bound: the totals are read
         // in one batch of loads, the table entries of every grid point are evaluated by different lanes, and the
-        // regression of a grid point visits only the bins within 40 bandwidths of it (beyond, exp(-d^2 / 2)
-        // underflows to exactly 0, so the sums are those over every bin, bit for bit).
+        // regression of a grid point visits only the bins within 12 bandwidths of it (fit_point: what lies beyond
+        // is below 1e-16 of the sums the estimate uses).
         const double h = s.dt[i];
         double ga, gb;
         if (i == 0) {
