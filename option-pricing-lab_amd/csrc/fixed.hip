@@ -997,6 +997,9 @@ double strip_greeks_on_device(const StripArgs<R>& s, long long n_pairs, std::uin
     return s.model == 1 ? run(strip_greeks_kernel<R, 1>) : run(strip_greeks_kernel<R, 0>);
 }
 
+// The bytes of the last strip table copied to each device's strip buffer, whatever its precision.
+std::vector<std::vector<unsigned char>> g_strip_sent;
+
 // The strip's constants on the host, and its (event, slot) table and event steps copied to device memory.
 template <typename R>
 StripArgs<R> strip_args(const optlab_strip_request& rq) {
@@ -1023,8 +1026,10 @@ StripArgs<R> strip_args(const optlab_strip_request& rq) {
         }
     }
     // The strip's own buffer, sized for the FP64 table so it is never reallocated; the table is uploaded only when
-    // it differs from the last one sent to this device (repeated calls on one strip skip the copy).
-    static std::vector<std::vector<unsigned char>> last;  // per device
+    // it differs from the last one sent to this device (repeated calls on one strip skip the copy). One record of the
+    // last upload for both precisions, since they share the buffer: a float table after a double one of the same
+    // strip differs in size, so it is sent again.
+    std::vector<std::vector<unsigned char>>& last = g_strip_sent;  // per device
     int device = 0;
     hip_check(hipGetDevice(&device), "hipGetDevice");
     if (static_cast<int>(last.size()) <= device) last.resize(device + 1);

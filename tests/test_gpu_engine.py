@@ -321,11 +321,15 @@ def test_strip_table_cache_never_serves_a_stale_table(lib):
     a, (_, _, pa, _) = _strip(n_paths=8192, strikes=(80, 100, 120))
     b, (_, _, pb, _) = _strip(n_paths=8192, strikes=(90, 95, 105, 110), mats=(26, 104))
     results = []
-    for rq, price, backend in ((a, pa, 1), (b, pb, 1), (a, pa, 0), (a, pa, 1), (b, pb, 0), (b, pb, 1)):
+    for rq, price, backend in ((a, pa, 1), (b, pb, 1), (a, pa, 0), (a, pa, 1), (b, pb, 0), (b, pb, 1), (b, pb, 0),
+                               (b, pb, 1)):
         assert lib.optlab_strip(C.byref(rq), backend) == 0
         results.append(price.copy())
     np.testing.assert_array_equal(results[0], results[3])
     np.testing.assert_array_equal(results[1], results[5])
+    # the same strip in FP64 then FP32 twice over: the two precisions share the device table's buffer
+    np.testing.assert_array_equal(results[4], results[6])
+    np.testing.assert_array_equal(results[5], results[7])
     assert not np.array_equal(results[0], results[2])  # FP32 against FP64 of the same strip: not bit-equal
 
 
