@@ -289,6 +289,17 @@ __device__ __forceinline__ long long to_fixed(R v) {
     return static_cast<long long>(rint(v * static_cast<R>(kFixed)));
 }
 
+// FP32: the same integer, rint(v 2^32), from two 32-bit conversions (gfx950 has no float-to-64-bit conversion, and
+// its generic expansion is 13 instructions): q = hi 2^16 + lo with hi = trunc(q / 2^16), the remainder exact in
+// float. Valid while |v| < 2^15, far beyond any adjoint weight here.
+template <>
+__device__ __forceinline__ long long to_fixed<float>(float v) {
+    const float q = rintf(v * static_cast<float>(kFixed));
+    const float hq = truncf(q * (1.0f / 65536.0f));
+    const int hi = static_cast<int>(hq), lo = static_cast<int>(fmaf(hq, -65536.0f, q));
+    return static_cast<long long>(static_cast<unsigned long long>(static_cast<long long>(hi)) << 16) + lo;
+}
+
 // 1 / x for the adjoint's weights: the hardware reciprocal in FP32 (1 ulp; an IEEE division is ten instructions in
 // a kernel that is bound by its instruction issue), the division in FP64.
 __device__ __forceinline__ float adj_rcp(float x) { return hw_rcp(x); }
