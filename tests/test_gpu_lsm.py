@@ -32,9 +32,13 @@ def engine_paths(model, market, times, n_paths, seed, first_pair):
     return spots[0]
 
 
-@pytest.mark.parametrize("case", ["bs_put", "bs_put_ragged", "lv_put", "lv_call"])
+@pytest.mark.parametrize("case", ["bs_put", "bs_put_ragged", "bs_put_deep_otm", "bs_put_low_vol", "lv_put", "lv_call"])
 def test_fp64_lsm_is_the_oracle_on_the_same_paths(case):
-    if case.startswith("bs_put"):
+    if case == "bs_put_deep_otm":  # S0 52 against K 40: few paths and a narrow band in the money at the early dates
+        model, market, strike, cp, spot = BlackScholesModel(SIGMA), Market.flat(52.0, R, 0.0), K, "put", 52.0
+    elif case == "bs_put_low_vol":  # at the money, sigma 0.08: the band in the money is a few percent wide
+        model, market, strike, cp, spot = BlackScholesModel(0.08), Market.flat(40.0, R, 0.0), K, "put", 40.0
+    elif case.startswith("bs_put"):
         model, market, strike, cp, spot = BlackScholesModel(SIGMA), Market.flat(S0, R, 0.0), K, "put", S0
     else:
         market, models = _golden_models()
@@ -68,7 +72,8 @@ def test_fp64_lsm_is_the_oracle_on_the_same_paths(case):
         assert back < 1e-12, (j, back)
         assert np.all(got.coefs[j, 4:] == 0.0)
     # the reference's lstsq on the same paths: the same exercise decisions up to ties
-    ref, _, _ = lo.lsm_price(S1, None, S2, None, disc, strike, spot, call=cp == "call", solver="lstsq")
+    ref, ref_se, _ = lo.lsm_price(S1, None, S2, None, disc, strike, spot, call=cp == "call", solver="lstsq")
+    print(case, "device", got.price, "lstsq", ref, "se", ref_se, flush=True)
     assert got.price == pytest.approx(ref, rel=1e-6)
 
 
