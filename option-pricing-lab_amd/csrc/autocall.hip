@@ -16,8 +16,9 @@
 //             workgroup recomputes its legs' SEG steps from the checkpoint, walks them backwards and adds the
 //             adjoint of every table entry it read into an LDS copy of the segment's rows as 64-bit fixed point
 //             (integer adds: the result does not depend on the order, so the Greeks are bit-reproducible); the
-//             rows are then written once to the workgroup's private table, and a reduction sums the workgroups
-//             of each of 16 batches. The tables' adjoints are contracted with dLV/dH on the device (surface.hip).
+//             segment's rows are then added into the table of the workgroup's batch (16 batches of consecutive
+//             workgroups) with 64-bit integer atomics in HBM, exact like the LDS adds. The batch tables' adjoints
+//             are contracted with dLV/dH on the device (surface.hip).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
