@@ -86,3 +86,55 @@ def test_the_fit_is_the_oracle_restatement_step_for_step():
     np.testing.assert_allclose(got[0], want[0], rtol=1e-13, atol=1e-15)
     assert got[1] == pytest.approx(want[1], rel=1e-12, abs=1e-15)
     assert np.linalg.norm(got[0] - target) < 0.05  # and it gets there
+
+
+def test_nearest_correlation_is_highams_published_example():
+    """Higham (2002), section 4: the nearest correlation matrix to the tridiagonal [-1, 2, -1] matrix of order 4,
+    printed there to four places; a matrix that already is a correlation comes back unchanged."""
+    from optlab.structured.multiasset import correlation_root, nearest_correlation
+
+    a = 2.0 * np.eye(4) - np.eye(4, k=1) - np.eye(4, k=-1)
+    x = nearest_correlation(a)
+    want = np.array([[1.0, -0.8084, 0.1916, 0.1068], [-0.8084, 1.0, -0.6562, 0.1916],
+                     [0.1916, -0.6562, 1.0, -0.8084], [0.1068, 0.1916, -0.8084, 1.0]])
+    np.testing.assert_allclose(x, want, atol=6e-5)
+    assert np.all(np.diag(x) == 1.0) and np.linalg.eigvalsh(x)[0] > -1e-12
+    r = correlation_root(x)
+    np.testing.assert_allclose(r @ r.T, x, atol=1e-12)
+    c = np.array([[1.0, 0.5, 0.2], [0.5, 1.0, 0.3], [0.2, 0.3, 1.0]])
+    assert np.array_equal(nearest_correlation(c), c)
+
+
+def test_low_matrix_gives_the_measured_matrix_back_at_w0_and_factors_are_aligned():
+    c = np.array([[1.0, 0.6, 0.45], [0.6, 1.0, 0.5], [0.45, 0.5, 1.0]])
+    low = lc.low_matrix(c, lc.W0)
+    off = ~np.eye(3, dtype=bool)
+    np.testing.assert_allclose(lc.W0 + (1.0 - lc.W0) * low[off], c[off], atol=1e-15)
+    from optlab.structured.multiasset import correlation_root
+
+    root = correlation_root(c)
+    f, sign, angle = lc.aligned_factors(c, root, lc.W0)
+    np.testing.assert_allclose(f @ f.T, low, atol=1e-12)
+    assert np.all(np.einsum("ij,ij->j", f[:, :-1], root[:, :-1]) >= 0) and f[:, -1].mean() > 0
+    assert sign * root[:, -1].mean() > 0
+    # the angle of the common factor: tan a = sqrt(1 - w0) mean(top column) / sqrt(w0)
+    assert np.tan(angle) == pytest.approx(np.sqrt(1.0 - lc.W0) * f[:, -1].mean() / np.sqrt(lc.W0), rel=1e-12)
+
+
+def test_equicorrelated_factors_draw_on_the_roots_own_columns():
+    """With repeated eigenvalues the eigen-solver's basis is arbitrary: the aligned factor's other columns are the
+    root's, scaled by 1 / sqrt(1 - w0), and at w0 the common factor cancels the second normal N and gives every
+    name the root's loading on the market normal, so the mixer's spots at w0 are the constant matrix's draws."""
+    from optlab.structured.multiasset import correlation_root
+
+    w0 = lc.W0
+    for n, rho in ((3, 0.5), (4, 0.6), (5, 0.45)):
+        c = np.full((n, n), rho) + (1.0 - rho) * np.eye(n)
+        root = correlation_root(c)
+        f, sign, a = lc.aligned_factors(c, root, w0)
+        np.testing.assert_allclose(np.sqrt(1.0 - w0) * f[:, :-1], root[:, :-1], atol=1e-12)
+        top = f[:, -1]
+        on_n = np.sqrt(w0) * np.sin(a) - np.sqrt(1.0 - w0) * top * np.cos(a)
+        on_m = np.sqrt(w0) * np.cos(a) + np.sqrt(1.0 - w0) * top * np.sin(a)
+        np.testing.assert_allclose(on_n, 0.0, atol=1e-12)
+        np.testing.assert_allclose(sign * on_m, root[:, -1], atol=1e-12)
