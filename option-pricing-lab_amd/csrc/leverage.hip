@@ -408,20 +408,21 @@ __global__ void __launch_bounds__(kRowThreads) prep_kernel(LevArgs s, int i) {
     const double start = before + incl - run;  // the counts before this thread's first fine bin
     __syncthreads();
     const double target[2] = {s.q_lo * total, s.q_hi * total};
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < 2; ++q) {  // each thread's first bin reaching the target, the wave's least, one LDS atomic
         double cum = start;
-        bool hit = false;
+        int cand = INT_MAX;
 #pragma unroll
         for (int m = 0; m < kPer; ++m) {
-            if (hit || k0 + m >= k1) continue;
+            if (cand != INT_MAX || k0 + m >= k1) continue;
             const double c = fc[m];
-            if (cum + c >= target[q] && c > 0) {
-                atomicMin(&first_hit[q], k0 + m);
-                hit = true;
-            } else {
+            if (cum + c >= target[q] && c > 0)
+                cand = k0 + m;
+            else
                 cum += c;
-            }
         }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) cand = min(cand, __shfl_xor(cand, off, 64));
+        if ((threadIdx.x & 63) == 0 && cand != INT_MAX) atomicMin(&first_hit[q], cand);
     }
     __syncthreads();
     const double width = (hi - lo) / static_cast<double>(nf);
@@ -676,6 +677,20 @@ __global__ void __launch_bounds__(kPersistThreads) slv_persistent_kernel(LevArgs
         for (int leg = 0; leg < 2; ++leg) xs[k][leg] = R(0), vs[k][leg] = static_cast<R>(s.v0);
 
     double lo = 0.0, hi = 0.0;  // the range of the particles at the start of the step (its bins' range)
+    R zn[PPL][2];
+    auto draw = [&](int step) {
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) {
+            const long long p = gid + k * lanes;
+            zn[k][0] = zn[k][1] = R(0);
+            if (p >= np) continue;
+            const U4 r4 = philox10(U4{static_cast<std::uint32_t>(p), static_cast<std::uint32_t>(static_cast<std::uint64_t>(p) >> 32),
+                                      static_cast<std::uint32_t>(step + 1), 0u},
+                                   s.k0, s.k1);
+            gauss2(r4.x, r4.y, zn[k][0], zn[k][1]);
+        }
+    };
+    draw(0);
     int shift_d = 0, shift_v = 0;
     unsigned wait_no = 0;
     for (int i = 0; i < s.n_steps; ++i) {
@@ -736,20 +751,21 @@ __global__ void __launch_bounds__(kPersistThreads) slv_persistent_kernel(LevArgs
             }
             const double start = before + static_cast<double>(incl - run);
             const double target[2] = {s.q_lo * total, s.q_hi * total};
-            for (int q = 0; q < 2; ++q) {
+            for (int q = 0; q < 2; ++q) {  // each thread's first bin reaching the target, the wave's least, one LDS atomic
                 double cum = start;
-                bool hit = false;
+                int cand = INT_MAX;
 #pragma unroll
                 for (int m = 0; m < kPer; ++m) {
-                    if (hit || k0 + m >= k1) continue;
+                    if (cand != INT_MAX || k0 + m >= k1) continue;
                     const double c = static_cast<double>(fc[m]);
-                    if (cum + c >= target[q] && c > 0) {
-                        atomicMin(&first_hit[q], k0 + m);
-                        hit = true;
-                    } else {
+                    if (cum + c >= target[q] && c > 0)
+                        cand = k0 + m;
+                    else
                         cum += c;
-                    }
                 }
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) cand = min(cand, __shfl_xor(cand, off, 64));
+                if (lane == 0 && cand != INT_MAX) atomicMin(&first_hit[q], cand);
             }
             // the bins' moments from the fixed-point totals (as prep_kernel), into B
             const double we = (hi + 1e-12 - lo) / nb;
@@ -841,11 +857,7 @@ __global__ void __launch_bounds__(kPersistThreads) slv_persistent_kernel(LevArgs
         for (int k = 0; k < PPL; ++k) {
             const long long p = gid + k * lanes;
             if (p >= np) continue;
-            const U4 r4 = philox10(U4{static_cast<std::uint32_t>(p), static_cast<std::uint32_t>(static_cast<std::uint64_t>(p) >> 32),
-                                      static_cast<std::uint32_t>(i + 1), 0u},
-                                   s.k0, s.k1);
-            R z0, z1;
-            gauss2(r4.x, r4.y, z0, z1);
+            const R z0 = zn[k][0], z1 = zn[k][1];
 #pragma unroll
             for (int leg = 0; leg < 2; ++leg) {
                 const R sg = leg ? R(-1) : R(1), zs = sg * z0, zv = sg * z1;
@@ -969,6 +981,7 @@ __global__ void __launch_bounds__(kPersistThreads) slv_persistent_kernel(LevArgs
                     __hip_atomic_fetch_add(t + k, static_cast<unsigned long long>(sums[k]), __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
         }
+        draw(i + 1);
         if (!grid_wait(ps, wait_no++)) return;
     }
 }

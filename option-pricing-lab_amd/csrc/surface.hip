@@ -226,12 +226,13 @@ __global__ void __launch_bounds__(256) node_derivative_kernel(SurfaceRows s, Ten
     }
 }
 
-// One workgroup per (row, surface); thread (b, na) sums over x, in order, the batch-b adjoint of the entry times its
-// derivative for active node na. Lanes of one batch read one adjoint value (a broadcast) and consecutive
-// derivatives (coalesced). The workgroup has a thread per (b, na), 320 for 16 batches x 20 nodes: with 256, one
-// wave walked the row a second time while the others waited (686 against 614 us a call); two threads per (b, na),
-// each on half the row, measured slower (667 us: fewer workgroups resident).
+// One workgroup per (row, surface): the row's batch adjoints and node derivatives are staged through LDS in chunks
+// of kChunk entries, each read once from HBM with coalesced loads; thread (b, na) then sums over x, in order, the
+// batch-b adjoint of the entry times its derivative for active node na, in four partial sums over x % 4 added in a
+// fixed order. (A first form read both straight from memory, thread by thread: 0.65 ms a call against the 520 MB it
+// reads.)
 constexpr int kContractThreads = 320;
+constexpr int kChunk = 128;
 
 // The adjoint of entry (a, row, x) of batch b is tbar[((a n_rows + row) n_x + x) x_stride + b batch_stride]: batches
 // interleaved (x_stride = n_batches, batch_stride = 1) or one table after the other (x_stride = 1, batch_stride =
@@ -240,26 +241,44 @@ template <typename Adj>
 __global__ void __launch_bounds__(kContractThreads) node_contraction_kernel(SurfaceSet set, DerivSet dset, int n_active,
                                                                const Adj* tbar, int n_batches, long long x_stride,
                                                                long long batch_stride, double unit, double* out) {
+    __shared__ double ts[kMaxBatches][kChunk + 1];  // + 1: the batches of one wave fall in different banks
+    __shared__ double ds[kChunk * 2 * kTentMaxStrikes];
     const long long row = blockIdx.x;
     const int a = blockIdx.y;
     const int n_x = set.s[a].n_x;
     const long long n_rows = set.s[a].n_rows;
-    for (int t = threadIdx.x; t < n_batches * n_active; t += blockDim.x) {
-        const int b = t / n_active, na = t - b * n_active;
-        const double* d = dset.d[a] + row * n_x * n_active + na;
-        const Adj* tb = tbar + (static_cast<long long>(a) * n_rows + row) * n_x * x_stride + b * batch_stride;
-        // four partial sums over x % 4 (loads in flight), added in a fixed order
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        int x = 0;
-        for (; x + 4 <= n_x; x += 4) {
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                acc[u] += static_cast<double>(tb[static_cast<long long>(x + u) * x_stride]) * d[(x + u) * n_active];
+    const long long base = (static_cast<long long>(a) * n_rows + row) * n_x;  // the row's first entry
+    const double* drow = dset.d[a] + row * n_x * n_active;
+    const int t = threadIdx.x, b = t / n_active, na = t - b * n_active;
+    const bool mine = t < n_batches * n_active;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int x0 = 0; x0 < n_x; x0 += kChunk) {
+        const int n = min(kChunk, n_x - x0);
+        if (x_stride == 1) {  // each batch's entries contiguous
+            for (int e = t; e < n_batches * n; e += blockDim.x) {
+                const int bb = e / n, xx = e - bb * n;
+                ts[bb][xx] = static_cast<double>(tbar[(base + x0 + xx) + bb * batch_stride]);
+            }
+        } else {  // (x, batch) pairs contiguous
+            for (int e = t; e < n_batches * n; e += blockDim.x) {
+                const int xx = e / n_batches, bb = e - xx * n_batches;
+                ts[bb][xx] = static_cast<double>(tbar[(base + x0 + xx) * x_stride + bb * batch_stride]);
+            }
         }
-        for (; x < n_x; ++x) acc[x & 3] += static_cast<double>(tb[static_cast<long long>(x) * x_stride]) * d[x * n_active];
-        out[((static_cast<long long>(a) * n_batches + b) * n_rows + row) * n_active + na] =
-            ((acc[0] + acc[1]) + (acc[2] + acc[3])) * unit;
+        for (int e = t; e < n * n_active; e += blockDim.x) ds[e] = drow[static_cast<long long>(x0) * n_active + e];
+        __syncthreads();
+        if (mine) {
+            int xx = 0;
+            for (; xx + 4 <= n; xx += 4) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) acc[(x0 + xx + u) & 3] += ts[b][xx + u] * ds[(xx + u) * n_active + na];
+            }
+            for (; xx < n; ++xx) acc[(x0 + xx) & 3] += ts[b][xx] * ds[xx * n_active + na];
+        }
+        __syncthreads();
     }
+    if (mine) out[((static_cast<long long>(a) * n_batches + b) * n_rows + row) * n_active + na] =
+        ((acc[0] + acc[1]) + (acc[2] + acc[3])) * unit;
 }
 
 // vega[(a * n_batches + b) * n_nodes + node] = the sum over rows of each row's part of node (i, j): one wave per
