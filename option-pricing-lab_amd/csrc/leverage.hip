@@ -654,6 +654,7 @@ __global__ void __launch_bounds__(kPersistThreads) slv_persistent_kernel(LevArgs
     __shared__ double qval[2];
     __shared__ int first_hit[2];
     __shared__ double sh_bw, sh_fallback;
+    __shared__ double fitv[kPersistMaxX];  // E[v | x] at the grid points
     constexpr int kPer = kMaxFine / kPersistThreads;
     constexpr int n_waves = kPersistThreads / 64;
     double* B = reinterpret_cast<double*>(fine);
@@ -827,18 +828,23 @@ __global__ void __launch_bounds__(kPersistThreads) slv_persistent_kernel(LevArgs
             ga = qa, gb = qa + (qb - qa) * (nx - 1) / (nx - 1);
         }
         __syncthreads();  // xev
-        // the table entries around every grid point, two lanes a point
-        for (int t = tid; t < 2 * nx; t += kPersistThreads) {
-            const int g = t >> 1, e = t & 1;
-            lvv[e][g] = local_var_part(s, i, xev[g], e);
+        // at once, on different waves: the table entries around every grid point (two lanes a point, the first half
+        // of the workgroup) and the regression at every grid point (as fit_kernel: a lane a point, the points spread
+        // over the second half's waves); then their quotient
+        constexpr int half = kPersistThreads / 2;
+        if (tid < half) {
+            for (int t = tid; t < 2 * nx; t += half) {
+                const int g = t >> 1, e = t & 1;
+                lvv[e][g] = local_var_part(s, i, xev[g], e);
+            }
+        } else {
+            const int t = tid - half, g = (t & 63) * (half / 64) + (t >> 6);
+            for (int gg = g; gg < nx; gg += half) fitv[gg] = i > 0 ? fit_point(s, B, xev[gg], sh_bw, sh_fallback, lo, hi) : s.v0;
         }
         __syncthreads();
-        // the regression at every grid point (as fit_kernel): a lane a point, the points spread over the waves
-        for (int t = tid; t < n_waves * 64; t += kPersistThreads) {
-            const int g = (t & 63) * n_waves + (t >> 6);
-            if (g >= nx) continue;
+        for (int g = tid; g < nx; g += kPersistThreads) {
             const double x0 = xev[g];
-            const double c = i > 0 ? fit_point(s, B, x0, sh_bw, sh_fallback, lo, hi) : s.v0;
+            const double c = fitv[g];
             const double loc = local_var_join(s, i, x0, lvv[0][g], lvv[1][g]);
             const double L = fmin(fmax(sqrt(loc / fmax(c, 1e-8)), s.l_min), s.l_max);
             lev[g] = static_cast<R>(L);
@@ -974,8 +980,16 @@ __global__ void __launch_bounds__(kPersistThreads) slv_persistent_kernel(LevArgs
         {
             unsigned* f = ps.fine_tot + static_cast<long long>(nxt) * nf;
             unsigned long long* t = reinterpret_cast<unsigned long long*>(ps.sum_tot + static_cast<long long>(nxt) * 5 * nb);
-            for (int k = tid; k < nf; k += kPersistThreads)
-                if (fine[k]) __hip_atomic_fetch_add(f + k, fine[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((nf & 1) == 0) {  // two bins a 64-bit atomic (a bin's total stays below 2^32: no carry crosses)
+                unsigned long long* f2 = reinterpret_cast<unsigned long long*>(f);
+                for (int q = tid; q < nf / 2; q += kPersistThreads) {
+                    const unsigned long long v = fine[2 * q] | (static_cast<unsigned long long>(fine[2 * q + 1]) << 32);
+                    if (v) __hip_atomic_fetch_add(f2 + q, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            } else {
+                for (int k = tid; k < nf; k += kPersistThreads)
+                    if (fine[k]) __hip_atomic_fetch_add(f + k, fine[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
             for (int k = tid; k < n_sums; k += kPersistThreads)
                 if (sums[k])
                     __hip_atomic_fetch_add(t + k, static_cast<unsigned long long>(sums[k]), __ATOMIC_RELAXED,
