@@ -97,7 +97,7 @@ struct Extremes {
 
 // paths.hpp:241-272, for the two legs of a lane's pair (output rows row[0], row[1]).
 template <typename R, int N>
-__device__ __forceinline__ void record(const PathArgs<R>& s, const long long (&row)[2], bool active, int j, int c,
+__device__ __forceinline__ void record(const PathArgs<R>& s, int n, const long long (&row)[2], bool active, int j, int c,
                                        const R (&fw)[N], const R (&x)[2][N], const R (&v)[2][N], Extremes<R, N>& e) {
     if (s.out_var && c >= 0 && active) {
 #pragma unroll
@@ -105,7 +105,7 @@ __device__ __forceinline__ void record(const PathArgs<R>& s, const long long (&r
     }
 #pragma unroll
     for (int a = 0; a < N; ++a) {
-        if (a >= s.n_names) break;
+        if (a >= n) break;
         const R f = fw[a];
         R S[2];
 #pragma unroll
@@ -149,7 +149,9 @@ __device__ __forceinline__ void record(const PathArgs<R>& s, const long long (&r
 // driver's stream), kept apart so that the other forms carry none of its registers;
 // kFormExercise, added to kFormCV or kFormGeneric (one name): nothing is written at the observations; an exercise
 // rule is applied there instead and the workgroup's sums of the pay are written.
-constexpr int kFormCV = 0, kFormGeneric = 1, kFormMixed = 2, kFormExercise = 4;
+// kFormTable: every name reads a local-variance table and none has a variance state or a mixer (local-vol names and
+// baskets): the model dispatch and the variance factor compile away, the tables staged as in kFormGeneric.
+constexpr int kFormCV = 0, kFormGeneric = 1, kFormMixed = 2, kFormTable = 3, kFormExercise = 4;
 
 // The exercise state of a lane's two legs: the exercise value (in units of the strike) and its date.
 struct Exercise {
@@ -221,12 +223,15 @@ template <typename R, int N, int FORM>
 __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
     constexpr bool EXER = (FORM & kFormExercise) != 0;
     constexpr bool CV = (FORM & 3) == kFormCV;
+    constexpr bool TAB = (FORM & 3) == kFormTable;
     static_assert(!EXER || (N == 1 && (FORM & 3) != kFormMixed), "an exercise rule is on one name, no mixer");
     extern __shared__ unsigned char lds_raw[];
     R* lds = reinterpret_cast<R*>(lds_raw);
     const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool active = p < s.n_pairs;
-    const int n = s.n_names;
+    // the dispatch launches N = 1, 2, 3, 4 for exactly that many names (N = 8 for 5 to 8): a constant count folds
+    // every per-name bound away
+    const int n = N <= 4 ? N : s.n_names;
     const long long row[2] = {p, s.n_pairs + p};
     const std::uint64_t pair = static_cast<std::uint64_t>(s.first_pair + (active ? p : 0));
 
@@ -234,13 +239,13 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
     int kind[N];
 #pragma unroll
     for (int a = 0; a < N; ++a) {
-        kind[a] = a < n ? ldc(s.kind, a) : 0;
+        kind[a] = TAB ? 1 : (a < n ? ldc(s.kind, a) : 0);
         cvar[a] = a < n ? ldc(s.cvar, a) : R(0);
         const bool hv = kind[a] == 2 || kind[a] == 3;
         x[0][a] = x[1][a] = R(0);
         v[0][a] = v[1][a] = (a < n && hv) ? ldc(s.hes, a * 5) : R(0);
     }
-    const bool staged = !CV && s.staged, has_var = !CV && s.has_var;
+    const bool staged = !CV && s.staged, has_var = !CV && !TAB && s.has_var;
     constexpr bool mixer = FORM == kFormMixed;
     Extremes<R, N> ext;
     Exercise exs{{0.0, 0.0}, {s.n_obs - 1, s.n_obs - 1}, {false, false}};
@@ -265,7 +270,7 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
             R fw[N];
 #pragma unroll
             for (int a = 0; a < N; ++a) fw[a] = next_fw[a];
-            record<R, N>(s, row, active, jj, next_col, fw, x, v, ext);
+            record<R, N>(s, n, row, active, jj, next_col, fw, x, v, ext);
         }
     };
     int j = 0;
@@ -392,7 +397,7 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
 #pragma unroll
             for (int a = 0; a < N; ++a) {
                 if (a >= n) break;
-                const int k = CV ? 0 : kind[a];
+                const int k = CV ? 0 : (TAB ? 1 : kind[a]);
                 R var;
                 const R vp = r_max(v[leg][a], R(0));
                 if (k == 0) {
@@ -472,6 +477,8 @@ void launch_n(const PathArgs<R>& s, size_t lds, hipStream_t stream) {
         launch_form<R, N, kFormMixed>(s, lds, stream);
     else if (s.n_tab == 0 && !s.has_var)
         launch_form<R, N, kFormCV>(s, lds, stream);
+    else if (s.n_tab == s.n_names && !s.has_var)
+        launch_form<R, N, kFormTable>(s, lds, stream);
     else
         launch_form<R, N, kFormGeneric>(s, lds, stream);
 }
@@ -553,6 +560,7 @@ double launch_typed(const optlab_paths_request& rq, double* d_out, double* d_var
     hip_check(hipEventRecord(e0, stream), "event record");
     if (n <= 1) launch_n<R, 1>(s, lds, stream);
     else if (n <= 2) launch_n<R, 2>(s, lds, stream);
+    else if (n == 3) launch_n<R, 3>(s, lds, stream);
     else if (n <= 4) launch_n<R, 4>(s, lds, stream);
     else launch_n<R, kMaxNames>(s, lds, stream);
     hip_check(hipGetLastError(), "paths kernel launch");
