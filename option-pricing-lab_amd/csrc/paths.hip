@@ -3,9 +3,10 @@
 // Semantics: packages/native/cuda/include/optlab_cuda/paths.hpp:196-361 (simulate_pair, record, table_value).
 //
 // MI355X layout: one antithetic pair per lane, state (x, v, window extremes, the normal buffer) in VGPRs with every
-// per-name loop unrolled to the compile-time bound N so nothing spills; the table rows of the current step are
-// staged in LDS by the workgroup (double-buffered: the rows of step i+1 are copied while step i runs, one barrier
-// per step); output columns are structure-of-arrays, so a wave writes 64 consecutive doubles per column.
+// per-name loop unrolled to the compile-time bound N so nothing spills; the table rows of the current step are read
+// where they lie (a step's rows are a few KB, every lane of the launch reads the same ones: they stay in L2 and the
+// lanes' gathers hit L1/L2; staging them in LDS with a barrier a step was measured 1.2-2.7x slower on every model);
+// output columns are structure-of-arrays, so a wave writes 64 consecutive doubles per column.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -20,7 +21,6 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxNames = 8;
-constexpr size_t kStageBytes = 64 * 1024;  // LDS for two buffers of the step's table rows
 
 // An exercise rule in device memory (one pointer in the kernel's arguments: the fields are loaded where the rule is
 // applied, so they hold no scalar registers over the step loop).
@@ -60,24 +60,10 @@ struct PathArgs {
     int win_lo[2], win_hi[2];
     double* out;
     double* out_var;     // or null: the variance factor of name 0 at the kept columns, as out's first name
-    int n_tab;           // names reading a table
-    int tab_of[8];       // those names, in order
-    int tab_slot[8];     // slot of name a among them (-1: none)
-    int staged;          // 1: the step's rows are copied to LDS
+    int n_tab;           // names reading a table (the form's choice)
     int has_var, has_slv;
     const ExerciseDev* ex;  // kFormExercise: the rule applied at the observations of one name, in place of writing
 };
-
-template <typename R>
-__device__ __forceinline__ void stage_rows(const PathArgs<R>& s, int step_index, R* buf) {
-    for (int t = 0; t < s.n_tab; ++t) {
-        const int a = s.tab_of[t];
-        const int row = ldc(s.step_row, a * s.n_steps + step_index);
-        const R* src = s.table + static_cast<long long>(row) * s.n_x;
-        R* dst = buf + t * s.n_x;
-        for (int c = threadIdx.x; c < s.n_x; c += blockDim.x) dst[c] = src[c];
-    }
-}
 
 // paths.hpp:196-205 on a row held at y.
 template <typename R>
@@ -150,7 +136,7 @@ __device__ __forceinline__ void record(const PathArgs<R>& s, int n, const long l
 // kFormExercise, added to kFormCV or kFormGeneric (one name): nothing is written at the observations; an exercise
 // rule is applied there instead and the workgroup's sums of the pay are written.
 // kFormTable: every name reads a local-variance table and none has a variance state or a mixer (local-vol names and
-// baskets): the model dispatch and the variance factor compile away, the tables staged as in kFormGeneric.
+// baskets): the model dispatch and the variance factor compile away.
 constexpr int kFormCV = 0, kFormGeneric = 1, kFormMixed = 2, kFormTable = 3, kFormExercise = 4;
 
 // The exercise state of a lane's two legs: the exercise value (in units of the strike) and its date.
@@ -225,8 +211,6 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
     constexpr bool CV = (FORM & 3) == kFormCV;
     constexpr bool TAB = (FORM & 3) == kFormTable;
     static_assert(!EXER || (N == 1 && (FORM & 3) != kFormMixed), "an exercise rule is on one name, no mixer");
-    extern __shared__ unsigned char lds_raw[];
-    R* lds = reinterpret_cast<R*>(lds_raw);
     const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool active = p < s.n_pairs;
     // the dispatch launches N = 1, 2, 3, 4 for exactly that many names (N = 8 for 5 to 8): a constant count folds
@@ -245,7 +229,7 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
         x[0][a] = x[1][a] = R(0);
         v[0][a] = v[1][a] = (a < n && hv) ? ldc(s.hes, a * 5) : R(0);
     }
-    const bool staged = !CV && s.staged, has_var = !CV && !TAB && s.has_var;
+    const bool has_var = !CV && !TAB && s.has_var;
     constexpr bool mixer = FORM == kFormMixed;
     Extremes<R, N> ext;
     Exercise exs{{0.0, 0.0}, {s.n_obs - 1, s.n_obs - 1}, {false, false}};
@@ -286,16 +270,9 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
     if (mixer) drv.init(pair, s.key0, s.key1, kMixTag);
     const bool jacobi = mixer && s.mixer == 2;  // the stochastic weight: per-leg state, its own driver
     R wst[2] = {s.mix[0], s.mix[0]};
-    const int stage_len = s.n_tab * s.n_x;
-    if (staged && s.n_steps > 0) stage_rows(s, 0, lds);
 
     R dt_next = s.n_steps > 0 ? ldc(s.dt, 0) : R(0);
     for (int i = 1; i <= s.n_steps; ++i) {
-        const R* cur = lds + ((i - 1) & 1) * stage_len;
-        if (staged) {
-            __syncthreads();
-            if (i < s.n_steps) stage_rows(s, i, lds + (i & 1) * stage_len);
-        }
         const R dt = dt_next, t0 = mixer ? ldc(s.t0, i - 1) : R(0);
         if (i < s.n_steps) dt_next = ldc(s.dt, i);
         R zs[N], zr[N], fresh = R(0);
@@ -406,7 +383,7 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
                     var = vp;
                 } else {
                     const int row = ldc(s.step_row, a * s.n_steps + (i - 1));
-                    const R* y = staged ? cur + s.tab_slot[a] * s.n_x : s.table + static_cast<long long>(row) * s.n_x;
+                    const R* y = s.table + static_cast<long long>(row) * s.n_x;
                     const R t = lerp_row(y, ldc(s.lo, row), ldc(s.inv_dx, row), s.n_x, x[leg][a]);
                     var = k == 1 ? t : t * vp;
                 }
@@ -457,30 +434,30 @@ struct Arena {
 };
 
 template <typename R, int N, int FORM>
-void launch_form(const PathArgs<R>& s, size_t lds, hipStream_t stream) {
+void launch_form(const PathArgs<R>& s, hipStream_t stream) {
     const unsigned grid = static_cast<unsigned>((s.n_pairs + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL((paths_kernel<R, N, FORM>), dim3(grid), dim3(kBlock), lds, stream, s);
+    hipLaunchKernelGGL((paths_kernel<R, N, FORM>), dim3(grid), dim3(kBlock), 0, stream, s);
 }
 
 template <typename R, int N>
-void launch_n(const PathArgs<R>& s, size_t lds, hipStream_t stream) {
+void launch_n(const PathArgs<R>& s, hipStream_t stream) {
     if constexpr (N == 1) {
         if (s.ex) {
             if (s.n_tab == 0 && !s.has_var)
-                launch_form<R, 1, kFormCV | kFormExercise>(s, lds, stream);
+                launch_form<R, 1, kFormCV | kFormExercise>(s, stream);
             else
-                launch_form<R, 1, kFormGeneric | kFormExercise>(s, lds, stream);
+                launch_form<R, 1, kFormGeneric | kFormExercise>(s, stream);
             return;
         }
     }
     if (s.mixer)
-        launch_form<R, N, kFormMixed>(s, lds, stream);
+        launch_form<R, N, kFormMixed>(s, stream);
     else if (s.n_tab == 0 && !s.has_var)
-        launch_form<R, N, kFormCV>(s, lds, stream);
+        launch_form<R, N, kFormCV>(s, stream);
     else if (s.n_tab == s.n_names && !s.has_var)
-        launch_form<R, N, kFormTable>(s, lds, stream);
+        launch_form<R, N, kFormTable>(s, stream);
     else
-        launch_form<R, N, kFormGeneric>(s, lds, stream);
+        launch_form<R, N, kFormGeneric>(s, stream);
 }
 
 template <typename R>
@@ -543,26 +520,22 @@ double launch_typed(const optlab_paths_request& rq, double* d_out, double* d_var
     }
     s.n_tab = 0;
     s.has_var = s.has_slv = 0;
-    for (int a = 0; a < 8; ++a) s.tab_slot[a] = -1, s.tab_of[a] = 0;
     for (int a = 0; a < n; ++a) {
         const int k = rq.kind[a];
-        if (k == 1 || k == 3) s.tab_slot[a] = s.n_tab, s.tab_of[s.n_tab++] = a;
+        s.n_tab += k == 1 || k == 3;
         s.has_var |= (k == 2 || k == 3);
         s.has_slv |= k == 3;
     }
-    const size_t stage_bytes = 2 * static_cast<size_t>(s.n_tab) * rq.n_x * sizeof(R);
-    s.staged = s.n_tab > 0 && rq.n_steps > 0 && stage_bytes <= kStageBytes;
-    const size_t lds = s.staged ? stage_bytes : 0;
 
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "event");
     hip_check(hipEventCreate(&e1), "event");
     hip_check(hipEventRecord(e0, stream), "event record");
-    if (n <= 1) launch_n<R, 1>(s, lds, stream);
-    else if (n <= 2) launch_n<R, 2>(s, lds, stream);
-    else if (n == 3) launch_n<R, 3>(s, lds, stream);
-    else if (n <= 4) launch_n<R, 4>(s, lds, stream);
-    else launch_n<R, kMaxNames>(s, lds, stream);
+    if (n <= 1) launch_n<R, 1>(s, stream);
+    else if (n <= 2) launch_n<R, 2>(s, stream);
+    else if (n == 3) launch_n<R, 3>(s, stream);
+    else if (n <= 4) launch_n<R, 4>(s, stream);
+    else launch_n<R, kMaxNames>(s, stream);
     hip_check(hipGetLastError(), "paths kernel launch");
     hip_check(hipEventRecord(e1, stream), "event record");
     hip_check(hipEventSynchronize(e1), "paths kernel");
