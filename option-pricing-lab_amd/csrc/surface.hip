@@ -226,59 +226,65 @@ __global__ void __launch_bounds__(256) node_derivative_kernel(SurfaceRows s, Ten
     }
 }
 
-// One workgroup per (row, surface): the row's batch adjoints and node derivatives are staged through LDS in chunks
-// of kChunk entries, each read once from HBM with coalesced loads; thread (b, na) then sums over x, in order, the
-// batch-b adjoint of the entry times its derivative for active node na, in four partial sums over x % 4 added in a
-// fixed order. (A first form read both straight from memory, thread by thread: 0.65 ms a call against the 520 MB it
-// reads.)
-constexpr int kContractThreads = 320;
-constexpr int kChunk = 128;
+// One workgroup per (row, surface): for that row, the (batch x node) products sum over x of the batch adjoint of
+// each entry times its node derivative -- a 16 x n_x by n_x x 20 matrix product -- on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64: 16 batches x 16 nodes, 4 entries a step; two node tiles cover 20 nodes). Each of the
+// workgroup's four waves takes a quarter of the row's entries, reads its adjoints and derivatives straight from
+// memory (every byte of the 520 MB once), and the four partial tiles are added in LDS in wave order: a fixed order,
+// so the result does not depend on scheduling. (A thread per (batch, node) summing in order took 0.65 ms a call
+// reading from memory and 0.385 ms staged through LDS.)
+// Lane layout of the 16 x 16 x 4 FP64 product (checked on the device): lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16] and holds D[4 k + l / 16][l % 16] in its k-th accumulator.
+constexpr int kContractWaves = 4;
+typedef double v4d __attribute__((ext_vector_type(4)));
 
 // The adjoint of entry (a, row, x) of batch b is tbar[((a n_rows + row) n_x + x) x_stride + b batch_stride]: batches
 // interleaved (x_stride = n_batches, batch_stride = 1) or one table after the other (x_stride = 1, batch_stride =
 // the entries of a table).
 template <typename Adj>
-__global__ void __launch_bounds__(kContractThreads) node_contraction_kernel(SurfaceSet set, DerivSet dset, int n_active,
-                                                               const Adj* tbar, int n_batches, long long x_stride,
-                                                               long long batch_stride, double unit, double* out) {
-    __shared__ double ts[kMaxBatches][kChunk + 1];  // + 1: the batches of one wave fall in different banks
-    __shared__ double ds[kChunk * 2 * kTentMaxStrikes];
+__global__ void __launch_bounds__(64 * kContractWaves) node_contraction_kernel(SurfaceSet set, DerivSet dset,
+                                                                             int n_active, const Adj* tbar,
+                                                                             int n_batches, long long x_stride,
+                                                                             long long batch_stride, double unit,
+                                                                             double* out) {
+    __shared__ double part[kContractWaves][2][4][64];
     const long long row = blockIdx.x;
     const int a = blockIdx.y;
     const int n_x = set.s[a].n_x;
     const long long n_rows = set.s[a].n_rows;
     const long long base = (static_cast<long long>(a) * n_rows + row) * n_x;  // the row's first entry
     const double* drow = dset.d[a] + row * n_x * n_active;
-    const int t = threadIdx.x, b = t / n_active, na = t - b * n_active;
-    const bool mine = t < n_batches * n_active;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int x0 = 0; x0 < n_x; x0 += kChunk) {
-        const int n = min(kChunk, n_x - x0);
-        if (x_stride == 1) {  // each batch's entries contiguous
-            for (int e = t; e < n_batches * n; e += blockDim.x) {
-                const int bb = e / n, xx = e - bb * n;
-                ts[bb][xx] = static_cast<double>(tbar[(base + x0 + xx) + bb * batch_stride]);
-            }
-        } else {  // (x, batch) pairs contiguous
-            for (int e = t; e < n_batches * n; e += blockDim.x) {
-                const int xx = e / n_batches, bb = e - xx * n_batches;
-                ts[bb][xx] = static_cast<double>(tbar[(base + x0 + xx) * x_stride + bb * batch_stride]);
-            }
-        }
-        for (int e = t; e < n * n_active; e += blockDim.x) ds[e] = drow[static_cast<long long>(x0) * n_active + e];
-        __syncthreads();
-        if (mine) {
-            int xx = 0;
-            for (; xx + 4 <= n; xx += 4) {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) acc[(x0 + xx + u) & 3] += ts[b][xx + u] * ds[(xx + u) * n_active + na];
-            }
-            for (; xx < n; ++xx) acc[(x0 + xx) & 3] += ts[b][xx] * ds[xx * n_active + na];
-        }
-        __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, kk = lane >> 4;  // A row (batch) / B column (node), and the entry within the step
+    const bool batch_ok = m < n_batches;
+    const Adj* tb = tbar + m * batch_stride;
+    // this wave's entries: steps of 4 split evenly over the waves
+    const int steps = (n_x + 3) / 4, per = (steps + kContractWaves - 1) / kContractWaves;
+    const int x_lo = 4 * per * wave, x_hi = min(n_x, 4 * per * (wave + 1));
+    v4d c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
+    for (int x0 = x_lo; x0 < x_hi; x0 += 4) {
+        const int x = x0 + kk;
+        const bool in = x < x_hi;
+        const double av = (in && batch_ok) ? static_cast<double>(tb[(base + x) * x_stride]) : 0.0;
+        const double* d = drow + static_cast<long long>(x) * n_active;
+        const double b0 = (in && m < n_active) ? d[m] : 0.0;
+        const double b1 = (in && m + 16 < n_active) ? d[m + 16] : 0.0;
+        c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b0, c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b1, c1, 0, 0, 0);
     }
-    if (mine) out[((static_cast<long long>(a) * n_batches + b) * n_rows + row) * n_active + na] =
-        ((acc[0] + acc[1]) + (acc[2] + acc[3])) * unit;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) part[wave][0][k][lane] = c0[k], part[wave][1][k][lane] = c1[k];
+    __syncthreads();
+    // thread t < 2 x 4 x 64 adds the waves' values of one (tile, k, lane) slot in wave order and writes it
+    for (int t = threadIdx.x; t < 2 * 4 * 64; t += blockDim.x) {
+        const int tile = t >> 8, k = (t >> 6) & 3, l = t & 63;
+        double v = part[0][tile][k][l];
+#pragma unroll
+        for (int w = 1; w < kContractWaves; ++w) v += part[w][tile][k][l];
+        const int b = 4 * k + (l >> 4), na = (l & 15) + 16 * tile;
+        if (b < n_batches && na < n_active)
+            out[((static_cast<long long>(a) * n_batches + b) * n_rows + row) * n_active + na] = v * unit;
+    }
 }
 
 // vega[(a * n_batches + b) * n_nodes + node] = the sum over rows of each row's part of node (i, j): one wave per
@@ -358,8 +364,8 @@ void contract(const SurfaceSet& set, const DerivSet& dset, const TentGrid& g, co
     const long long n_rows = set.s[0].n_rows;
     double* per_row = static_cast<double*>(
         scratch(kSlotNodes, static_cast<size_t>(set.n) * n_batches * n_rows * n_active * sizeof(double)));
-    const int threads = (n_batches * n_active + 63) / 64 * 64;  // a thread per (batch, node), whole waves
-    if (threads > kContractThreads) invalid("at most 320 (batch, node) pairs a row");
+    if (n_active > 32) invalid("at most 32 active nodes a row (two 16-node tiles)");
+    const int threads = 64 * kContractWaves;
     const long long table_entries = static_cast<long long>(set.n) * n_rows * set.s[0].n_x;
     const long long x_stride = batch_major ? 1 : n_batches, batch_stride = batch_major ? table_entries : 1;
     hipLaunchKernelGGL(node_contraction_kernel<A>, dim3(static_cast<unsigned>(n_rows), set.n), dim3(threads), 0,
