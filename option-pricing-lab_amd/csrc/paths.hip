@@ -33,6 +33,14 @@ struct ExerciseDev {
 };
 typedef const __attribute__((address_space(4))) ExerciseDev* ExerciseConst;
 
+// One observation of the schedule, loaded with one scalar load: the step after which it falls, its output column
+// (-1: not kept) and its window flags (bit w: inside window w; bit 2 + w: its first observation; bit 4 + w: its
+// last), formed on the host from the windows' index ranges. (The step, the column and the windows' bounds read and
+// compared one by one cost the fixing-heavy pass 4-6 % and the phoenix script's daily window 8 %.)
+struct ObsRec {
+    int step, col, flags, pad;
+};
+
 template <typename R>
 struct PathArgs {
     int n_names, n_steps, n_obs, n_x;
@@ -40,7 +48,7 @@ struct PathArgs {
     std::uint32_t key0, key1;
     const R* dt;
     const R* t0;
-    const int* obs_step;
+    const ObsRec* orec;  // n_obs
     const R* root;
     const int* kind;
     const R* cvar;
@@ -55,9 +63,6 @@ struct PathArgs {
     R mix[8];
     const R* low_root;
     int n_cols, n_keep;
-    const int* col_of;
-    int n_win;
-    int win_lo[2], win_hi[2];
     double* out;
     double* out_var;     // or null: the variance factor of name 0 at the kept columns, as out's first name
     int n_tab;           // names reading a table (the form's choice)
@@ -83,7 +88,7 @@ struct Extremes {
 
 // paths.hpp:241-272, for the two legs of a lane's pair (output rows row[0], row[1]).
 template <typename R, int N>
-__device__ __forceinline__ void record(const PathArgs<R>& s, int n, const long long (&row)[2], bool active, int j, int c,
+__device__ __forceinline__ void record(const PathArgs<R>& s, int n, const long long (&row)[2], bool active, int flags, int c,
                                        const R (&fw)[N], const R (&x)[2][N], const R (&v)[2][N], Extremes<R, N>& e) {
     if (s.out_var && c >= 0 && active) {
 #pragma unroll
@@ -101,19 +106,20 @@ __device__ __forceinline__ void record(const PathArgs<R>& s, int n, const long l
 #pragma unroll
             for (int l = 0; l < 2; ++l) base[c * s.rows + row[l]] = static_cast<double>(S[l]);
         }
+        if (!flags) continue;
 #pragma unroll
         for (int w = 0; w < 2; ++w) {
-            if (w >= s.n_win || j < s.win_lo[w] || j > s.win_hi[w]) continue;
+            if (!(flags & (1 << w))) continue;
 #pragma unroll
             for (int l = 0; l < 2; ++l) {
-                if (j == s.win_lo[w]) {
+                if (flags & (4 << w)) {
                     e.lo[l][a][w] = e.hi[l][a][w] = S[l];
                 } else {
                     e.lo[l][a][w] = r_min(e.lo[l][a][w], S[l]);
                     e.hi[l][a][w] = r_max(e.hi[l][a][w], S[l]);
                 }
             }
-            if (j == s.win_hi[w] && active) {
+            if ((flags & (16 << w)) && active) {
                 const long long cl = s.n_keep + 2 * w, ch = cl + 1;
 #pragma unroll
                 for (int l = 0; l < 2; ++l) {
@@ -128,7 +134,7 @@ __device__ __forceinline__ void record(const PathArgs<R>& s, int n, const long l
 // paths.hpp:274-361: lane p carries both legs of pair p (output rows p and n_pairs + p).
 // The kernel's form, by what the request needs:
 // kFormCV: every name has a constant variance and there is no mixer (Black-Scholes names and baskets), so the
-// model dispatch, the variance factor and the table staging compile away: the same arithmetic on fewer scalar
+// model dispatch, the variance factor and the table reads compile away: the same arithmetic on fewer scalar
 // instructions a step;
 // kFormGeneric: tables, Heston and SLV names, no mixer;
 // kFormMixed: the generic form with a mixer (local correlation, or the stochastic weight with its state and its
@@ -236,13 +242,15 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
     // The next observation's step, output column and forwards are loaded one observation ahead, and the next
     // step's dt one step ahead: a scalar load's wait then falls after a step of vector work instead of at its
     // start (every step waited for its constants when each was loaded where it was used).
-    int next_step = -1, next_col = -1;
+    int next_step = -1, next_col = -1, next_flags = 0;
     R next_fw[N];
     auto load_obs = [&](int jj) {
         const bool in = jj < s.n_obs;
-        next_step = in ? ldc(s.obs_step, jj) : -1;
+        typedef int v4i __attribute__((ext_vector_type(4)));
+        const v4i o = in ? ldc(reinterpret_cast<const v4i*>(s.orec), jj) : v4i{-1, -1, 0, 0};
+        next_step = o.x;
         if constexpr (!EXER) {
-            next_col = in ? ldc(s.col_of, jj) : -1;
+            next_col = o.y, next_flags = o.z;
 #pragma unroll
             for (int a = 0; a < N; ++a) next_fw[a] = (in && a < n) ? ldc(s.fwd, a * s.n_obs + jj) : R(0);
         }
@@ -254,7 +262,7 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
             R fw[N];
 #pragma unroll
             for (int a = 0; a < N; ++a) fw[a] = next_fw[a];
-            record<R, N>(s, n, row, active, jj, next_col, fw, x, v, ext);
+            record<R, N>(s, n, row, active, next_flags, next_col, fw, x, v, ext);
         }
     };
     int j = 0;
@@ -467,7 +475,18 @@ double launch_typed(const optlab_paths_request& rq, double* d_out, double* d_var
     const long long nsn = static_cast<long long>(n) * rq.n_steps;
     Arena ar;
     const size_t o_dt = ar.reals<R>(rq.dt, rq.n_steps), o_t0 = ar.reals<R>(rq.t0, rq.n_steps);
-    const size_t o_obs = ar.ints(rq.obs_step, rq.n_obs);
+    std::vector<ObsRec> orec(static_cast<size_t>(std::max(rq.n_obs, 1)));
+    for (int j = 0; j < rq.n_obs; ++j) {
+        int flags = 0;
+        for (int w = 0; w < rq.n_windows && w < 2; ++w) {
+            if (j < rq.win_lo[w] || j > rq.win_hi[w]) continue;
+            flags |= 1 << w;
+            if (j == rq.win_lo[w]) flags |= 4 << w;
+            if (j == rq.win_hi[w]) flags |= 16 << w;
+        }
+        orec[j] = ObsRec{rq.obs_step[j], rq.col_of ? rq.col_of[j] : -1, flags, 0};
+    }
+    const size_t o_obs = ar.put(orec.data(), orec.size() * sizeof(ObsRec));
     const size_t o_root = ar.reals<R>(rq.root, n * n), o_kind = ar.ints(rq.kind, n);
     const size_t o_cvar = ar.reals<R>(rq.cvar, n), o_hes = ar.reals<R>(rq.heston, 5 * n);
     bool any_table = false;
@@ -478,7 +497,6 @@ double launch_typed(const optlab_paths_request& rq, double* d_out, double* d_var
     const size_t o_tab = ar.reals<R>(any_table ? rq.table : nullptr, static_cast<long long>(rq.n_rows) * rq.n_x);
     const size_t o_fwd = ar.reals<R>(rq.fwd, static_cast<long long>(n) * rq.n_obs);
     const size_t o_low = ar.reals<R>(rq.mixer ? rq.low_root : nullptr, n * n);
-    const size_t o_col = ar.ints(rq.col_of, rq.n_obs);
     size_t o_ex = 0;
     if (rule) {
         if (n != 1 || rq.mixer) invalid("an exercise rule is on one name, no mixer");
@@ -496,7 +514,7 @@ double launch_typed(const optlab_paths_request& rq, double* d_out, double* d_var
     s.key0 = static_cast<std::uint32_t>(rq.seed), s.key1 = static_cast<std::uint32_t>(rq.seed >> 32);
     auto at = [&](size_t o) { return dev + o; };
     s.dt = reinterpret_cast<const R*>(at(o_dt)), s.t0 = reinterpret_cast<const R*>(at(o_t0));
-    s.obs_step = reinterpret_cast<const int*>(at(o_obs));
+    s.orec = reinterpret_cast<const ObsRec*>(at(o_obs));
     s.root = reinterpret_cast<const R*>(at(o_root)), s.kind = reinterpret_cast<const int*>(at(o_kind));
     s.cvar = reinterpret_cast<const R*>(at(o_cvar)), s.hes = reinterpret_cast<const R*>(at(o_hes));
     s.same = rq.same;
@@ -506,12 +524,7 @@ double launch_typed(const optlab_paths_request& rq, double* d_out, double* d_var
     s.mixer = rq.mixer;
     for (int k = 0; k < 8; ++k) s.mix[k] = static_cast<R>(rq.mix[k]);
     s.low_root = reinterpret_cast<const R*>(at(o_low));
-    s.n_cols = rq.n_cols, s.n_keep = rq.n_keep, s.col_of = reinterpret_cast<const int*>(at(o_col));
-    s.n_win = rq.n_windows;
-    for (int w = 0; w < 2; ++w) {
-        s.win_lo[w] = w < rq.n_windows ? rq.win_lo[w] : 0;
-        s.win_hi[w] = w < rq.n_windows ? rq.win_hi[w] : -1;
-    }
+    s.n_cols = rq.n_cols, s.n_keep = rq.n_keep;
     s.out = d_out;
     s.out_var = d_var;
     if (rule) {
