@@ -87,8 +87,8 @@ struct AcArgs {
 
 // The normals of one step: n spot normals from stream 0, then (VAR) n variance normals from the aux stream or
 // stream 0 -- the order of the path engine's steps -- correlated into z and w for the leg of sign `sign`.
-template <typename R, int N, bool VAR>
-__device__ __forceinline__ void step_normals(const AcArgs<R>& s, NormalStream<R>& rng, NormalStream<R>& aux, R sign,
+template <typename R, int N, bool VAR, typename Stream>
+__device__ __forceinline__ void step_normals(const AcArgs<R>& s, Stream& rng, Stream& aux, R sign,
                                              R (&z)[N], R (&w)[N]) {
     R zs[N], zr[N];
 #pragma unroll
@@ -291,14 +291,18 @@ __device__ __forceinline__ long long to_fixed(R v) {
 }
 
 // FP32: the same integer, rint(v 2^32), from two 32-bit conversions (gfx950 has no float-to-64-bit conversion, and
-// its generic expansion is 13 instructions): q = hi 2^16 + lo with hi = trunc(q / 2^16), the remainder exact in
-// float. Valid while |v| < 2^15, far beyond any adjoint weight here.
+// its generic expansion is 13 instructions): q = hi 2^16 + lo with hi = floor(q / 2^16) and 0 <= lo < 2^16, both exact
+// in float, so the 64-bit integer's low word is hi << 16 | lo and its high word hi >> 16 (arithmetic). Valid while
+// |v| < 2^15, far beyond any adjoint weight here.
 template <>
 __device__ __forceinline__ long long to_fixed<float>(float v) {
     const float q = rintf(v * static_cast<float>(kFixed));
-    const float hq = truncf(q * (1.0f / 65536.0f));
-    const int hi = static_cast<int>(hq), lo = static_cast<int>(fmaf(hq, -65536.0f, q));
-    return static_cast<long long>(static_cast<unsigned long long>(static_cast<long long>(hi)) << 16) + lo;
+    const float hq = floorf(q * (1.0f / 65536.0f));
+    const int hi = static_cast<int>(hq);
+    const unsigned lo = static_cast<unsigned>(fmaf(hq, -65536.0f, q));
+    const unsigned low = (static_cast<unsigned>(hi) << 16) | lo;
+    const unsigned high = static_cast<unsigned>(hi >> 16);
+    return static_cast<long long>((static_cast<unsigned long long>(high) << 32) | low);
 }
 
 // 1 / x for the adjoint's weights: the hardware reciprocal in FP32 (1 ulp; an IEEE division is ten instructions in
@@ -319,10 +323,21 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
     // x-bar of the workgroup's legs from one segment to the one before: in LDS after the rows when xl_legs > 0
     // ([name][xl_legs], a lane's own entries only), else in device memory
     R* xl = reinterpret_cast<R*>(acc + SEG * row_len);
+    // The rows start at zero; the flush at the end of each segment clears what it reads, so they start the next
+    // segment at zero too.
+    for (int e = threadIdx.x; e < SEG * row_len; e += kBwdBlock) acc[e] = 0ull;
+    // The flush walks the entries e = threadIdx.x + m kBwdBlock as (row, column) = (e / n_x, e % n_x), advanced by
+    // (kBwdBlock / n_x, kBwdBlock % n_x) each time instead of dividing.
+    const int adv_rows = kBwdBlock / s.n_x, adv_cols = kBwdBlock - adv_rows * s.n_x;
+    const int first_row = static_cast<int>(threadIdx.x) / s.n_x;
+    const int first_col = static_cast<int>(threadIdx.x) - first_row * s.n_x;
+    long long b = blk * kBatches / G;
+    while (b > 0 && b * G / kBatches > blk) --b;
+    while (b + 1 < kBatches && (b + 1) * G / kBatches <= blk) ++b;
+    unsigned long long* const tb = batch + b * static_cast<long long>(N) * s.n_steps * s.n_x;
+    __syncthreads();
     for (int seg = n_seg - 1; seg >= 0; --seg) {
         const int s0 = seg * SEG, len = min(SEG, s.n_steps - s0);
-        for (int e = threadIdx.x; e < SEG * row_len; e += kBwdBlock) acc[e] = 0ull;
-        __syncthreads();
         for (long long chunk = c0; chunk < c1; ++chunk) {
             const long long i = chunk * kBwdBlock + threadIdx.x;
             const bool active = i < s.n_legs;
@@ -336,7 +351,10 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
                 v[a] = VAR ? s.ckv[(static_cast<long long>(seg) * N + a) * s.n_legs + leg] : R(1);
                 xb[a] = seg == n_seg - 1 ? R(0) : (xl_legs ? xl[a * xl_legs + local] : xbar[a * s.n_legs + leg]);
             }
-            NormalStream<R> rng, aux;
+            // the pair's two legs are lanes 2p and 2p + 1 (chunks start at multiples of the workgroup size), and
+            // every lane draws, active or not (an inactive pair draws pair 0's normals on both of its lanes): the
+            // two lanes of a pair share each refill's Philox and Box-Muller work
+            PairNormalStream<R> rng, aux;
             rng.init(static_cast<std::uint64_t>(s.first_pair + (leg >> 1)), s.key0, s.key1, 0u);
             // a segment's normals start at a Philox block: SEG n of stream 0 a segment (2 SEG n when the variance
             // normals come from stream 0 too), SEG n of the aux stream
@@ -431,7 +449,16 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
                             r1 = rr[1];  // i0 + 1 <= n_x - 1: the lerp keeps u below n_x - 1
                         }
                         unsigned long long* r = acc + (k * N + a) * s.n_x + i0;
-                        const long long q0 = to_fixed(vbar * (R(1) - f) * r0), q1 = to_fixed(vbar * f * r1);
+                        // the two entries' shares: with no ratio the left one is the whole weight's integer less
+                        // the right one's (one conversion fewer; the sum of the pair is then exactly the whole's)
+                        long long q0, q1;
+                        if (VAR && s.ratio[a]) {
+                            q0 = to_fixed(vbar * (R(1) - f) * r0);
+                            q1 = to_fixed(vbar * f * r1);
+                        } else {
+                            q1 = to_fixed(vbar * f);
+                            q0 = to_fixed(vbar) - q1;
+                        }
                         if (q0) atomicAdd(r, static_cast<unsigned long long>(q0));
                         if (q1) atomicAdd(r + 1, static_cast<unsigned long long>(q1));
                     }
@@ -452,16 +479,21 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
         // the segment's rows into this workgroup's batch table (batch b = the workgroups [b G / 16, (b + 1) G / 16)):
         // integer atomics, so the batch's table does not depend on the order of its workgroups' additions
         {
-            long long b = blk * kBatches / G;
-            while (b > 0 && b * G / kBatches > blk) --b;
-            while (b + 1 < kBatches && (b + 1) * G / kBatches <= blk) ++b;
-            unsigned long long* tb = batch + b * static_cast<long long>(N) * s.n_steps * s.n_x;
+            int row = first_row, col = first_col;
             for (int e = threadIdx.x; e < len * row_len; e += kBwdBlock) {
                 const unsigned long long v = acc[e];
-                if (!v) continue;
-                const int k = e / row_len, rest = e - k * row_len, a = rest / s.n_x, xx = rest - a * s.n_x;
-                __hip_atomic_fetch_add(tb + (static_cast<long long>(a) * s.n_steps + s0 + k) * s.n_x + xx, v,
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (v) {
+                    acc[e] = 0ull;
+                    const int k = row / N, a = row - k * N;
+                    __hip_atomic_fetch_add(tb + (static_cast<long long>(a) * s.n_steps + s0 + k) * s.n_x + col, v,
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                row += adv_rows;
+                col += adv_cols;
+                if (col >= s.n_x) {
+                    col -= s.n_x;
+                    ++row;
+                }
             }
         }
         __syncthreads();

@@ -132,6 +132,71 @@ struct NormalStream {
     }
 };
 
+// Normals of one pair on one stream drawn by the pair's two lanes together, for kernels that carry one leg a lane
+// (the legs of pair p on lanes 2p and 2p + 1 of a wave; both lanes must be active whenever one draws). Each refill
+// evaluates Philox and Box-Muller once a lane, for two consecutive blocks -- block b on the even lane, b + 1 on the odd
+// one -- and a quad permutation hands each lane the other's four normals: eight normals a refill for the work of
+// four, the same values as NormalStream block by block, since each block's arithmetic is the same wherever it runs.
+template <typename R>
+struct PairNormalStream {
+    std::uint32_t p0, p1, k0, k1, tag;
+    std::uint32_t block;
+    int used;
+    R buf[8];
+
+    __device__ __forceinline__ void init(std::uint64_t pair, std::uint32_t key0, std::uint32_t key1, std::uint32_t t) {
+        p0 = static_cast<std::uint32_t>(pair);
+        p1 = static_cast<std::uint32_t>(pair >> 32);
+        k0 = key0;
+        k1 = key1;
+        tag = t;
+        block = 0;
+        used = 8;
+    }
+    // value v of lane 2q + L of each quad pair, on both of its lanes (DPP quad_perm [L, L, 2 + L, 2 + L])
+    template <int L>
+    __device__ __forceinline__ static float from_lane(float v) {
+        constexpr int kPerm = L == 0 ? 0xA0 : 0xF5;
+        return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), kPerm, 0xF, 0xF,
+                                                                      false));
+    }
+    template <int L>
+    __device__ __forceinline__ static double from_lane(double v) {
+        constexpr int kPerm = L == 0 ? 0xA0 : 0xF5;
+        const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+        const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), kPerm, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), kPerm, 0xF, 0xF, false);
+        return __builtin_bit_cast(double, (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) |
+                                              static_cast<unsigned>(lo));
+    }
+    __device__ __forceinline__ R next() {
+        if (used == 8) {
+            const std::uint32_t odd = threadIdx.x & 1u;
+            // the keys made opaque here, so that the compiler forms the round keys at each refill (scalar adds)
+            // instead of holding all twenty in scalar registers across the kernel
+            std::uint32_t r0 = k0, r1 = k1;
+            asm volatile("" : "+s"(r0), "+s"(r1));
+            const U4 r = philox10(U4{p0, p1, block + odd, tag}, r0, r1);
+            R own[4];
+            gauss2(r.x, r.y, own[0], own[1]);
+            gauss2(r.z, r.w, own[2], own[3]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                buf[q] = from_lane<0>(own[q]);
+                buf[4 + q] = from_lane<1>(own[q]);
+            }
+            block += 2;
+            used = 0;
+        }
+        const int u = used++;
+        R v = buf[0];
+#pragma unroll
+        for (int q = 1; q < 8; ++q)
+            if (u == q) v = buf[q];
+        return v;
+    }
+};
+
 // ---- wave64 reductions -----------------------------------------------------------------------------------------
 
 __device__ __forceinline__ double shfl_xor_d(double v, int mask) { return __shfl_xor(v, mask, 64); }

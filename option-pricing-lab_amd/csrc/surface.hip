@@ -241,6 +241,8 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // The adjoint of entry (a, row, x) of batch b is tbar[((a n_rows + row) n_x + x) x_stride + b batch_stride]: batches
 // interleaved (x_stride = n_batches, batch_stride = 1) or one table after the other (x_stride = 1, batch_stride =
 // the entries of a table).
+constexpr int kContractUnroll = 8;
+
 template <typename Adj>
 __global__ void __launch_bounds__(64 * kContractWaves) node_contraction_kernel(SurfaceSet set, DerivSet dset,
                                                                              int n_active, const Adj* tbar,
@@ -257,20 +259,37 @@ __global__ void __launch_bounds__(64 * kContractWaves) node_contraction_kernel(S
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int m = lane & 15, kk = lane >> 4;  // A row (batch) / B column (node), and the entry within the step
     const bool batch_ok = m < n_batches;
-    const Adj* tb = tbar + m * batch_stride;
+    const Adj* tb = tbar + (batch_ok ? m : 0) * batch_stride;
+    const Adj* tb_row = tb + base * x_stride;
+    const int m0 = min(m, n_active - 1), m1 = min(m + 16, n_active - 1);
     // this wave's entries: steps of 4 split evenly over the waves
     const int steps = (n_x + 3) / 4, per = (steps + kContractWaves - 1) / kContractWaves;
     const int x_lo = 4 * per * wave, x_hi = min(n_x, 4 * per * (wave + 1));
     v4d c0 = {0.0, 0.0, 0.0, 0.0}, c1 = {0.0, 0.0, 0.0, 0.0};
-    for (int x0 = x_lo; x0 < x_hi; x0 += 4) {
-        const int x = x0 + kk;
-        const bool in = x < x_hi;
-        const double av = (in && batch_ok) ? static_cast<double>(tb[(base + x) * x_stride]) : 0.0;
-        const double* d = drow + static_cast<long long>(x) * n_active;
-        const double b0 = (in && m < n_active) ? d[m] : 0.0;
-        const double b1 = (in && m + 16 < n_active) ? d[m + 16] : 0.0;
-        c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b0, c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b1, c1, 0, 0, 0);
+    // kContractUnroll steps of 4 entries a pass: every load of the pass is issued before its first product, so a wave
+    // has that many loads in flight instead of one; the products still accumulate step after step, in entry order
+    for (int x0 = x_lo; x0 < x_hi; x0 += 4 * kContractUnroll) {
+        // the loads read clamped, always valid addresses with no branch around them (a branch would make the
+        // compiler wait for each load inside it); the operands outside the row or the tiles are zeroed afterwards
+        Adj raw[kContractUnroll];
+        double d0[kContractUnroll], d1[kContractUnroll];
+#pragma unroll
+        for (int u = 0; u < kContractUnroll; ++u) {
+            const int x = min(x0 + 4 * u + kk, x_hi - 1);
+            raw[u] = tb_row[static_cast<long long>(x) * x_stride];
+            const double* d = drow + static_cast<long long>(x) * n_active;
+            d0[u] = d[m0];
+            d1[u] = d[m1];
+        }
+#pragma unroll
+        for (int u = 0; u < kContractUnroll; ++u) {
+            const bool in = x0 + 4 * u + kk < x_hi;
+            const double av = (in && batch_ok) ? static_cast<double>(raw[u]) : 0.0;
+            const double b0 = (in && m < n_active) ? d0[u] : 0.0;
+            const double b1 = (in && m + 16 < n_active) ? d1[u] : 0.0;
+            c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b0, c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b1, c1, 0, 0, 0);
+        }
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) part[wave][0][k][lane] = c0[k], part[wave][1][k][lane] = c1[k];
