@@ -13,7 +13,8 @@
 //             launch reads the same rows at the same step); with Greeks on, the state every SEG steps goes to a
 //             checkpoint array (structure of arrays, coalesced) and the payoff's adjoint seeds are stored per leg.
 //   backward  persistent workgroups; segments of SEG steps from the last to the first; within a segment each
-//             workgroup recomputes its legs' SEG steps from the checkpoint, walks them backwards and adds the
+//             workgroup recomputes its legs' SEG steps from the checkpoint (the two lanes of a pair sharing each
+//             Philox and Box-Muller evaluation: PairNormalStream), walks them backwards and adds the
 //             adjoint of every table entry it read into an LDS copy of the segment's rows as 64-bit fixed point
 //             (integer adds: the result does not depend on the order, so the Greeks are bit-reproducible); the
 //             segment's rows are then added into the table of the workgroup's batch (16 batches of consecutive
@@ -281,7 +282,16 @@ __global__ void batch_sums_kernel(const double* partials, long long n_blocks, do
     const int b = threadIdx.x / 16, k = threadIdx.x % 16;
     const long long b0 = b * n_blocks / kBatches, b1 = (b + 1) * n_blocks / kBatches;
     double t = 0.0;
-    for (long long blk = b0; blk < b1; ++blk) t += partials[blk * 16 + k];
+    long long blk = b0;
+    // eight loads in flight a pass, added in block order
+    for (; blk + 8 <= b1; blk += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = partials[(blk + u) * 16 + k];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t += v[u];
+    }
+    for (; blk < b1; ++blk) t += partials[blk * 16 + k];
     batches[threadIdx.x] = t;
 }
 

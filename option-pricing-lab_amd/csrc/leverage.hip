@@ -648,7 +648,8 @@ __global__ void __launch_bounds__(kPersistThreads) slv_persistent_kernel(LevArgs
     __shared__ long long sums[5 * kMaxBins];  // phase C: the regression bins' fixed-point sums
     __shared__ double xev[kPersistMaxX];
     __shared__ R lev[kPersistMaxX];
-    __shared__ double red[3][kPersistThreads / 64];
+    __shared__ double red[3][kPersistThreads / 64];   // the workgroup's range
+    __shared__ double red2[3][kPersistThreads / 64];  // the launch's range
     __shared__ double scan[4 * (kPersistThreads / 64)];
     __shared__ double lvv[2][kPersistMaxX];  // the two table entries around each grid point (or its one entry)
     __shared__ double qval[2];
@@ -854,6 +855,12 @@ __global__ void __launch_bounds__(kPersistThreads) slv_persistent_kernel(LevArgs
             }
         }
         __syncthreads();
+        // the LDS bins of phase C cleared now (fine doubles as the bins' moments, read for the last time above); the
+        // waits below order these stores before the binning
+        if (i + 1 < s.n_steps) {
+            for (int k = tid; k < nf; k += kPersistThreads) fine[k] = 0u;
+            for (int k = tid; k < 5 * nb; k += kPersistThreads) sums[k] = 0;
+        }
         // ---- B: one Euler step of both legs of the workgroup's pairs (as step_kernel)
         const R a = static_cast<R>(ga), b = static_cast<R>(gb);
         const R hr = static_cast<R>(h);
@@ -925,13 +932,12 @@ __global__ void __launch_bounds__(kPersistThreads) slv_persistent_kernel(LevArgs
             mx = fmax(mx, __shfl_xor(mx, m, 64));
             mv = fmax(mv, __shfl_xor(mv, m, 64));
         }
-        __syncthreads();  // red[] was read by thread 0 above
-        if (lane == 0) red[0][wave] = mn, red[1][wave] = mx, red[2][wave] = mv;
+        if (lane == 0) red2[0][wave] = mn, red2[1][wave] = mx, red2[2][wave] = mv;
         __syncthreads();
         for (int w = 0; w < n_waves; ++w) {
-            mn = fmin(mn, red[0][w]);
-            mx = fmax(mx, red[1][w]);
-            mv = fmax(mv, red[2][w]);
+            mn = fmin(mn, red2[0][w]);
+            mx = fmax(mx, red2[1][w]);
+            mv = fmax(mv, red2[2][w]);
         }
         lo = mn, hi = mx;
         // ---- C: bin the particles of step i + 1 (as bin_kernel) and add the workgroup's bins to the totals
@@ -939,9 +945,6 @@ __global__ void __launch_bounds__(kPersistThreads) slv_persistent_kernel(LevArgs
         if (mv > 0) frexp(mv, &ev);
         shift_d = 61 - bn;
         shift_v = min(52, 61 - bn - max(ev, -8));
-        for (int k = tid; k < nf; k += kPersistThreads) fine[k] = 0u;
-        for (int k = tid; k < 5 * nb; k += kPersistThreads) sums[k] = 0;
-        __syncthreads();
         const double sd = ldexp(1.0, shift_d), sv = ldexp(1.0, shift_v);
         const R rlo = static_cast<R>(lo), rhi = static_cast<R>(hi);
         const R wf = (rhi - rlo) / static_cast<R>(nf);
