@@ -19,6 +19,11 @@ namespace optlab_mi {
 
 namespace {
 
+// A close from its log-forward-moneyness: FP32 by the hardware exp2 (exp(x) = 2^(x log2 e), about 2 ulp; the library
+// expf adds a range reduction and clamps that a close never needs, 11 instructions against 2); FP64 by exp.
+__device__ __forceinline__ float close_exp(float x) { return hw_exp(x); }
+__device__ __forceinline__ double close_exp(double x) { return exp(x); }
+
 constexpr int kBlock = 256;
 constexpr int kMaxNames = 8;
 
@@ -100,7 +105,7 @@ __device__ __forceinline__ void record(const PathArgs<R>& s, int n, const long l
         const R f = fw[a];
         R S[2];
 #pragma unroll
-        for (int l = 0; l < 2; ++l) S[l] = f * r_exp(x[l][a]);
+        for (int l = 0; l < 2; ++l) S[l] = f * close_exp(x[l][a]);
         double* base = s.out + static_cast<long long>(a) * s.n_cols * s.rows;
         if (c >= 0 && active) {
 #pragma unroll
@@ -171,7 +176,7 @@ __device__ __forceinline__ void exercise(const PathArgs<R>& s, int j, const R (&
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
         if (e.done[l]) continue;
-        const double sx = static_cast<double>(f * r_exp(x[l][0])) * r->inv_strike;
+        const double sx = static_cast<double>(f * close_exp(x[l][0])) * r->inv_strike;
         const double ex = fmax(r->phi * (sx - 1.0), 0.0);
         if (j == last) {
             e.cash[l] = ex;
@@ -280,7 +285,7 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
     R wst[2] = {s.mix[0], s.mix[0]};
 
     R dt_next = s.n_steps > 0 ? ldc(s.dt, 0) : R(0);
-    for (int i = 1; i <= s.n_steps; ++i) {
+    auto one_step = [&](int i) __attribute__((always_inline)) {
         const R dt = dt_next, t0 = mixer ? ldc(s.t0, i - 1) : R(0);
         if (i < s.n_steps) dt_next = ldc(s.dt, i);
         R zs[N], zr[N], fresh = R(0);
@@ -408,7 +413,21 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
             observe(j++);
             load_obs(j);
         }
+    };
+    // Constant-variance and table forms of up to four names draw exactly N normals a step from stream 0, so four
+    // steps draw N whole Philox blocks: unrolled by four, the stream's buffer position is a compile-time constant
+    // at every draw (no select among the buffered normals, no refill test).
+    constexpr bool kWholeBlocks = (CV || TAB) && N <= 4;
+    int i = 1;
+    if constexpr (kWholeBlocks) {
+        for (; i + 3 <= s.n_steps; i += 4) {
+            one_step(i);
+            one_step(i + 1);
+            one_step(i + 2);
+            one_step(i + 3);
+        }
     }
+    for (; i <= s.n_steps; ++i) one_step(i);
     if constexpr (EXER) {  // products.py:150 and :30-35: K cash D(t_tau) of each leg, the pair's average
         const ExerciseConst r = (ExerciseConst)s.ex;
         double d = 0.0;
