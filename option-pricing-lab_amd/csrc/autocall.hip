@@ -147,7 +147,7 @@ __global__ void __launch_bounds__(kFwdBlock) autocall_forward(AcArgs<R> s) {
     Note<R> note{R(1), R(0), R(0)};
     int j = 0, next = ldc(s.obs_step, 0);
 
-    for (int step = 0; step < s.n_steps; ++step) {
+    auto one_step = [&](int step) __attribute__((always_inline)) {
         if (s.greeks && step % SEG == 0 && active) {
             const long long c = step / SEG;
 #pragma unroll
@@ -187,7 +187,19 @@ __global__ void __launch_bounds__(kFwdBlock) autocall_forward(AcArgs<R> s) {
             ++j;
             next = j < s.n_obs ? ldc(s.obs_step, j) : -1;
         }
+    };
+    // Without a variance state a step draws exactly N normals from stream 0, so four steps draw N whole Philox
+    // blocks: unrolled by four, the stream's buffer position is a compile-time constant at every draw.
+    int step = 0;
+    if constexpr (!VAR) {
+        for (; step + 4 <= s.n_steps; step += 4) {
+            one_step(step);
+            one_step(step + 1);
+            one_step(step + 2);
+            one_step(step + 3);
+        }
     }
+    for (; step < s.n_steps; ++step) one_step(step);
 
     // Reverse sweep of the script for dV / dperf_j and dV / dlow (the state before date j recomputed from the
     // stored performances: n_obs^2 / 2 cheap steps instead of n_obs saved states).

@@ -142,7 +142,7 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
     __syncthreads();  // the previous pass has read its last rows
     stage_rows<R, NM>(a, base, 0, rows);
     int to_fix = obs_every;  // steps to the next fixing (a countdown: a runtime modulo costs a division a step)
-    for (int step = 0; step < n_steps; ++step) {
+    auto one_step = [&](int step) __attribute__((always_inline)) {
         const R* cur = rows + (step & 1) * stage_len;
         __syncthreads();  // this step's rows are in; everyone is done with the last step's buffer
         if (step + 1 < n_steps) stage_rows<R, NM>(a, base, step + 1, rows + ((step + 1) & 1) * stage_len);
@@ -194,7 +194,17 @@ __device__ double pair_value(const BookArgs<R>& a, const TradeDev<R>& t, const S
             }
         }
         if (product == 3 && fix) ++obs_j;
+    };
+    // a step draws exactly NM normals, so four steps draw NM whole Philox blocks: unrolled by four, the stream's
+    // buffer position is a compile-time constant at every draw
+    int step = 0;
+    for (; step + 4 <= n_steps; step += 4) {
+        one_step(step);
+        one_step(step + 1);
+        one_step(step + 2);
+        one_step(step + 3);
     }
+    for (; step < n_steps; ++step) one_step(step);
     const R D = ldc(a.disc, n_steps);
     const R strike = ldc(&t.strike, 0);
     double v = 0.0;
