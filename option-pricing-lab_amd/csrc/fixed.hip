@@ -250,7 +250,7 @@ __device__ __forceinline__ void fixed_unit(const Consts<R>& c, NormalStream<R>& 
     } else {
         R y[2] = {R(0), R(0)}, v[2] = {c.v0, c.v0}, sum_s[2] = {R(0), R(0)}, sum_y[2] = {R(0), R(0)};
         bool alive[2] = {true, true};
-        for (int i = 0; i < c.n_steps; ++i) {
+        auto one_step = [&]() __attribute__((always_inline)) {
             R zv = R(0), zx = R(0), z = R(0);
             if (MODEL == 1) {
                 zv = src.next();
@@ -274,7 +274,18 @@ __device__ __forceinline__ void fixed_unit(const Consts<R>& c, NormalStream<R>& 
                 if (PRODUCT == 2) alive[a] = alive[a] && !(y[a] <= c.ln_h);
                 if (PRODUCT == 3) alive[a] = alive[a] && !(y[a] >= c.ln_h);
             }
+        };
+        // Black-Scholes: the four steps a Philox block serves run together, so the stream's buffer position is a
+        // compile-time constant at every draw (measured: the down-and-out 1.12 -> 0.96 ms, the same price; two Heston
+        // steps a pass moved its times by -3.5 % and +0.7 % and, contracted differently, its FP32 bits)
+        int i = 0;
+        if constexpr (MODEL == 0) {
+            for (; i + 4 <= c.n_steps; i += 4) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) one_step();
+            }
         }
+        for (; i < c.n_steps; ++i) one_step();
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
             const R S = c.S0 * r_exp(y[a]);
