@@ -426,6 +426,15 @@ __global__ void __launch_bounds__(kBlock) paths_kernel(PathArgs<R> s) {
             one_step(i + 2);
             one_step(i + 3);
         }
+    } else if constexpr (!mixer && N <= 2) {
+        // the generic form of one or two names whose variance normals come from stream 0 (no SLV name) draws 2 N
+        // normals a step from it: two steps are N whole blocks, and under this test the draws' source is known
+        if (has_var && !s.has_slv) {
+            for (; i + 1 <= s.n_steps; i += 2) {
+                one_step(i);
+                one_step(i + 1);
+            }
+        }
     }
     for (; i <= s.n_steps; ++i) one_step(i);
     if constexpr (EXER) {  // products.py:150 and :30-35: K cash D(t_tau) of each leg, the pair's average
