@@ -59,8 +59,8 @@ struct AcArgs {
     R dt;
     const R* root;                 // n x n
     const R* table[kMaxNames];     // per name: n_steps x n_x
-    const double* lo[kMaxNames];   // per name: n_steps
-    const double* inv[kMaxNames];  // per name: n_steps
+    const R* lo[kMaxNames];        // per name: n_steps, each row's first abscissa in the path precision
+    const R* inv[kMaxNames];       // per name: n_steps, each row's 1 / dx in the path precision
     const R* lf;                   // n x (n_steps + 1): ln(F_a(s dt) / fixing_a)
     const int* obs_step;           // n_obs
     const int* obs_of;             // n_steps + 1: the observation at that step, or -1
@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(kFwdBlock) autocall_forward(AcArgs<R> s) {
         for (int a = 0; a < N; ++a) {
             const R* y = s.table[a] + static_cast<long long>(step) * s.n_x;
             const Lerp<R> l =
-                lerp_at(y, static_cast<R>(ldc(s.lo[a], step)), static_cast<R>(ldc(s.inv[a], step)), s.n_x, x[a]);
+                lerp_at(y, ldc(s.lo[a], step), ldc(s.inv[a], step), s.n_x, x[a]);
             R vp;
             const R var = step_variance<R, VAR>(s, a, l.var, vs[a], z[a], VAR ? wz[a] : R(0), vp);
             x[a] += R(-0.5) * var * s.dt + r_sqrt(var * s.dt) * z[a];
@@ -404,7 +404,7 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
 #pragma unroll
                 for (int a = 0; a < N; ++a) {
                     const R* y = s.table[a] + static_cast<long long>(step) * s.n_x;
-                    const Lerp<R> l = lerp_at(y, static_cast<R>(ldc(s.lo[a], step)), static_cast<R>(ldc(s.inv[a], step)),
+                    const Lerp<R> l = lerp_at(y, ldc(s.lo[a], step), ldc(s.inv[a], step),
                                               s.n_x, x[a]);
                     if constexpr (!kKeep) {
                         xs[k][a] = x[a];
@@ -452,8 +452,8 @@ __global__ void __launch_bounds__(bwd_block<R>()) autocall_backward(AcArgs<R> s,
                     } else {
                         const int step = s0 + k;
                         const R* y = s.table[a] + static_cast<long long>(step) * s.n_x;
-                        const Lerp<R> l = lerp_at(y, static_cast<R>(ldc(s.lo[a], step)),
-                                                  static_cast<R>(ldc(s.inv[a], step)), s.n_x, xs[k][a]);
+                        const Lerp<R> l = lerp_at(y, ldc(s.lo[a], step),
+                                                  ldc(s.inv[a], step), s.n_x, xs[k][a]);
                         const R vp = VAR ? vv[VAR ? k : 0][a] : R(1);
                         const R sd = r_sqrt(l.var * vp * s.dt);
                         u = l.u, slope = l.slope;
@@ -533,6 +533,17 @@ __global__ void slv_tables_kernel(const double* lev2, const double* lv_base, con
     const double r = lev2[i] / lv_base[i];
     if (ratio) ratio[i] = static_cast<R>(r);
     table[i] = lv_moved ? static_cast<R>(r * static_cast<double>(lv_moved[i])) : static_cast<R>(lev2[i]);
+}
+
+// A name's row ends (first abscissa, 1 / dx) in the path precision: converted once a call, so that the step loops
+// read them in the precision they use (the same conversion the loops made at every read).
+template <typename R>
+__global__ void row_ends_kernel(const double* lo, const double* inv, int n, R* out) {
+    const int i = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) {
+        out[i] = static_cast<R>(lo[i]);
+        out[n + i] = static_cast<R>(inv[i]);
+    }
 }
 
 template <typename R>
@@ -692,7 +703,8 @@ void run_typed(optlab_autocall_request& rq, const optlab_autocall_models* models
     Arena ar;
     const size_t o_root = ar.reals<R>(root), o_lf = ar.reals<R>(lf), o_disc = ar.reals<R>(disc);
     const size_t o_obs = ar.put(rq.obs_step, rq.n_obs * sizeof(int)), o_of = ar.put(obs_of.data(), obs_of.size() * 4);
-    ar.put(nullptr, 0);
+    const std::vector<R> ends_room(static_cast<size_t>(2) * n * ns, R(0));  // filled on the device below
+    const size_t o_ends = ar.put(ends_room.data(), ends_room.size() * sizeof(R));
     unsigned char* dev = static_cast<unsigned char*>(scratch(kSlotSpec, std::max<size_t>(ar.host.size(), 256)));
     hip_check(hipMemcpyAsync(dev, ar.host.data(), ar.host.size(), hipMemcpyHostToDevice, stream), "copy spec");
 
@@ -703,9 +715,13 @@ void run_typed(optlab_autocall_request& rq, const optlab_autocall_models* models
     s.dt = static_cast<R>(rq.dt);
     s.root = reinterpret_cast<const R*>(dev + o_root);
     for (int a = 0; a < n; ++a) {
+        R* ends = reinterpret_cast<R*>(dev + o_ends) + static_cast<size_t>(2) * a * ns;
+        hipLaunchKernelGGL(row_ends_kernel<R>, dim3((ns + 255) / 256), dim3(256), 0, stream, rows[a]->lo,
+                           rows[a]->inv_dx, ns, ends);
+        hip_check(hipGetLastError(), "row ends launch");
         s.table[a] = tables[a];
-        s.lo[a] = rows[a]->lo;
-        s.inv[a] = rows[a]->inv_dx;
+        s.lo[a] = ends;
+        s.inv[a] = ends + ns;
         s.spot[a] = static_cast<R>(spot[a]);
     }
     s.lf = reinterpret_cast<const R*>(dev + o_lf);
