@@ -675,7 +675,14 @@ __device__ __forceinline__ void strip_greeks_body(const StripArgs<R>& s, long lo
         R ty0 = R(0), ty1 = R(0), tv0 = R(1), tv1 = R(1);  // d/dv0 (Heston) or d/dsigma (Black-Scholes)
         int e = 0, next_event = s.mat_step[0];
         R z[4] = {R(0), R(0), R(0), R(0)};
-        for (int i = 1; i <= n; ++i) {
+        // FP32: a Philox block's steps a pass, unrolled, so each step's place in its block (and its normals) is a
+        // compile-time constant; FP64 keeps one step a pass (unrolled it measured 13.5 -> 17.3 ms)
+        constexpr int kUnroll = std::is_same<R, float>::value ? per_block : 1;
+        for (int i0 = 0; i0 < n; i0 += kUnroll) {
+#pragma unroll
+        for (int jj = 0; jj < kUnroll; ++jj) {
+            const int i = i0 + jj + 1;
+            if (i > n) break;
             const int j = (i - 1) & (per_block - 1);
             if (j == 0) {
                 const U4 r = philox10(U4{p0, p1, static_cast<std::uint32_t>((i - 1) / per_block), 0u}, k0, k1);
@@ -701,6 +708,7 @@ __device__ __forceinline__ void strip_greeks_body(const StripArgs<R>& s, long lo
                 ++e;
                 next_event = e < n_mats ? s.mat_step[e] : -1;
             }
+        }
         }
     }
     // the block's waves in a fixed order
